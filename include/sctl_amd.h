@@ -1,0 +1,161 @@
+/*
+ * sctl_amd.h -- C-ABI of the MI355X (gfx950) direct kernel summation.
+ *
+ * Drop-in device backend for SCTL's GenericKernel<uKernel>::Eval /
+ * KernelMatrix (include/sctl/generic-kernel.hpp:99,112,124) for the four
+ * micro-kernels Laplace3D_FxU, Laplace3D_DxU, Stokes3D_FxU, Stokes3D_DxU
+ * (include/sctl/kernel_functions.hpp:32-79,125-218), and for the batches of
+ * near-field blocks that BoundaryIntegralOp evaluates through them
+ * (include/sctl/boundary_integral.txx:1009-1032,1165-1240,1311-1381).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; every function returns an sctl_amd_status
+ *    and never aborts.  sctl_amd_last_error() gives the message of the last
+ *    failing call on the calling thread.
+ *  - Coordinates, normals and densities are array-of-structures doubles exactly
+ *    as in SCTL's Vector arguments: r[i*3+k], n[i*3+k], v[i*K0+k0],
+ *    v_trg[t*K1+k1].
+ *  - Array arguments are device pointers unless SCTL_AMD_HOST_PTRS is set in
+ *    `flags`; then they are host pointers, the library stages them and the
+ *    call returns after the result has been copied back.
+ *  - `stream` is a hipStream_t (NULL: the calling thread's per-thread default
+ *    stream).  Device-pointer calls are asynchronous on that stream.
+ *  - Thread-safe: calls from different host threads on different streams may
+ *    overlap.  Plans (sctl_amd_plan) are not shared between threads.
+ */
+#ifndef SCTL_AMD_H_
+#define SCTL_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SCTL_AMD_SUCCESS = 0,
+  SCTL_AMD_ERROR_INVALID_VALUE = 1,  /* bad size, pointer, flag, digits or layout */
+  SCTL_AMD_ERROR_UNKNOWN_KERNEL = 2, /* kernel id outside the four micro-kernels */
+  SCTL_AMD_ERROR_UNSUPPORTED = 3,    /* precision/option combination not provided */
+  SCTL_AMD_ERROR_NO_DEVICE = 4,      /* no HIP device visible */
+  SCTL_AMD_ERROR_OUT_OF_MEMORY = 5,  /* device allocation failed */
+  SCTL_AMD_ERROR_HIP = 6,            /* a HIP runtime call failed */
+  SCTL_AMD_ERROR_QUEUE = 7           /* batched work queue did not drain as planned */
+} sctl_amd_status;
+
+/* Micro-kernel ids (kernel_functions.hpp:669-684 aliases). */
+enum {
+  SCTL_AMD_LAPLACE3D_FXU = 0, /* K = 1/r,                scale 1/(4 pi), K0=K1=1, no normal */
+  SCTL_AMD_LAPLACE3D_DXU = 1, /* K = (r.n)/r^3,          scale 1/(4 pi), K0=K1=1, normal    */
+  SCTL_AMD_STOKES3D_FXU = 2,  /* K = I/r + r r^T/r^3,    scale 1/(8 pi), K0=K1=3, no normal */
+  SCTL_AMD_STOKES3D_DXU = 3   /* K = r r^T (r.n)/r^5,    scale 3/(4 pi), K0=K1=3, normal    */
+};
+
+/* Arithmetic of an evaluation. */
+enum {
+  SCTL_AMD_FP64 = 0,  /* FP64 throughout; digits selects the Newton steps after v_rsq_f64 */
+  SCTL_AMD_MIXED = 1, /* FP64 tile-centred differencing, FP32 pairwise, Kahan FP64 accumulation */
+  SCTL_AMD_AUTO = 2   /* MIXED when 0 <= digits <= 7, else FP64 */
+};
+
+/* flags */
+#define SCTL_AMD_HOST_PTRS 0x1u     /* array arguments are host memory; call is synchronous */
+#define SCTL_AMD_MIXED_NEWTON 0x2u  /* MIXED: one FP32 Newton step after v_rsq_f32 */
+
+/* ------------------------------------------------------------------ setup */
+
+/* Selects and initialises `device` for the calling thread (optional: every
+ * entry point initialises the current device on first use). */
+int sctl_amd_init(int device);
+
+/* Message of the last failing call on the calling thread ("" if none). */
+const char* sctl_amd_last_error(void);
+
+/* Library version string. */
+const char* sctl_amd_version(void);
+
+/* Kernel metadata: CoordDim/SrcDim/TrgDim/NormalDim (generic-kernel.hpp:48-62),
+ * the reference's FLOPS() Profile count and uKerScaleFactor<double>()
+ * (kernel_functions.hpp:37-41,66-70,130-134,185-189), and Name().
+ * Any output pointer may be NULL. */
+int sctl_amd_kernel_info(int kernel, int* coord_dim, int* src_dim, int* trg_dim, int* normal_dim, int* flops,
+                         double* scale, const char** name);
+
+/* ------------------------------------------------------------------ Eval
+ * Replaces GenericKernel<uKernel>::Eval<double,omp,digits>
+ * (generic-kernel.hpp:112, generic-kernel.txx:102-211) and the type-erased
+ * static Eval(..., digits, self) that ParticleFMM stores as a function pointer
+ * (generic-kernel.hpp:99, generic-kernel.txx:72-100, fmm-wrapper.txx:153-154,618).
+ *
+ *   v_trg[t*K1+k1] += s * sum_{s} sum_{k0} K(r_trg[t]-r_src[s], n_src[s])[k0][k1] * v_src[s*K0+k0]
+ *
+ * v_trg holds nt*K1 values and is accumulated into (the caller resizes and
+ * zeroes it first, as generic-kernel.txx:129-132 does on a size mismatch).
+ * n_src may be NULL iff the kernel has no normal.  ns == 0 or nt == 0 leaves
+ * v_trg unchanged.  digits: -1 (machine precision) or 0..15; >= 16 means -1.
+ * Coincident points (r == 0) contribute exactly zero. */
+int sctl_amd_eval(int kernel, int precision, int digits, const double* r_trg, int64_t nt, const double* r_src,
+                  const double* n_src, const double* v_src, int64_t ns, double* v_trg, uint32_t flags, void* stream);
+
+/* ------------------------------------------------------------ KernelMatrix
+ * Replaces GenericKernel<uKernel>::KernelMatrix<double,omp,digits>
+ * (generic-kernel.hpp:124, generic-kernel.txx:213-329):
+ *
+ *   M[(s*K0+k0)*(nt*K1) + t*K1+k1] = s * K(r_trg[t]-r_src[s], n_src[s])[k0][k1]
+ *
+ * M is row-major (ns*K0) x (nt*K1) and is overwritten. FP64. */
+int sctl_amd_kernel_matrix(int kernel, int digits, const double* r_trg, int64_t nt, const double* r_src,
+                           const double* n_src, int64_t ns, double* M, uint32_t flags, void* stream);
+
+/* ------------------------------------------------------------ batched blocks
+ * A plan holds nblocks (target-set, source-set) blocks over one target array
+ * and one source array -- the near-field blocks (elem, t0, cnt) that
+ * BoundaryIntegralOp builds (boundary_integral.txx:1009-1032) and evaluates
+ * block by block (:1169-1238 KernelMatrix, :1342-1364 matrix-free), and that
+ * ParticleFMM::EvalDirect sums directly (fmm-wrapper.txx:551-623).
+ *
+ * Block b covers targets [trg_dsp[b], trg_dsp[b]+trg_cnt[b]) and sources
+ * [src_dsp[b], src_dsp[b]+src_cnt[b]).  Target ranges of different blocks must
+ * not overlap.  sctl_amd_plan_apply computes, for every block, the same
+ * accumulation as sctl_amd_eval restricted to that block, in one persistent
+ * launch fed by a work queue.
+ *
+ * The offset/count arrays are host memory, copied at creation. */
+typedef struct sctl_amd_plan_s* sctl_amd_plan;
+
+int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int digits, int64_t nblocks,
+                         const int64_t* trg_dsp, const int64_t* trg_cnt, const int64_t* src_dsp,
+                         const int64_t* src_cnt, int64_t nt_total, int64_t ns_total);
+
+/* Uploads (HOST_PTRS) or copies (device pointers) the geometry: r_trg
+ * nt_total x 3, r_src ns_total x 3, n_src ns_total x 3 (NULL iff no normal).
+ * The plan keeps its own device copy, so the caller's buffers may be reused. */
+int sctl_amd_plan_set_geometry(sctl_amd_plan plan, const double* r_trg, const double* r_src, const double* n_src,
+                               uint32_t flags, void* stream);
+
+/* v_trg (nt_total*K1) += block sums for densities v_src (ns_total*K0).
+ * Allocates nothing; may be captured into a hipGraph. */
+int sctl_amd_plan_apply(sctl_amd_plan plan, const double* v_src, double* v_trg, uint32_t flags, void* stream);
+
+/* Number of pairwise interactions one apply evaluates (sum of Ns*Nt). */
+int sctl_amd_plan_interactions(sctl_amd_plan plan, int64_t* interactions);
+
+int sctl_amd_plan_destroy(sctl_amd_plan plan);
+
+/* -------------------------------------------------------- device plumbing
+ * Minimal device-memory and stream helpers for hosts that have no HIP
+ * allocator of their own (bindings, tests).  kind: 0 host->device,
+ * 1 device->host, 2 device->device. */
+int sctl_amd_malloc(void** ptr, size_t bytes);
+int sctl_amd_free(void* ptr);
+int sctl_amd_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream);
+int sctl_amd_stream_create(void** stream);
+int sctl_amd_stream_destroy(void* stream);
+int sctl_amd_stream_synchronize(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCTL_AMD_H_ */

@@ -1,0 +1,385 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * may load this library, and only as the checker.  The product (libsctl_amd)
+ * never links, loads or calls it.
+ *
+ * A CPU restatement of SCTL's direct kernel summation for the four
+ * micro-kernels Laplace3D_FxU, Laplace3D_DxU, Stokes3D_FxU, Stokes3D_DxU:
+ *
+ *   GenericKernel::Eval           generic-kernel.txx:102-211
+ *       v_trg[t*K1+k1] += s * sum_s sum_k0 K(x_t - x_s, n_s)[k0][k1] * v_src[s*K0+k0]
+ *       sources summed in index order per target, scale s applied once per
+ *       target at the end (:154,206), v_trg accumulated into (:129-132).
+ *   GenericKernel::KernelMatrix   generic-kernel.txx:213-329
+ *       M[s*K0+k0][t*K1+k1] = s * K(x_t - x_s, n_s)[k0][k1]   (overwrites M)
+ *   micro-kernels                 kernel_functions.hpp:32-79 (Laplace),
+ *                                 :125-178 (Stokes FxU, fused apply :165-176),
+ *                                 :180-218 (Stokes DxU, fused apply :204-217)
+ *   masked rsqrt                  vec.txx:379-382, intrin-wrapper.hpp:540-579,3383-3408
+ *
+ * Three arithmetic modes:
+ *   ORACLE_EXACT (0)  the reference's scalar build (DefaultVecLen()==1,
+ *                     vec.txx:27-41): rsqrt is exactly x==0 ? 0 : 1/sqrt(x)
+ *                     (intrin-wrapper.hpp:563-579) and FMA is a separate
+ *                     multiply and add (intrin-wrapper.hpp:269-271).  Build
+ *                     with -ffp-contract=off so every operation rounds where
+ *                     the reference's does.
+ *   ORACLE_AVX512 (1) the reference's AVX-512 build: a 14-bit hardware
+ *                     estimate (_mm512_maskz_rsqrt14_pd) followed by
+ *                     mylog2(digits/4.2144199393) deferred-scaling Newton
+ *                     steps (intrin-wrapper.hpp:540-562,3401-3407), FMA fused.
+ *   ORACLE_LDOUBLE (2) brute force in long double with an exact 1/sqrtl; the
+ *                     mathematical reference, independent of SCTL's rounding.
+ *
+ * Parity pins for every function here are listed in the test suite
+ * (tests/test_oracle.py): golden vectors from the compiled reference,
+ * closed forms, the long-double brute force, surface identities,
+ * Eval == KernelMatrix * v, and the rsqrt error bounds of vec-test.hpp:431-465.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__x86_64__)
+#include <immintrin.h>
+#endif
+
+enum { LAP_FXU = 0, LAP_DXU = 1, STK_FXU = 2, STK_DXU = 3, NKER = 4 };
+enum { ORACLE_EXACT = 0, ORACLE_AVX512 = 1, ORACLE_LDOUBLE = 2 };
+
+static const int KD0[NKER] = {1, 1, 3, 3};
+static const int KD1[NKER] = {1, 1, 3, 3};
+static const int NDIM[NKER] = {0, 3, 0, 3};
+static const int FLOPS[NKER] = {6, 14, 23, 26}; /* kernel_functions.hpp:37,66,130,185 */
+
+/* const_pi<double>(), math_utils.hpp:43 */
+static double pi_d(void) {
+  return (double)14488038916154245684ull / (double)(1ull << 62) +
+         (double)7089564414062235240ull / (double)(1ull << 62) / (double)(1ull << 63);
+}
+static long double pi_ld(void) {
+  return (long double)14488038916154245684ull / (long double)(1ull << 62) +
+         (long double)7089564414062235240ull / (long double)(1ull << 62) / (long double)(1ull << 63);
+}
+
+/* uKerScaleFactor<double>(), kernel_functions.hpp:40,69,133,188 */
+double oracle_scale(int ker) {
+  switch (ker) {
+    case LAP_FXU: return 1 / (4 * pi_d());
+    case LAP_DXU: return 1 / (4 * pi_d());
+    case STK_FXU: return 1 / (8 * pi_d());
+    case STK_DXU: return 3 / (4 * pi_d());
+  }
+  return 0;
+}
+static long double scale_ld(int ker) {
+  switch (ker) {
+    case LAP_FXU: return 1 / (4 * pi_ld());
+    case LAP_DXU: return 1 / (4 * pi_ld());
+    case STK_FXU: return 1 / (8 * pi_ld());
+    case STK_DXU: return 3 / (4 * pi_ld());
+  }
+  return 0;
+}
+
+int oracle_dims(int ker, int* dims) {
+  if (ker < 0 || ker >= NKER) return -1;
+  dims[0] = 3; dims[1] = KD0[ker]; dims[2] = KD1[ker]; dims[3] = NDIM[ker]; dims[4] = FLOPS[ker];
+  return 0;
+}
+
+/* digits_ = digits==-1 ? floor(SigBits*log10(2)) : digits, generic-kernel.txx:103 */
+static int resolve_digits(int digits) { return digits == -1 ? (int)(52 * 0.3010299957) : digits; }
+/* mylog2, intrin-wrapper.hpp:560-562 */
+static int mylog2(int x) { return x < 1 ? 0 : 1 + mylog2(x / 2); }
+/* Newton count for the rsqrt14 estimate, intrin-wrapper.hpp:3399 */
+int oracle_newton_iters_avx512(int digits) { return mylog2((int)(resolve_digits(digits) / 4.2144199393)); }
+
+/* ---------------------------------------------------------------- exact mode
+ * Scalar arithmetic in the reference's operation order.  rinv() is the masked
+ * rsqrt of the scalar build: (r2 > 0) & (r2 == 0 ? 0 : 1/sqrt(r2)).
+ */
+static inline double rinv_exact(double r2) {
+  double y = (r2 == 0 ? 0 : 1 / sqrt(r2));
+  return (r2 > 0) ? y : 0.0;
+}
+
+/* One interaction of Eval for the scalar build: vt += K * vs (fused apply for
+ * the Stokes kernels, uKerMatrix + FMA for Laplace, generic-kernel.txx:107-121). */
+static inline void interact_exact(int ker, double* vt, const double* r, const double* n, const double* f) {
+  const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+  const double rinv = rinv_exact(r2);
+  switch (ker) {
+    case LAP_FXU: {
+      const double u = rinv;
+      vt[0] = u * f[0] + vt[0];
+    } break;
+    case LAP_DXU: {
+      const double rdotn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
+      const double rinv3 = rinv * rinv * rinv;
+      const double u = rdotn * rinv3;
+      vt[0] = u * f[0] + vt[0];
+    } break;
+    case STK_FXU: {
+      const double rinv3 = rinv * rinv * rinv;
+      const double fdotr_rinv3 = (f[0] * r[0] + f[1] * r[1] + f[2] * r[2]) * rinv3;
+      for (int i = 0; i < 3; i++) vt[i] += f[i] * rinv + r[i] * fdotr_rinv3;
+    } break;
+    case STK_DXU: {
+      const double rinv2 = rinv * rinv;
+      const double rinv5 = rinv2 * rinv2 * rinv;
+      const double rdotn_rinv5 = (r[0] * n[0] + r[1] * n[1] + r[2] * n[2]) * rinv5;
+      const double fdotr_rdotn_rinv5 = (f[0] * r[0] + f[1] * r[1] + f[2] * r[2]) * rdotn_rinv5;
+      for (int i = 0; i < 3; i++) vt[i] += r[i] * fdotr_rdotn_rinv5;
+    } break;
+  }
+}
+
+/* uKerMatrix for the scalar build: u[k0][k1] (not scaled). */
+static inline void matrix_exact(int ker, double u[3][3], const double* r, const double* n) {
+  const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+  const double rinv = rinv_exact(r2);
+  switch (ker) {
+    case LAP_FXU: u[0][0] = rinv; break;
+    case LAP_DXU: {
+      const double rdotn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
+      const double rinv3 = rinv * rinv * rinv;
+      u[0][0] = rdotn * rinv3;
+    } break;
+    case STK_FXU: {
+      const double rinv3 = rinv * rinv * rinv;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) u[i][j] = (i == j ? rinv : 0.0) + r[i] * r[j] * rinv3;
+    } break;
+    case STK_DXU: {
+      const double rinv2 = rinv * rinv;
+      const double rinv5 = rinv2 * rinv2 * rinv;
+      const double rdotn_rinv5 = (r[0] * n[0] + r[1] * n[1] + r[2] * n[2]) * rinv5;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) u[i][j] = r[i] * r[j] * rdotn_rinv5;
+    } break;
+  }
+}
+
+static void eval_exact(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
+                       const double* v_src, long ns, double* v_trg, int nthreads) {
+  const int K0 = KD0[ker], K1 = KD1[ker], ND = NDIM[ker];
+  const double sf = oracle_scale(ker);
+  const double zero3[3] = {0, 0, 0};
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (long t = 0; t < nt; t++) {
+    double vt[3] = {0, 0, 0};
+    for (long s = 0; s < ns; s++) {
+      const double r[3] = {r_trg[t * 3 + 0] - r_src[s * 3 + 0], r_trg[t * 3 + 1] - r_src[s * 3 + 1],
+                           r_trg[t * 3 + 2] - r_src[s * 3 + 2]};
+      interact_exact(ker, vt, r, ND ? n_src + s * 3 : zero3, v_src + s * K0);
+    }
+    for (int k = 0; k < K1; k++) v_trg[t * K1 + k] += vt[k] * sf;
+  }
+}
+
+/* ------------------------------------------------------------ long double */
+static void eval_ldouble(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
+                         const double* v_src, long ns, double* v_trg, int nthreads) {
+  const int K0 = KD0[ker], K1 = KD1[ker], ND = NDIM[ker];
+  const long double sf = scale_ld(ker);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (long t = 0; t < nt; t++) {
+    long double vt[3] = {0, 0, 0};
+    for (long s = 0; s < ns; s++) {
+      long double r[3], n[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+      for (int k = 0; k < 3; k++) r[k] = (long double)r_trg[t * 3 + k] - (long double)r_src[s * 3 + k];
+      if (ND) for (int k = 0; k < 3; k++) n[k] = n_src[s * 3 + k];
+      for (int k = 0; k < K0; k++) f[k] = v_src[s * K0 + k];
+      const long double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      if (!(r2 > 0)) continue;
+      const long double rinv = 1 / sqrtl(r2);
+      const long double rinv3 = rinv * rinv * rinv;
+      switch (ker) {
+        case LAP_FXU: vt[0] += f[0] * rinv; break;
+        case LAP_DXU: vt[0] += f[0] * (r[0] * n[0] + r[1] * n[1] + r[2] * n[2]) * rinv3; break;
+        case STK_FXU: {
+          const long double fr = f[0] * r[0] + f[1] * r[1] + f[2] * r[2];
+          for (int i = 0; i < 3; i++) vt[i] += f[i] * rinv + r[i] * fr * rinv3;
+        } break;
+        case STK_DXU: {
+          const long double fr = f[0] * r[0] + f[1] * r[1] + f[2] * r[2];
+          const long double rn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
+          for (int i = 0; i < 3; i++) vt[i] += r[i] * fr * rn * rinv3 * rinv * rinv;
+        } break;
+      }
+    }
+    for (int k = 0; k < K1; k++) v_trg[t * K1 + k] = (double)((long double)v_trg[t * K1 + k] + vt[k] * sf);
+  }
+}
+
+/* ----------------------------------------------------------------- AVX-512
+ * Eight targets per vector, sources broadcast one at a time, exactly the loop
+ * structure of generic-kernel.txx:175-188 with VecLen = 8.
+ */
+#if defined(__x86_64__)
+#define AVX512 __attribute__((target("avx512f,avx512dq,avx512vl,fma")))
+
+/* rsqrt_newton_iter<MAX,ITER> unrolled: z <- z*(x*z*z + c1_j), final *c2,
+ * intrin-wrapper.hpp:540-559.  c1_j = -3*2^(3^j-1); c2 = (-1/2)^((3^(n-1)-1)*3/2+1). */
+AVX512 static inline __m512d newton_avx512(__m512d z, __m512d x, int n) {
+  double pow3 = 1; /* 3^j */
+  for (int j = 0; j < n; j++) {
+    const double c1 = -3.0 * ldexp(1.0, (int)pow3 - 1);
+    z = _mm512_mul_pd(z, _mm512_fmadd_pd(x, _mm512_mul_pd(z, z), _mm512_set1_pd(c1)));
+    if (j == n - 1) {
+      const int e = (int)((pow3 - 1) * 3 / 2 + 1);
+      double c2 = 1;
+      for (int k = 0; k < e; k++) c2 *= -0.5;
+      z = _mm512_mul_pd(z, _mm512_set1_pd(c2));
+    }
+    pow3 *= 3;
+  }
+  return z;
+}
+
+AVX512 static inline __m512d rinv_avx512(__m512d r2, int n_newton) {
+  const __mmask8 m = _mm512_cmp_pd_mask(r2, _mm512_setzero_pd(), _CMP_GT_OQ);
+  return newton_avx512(_mm512_maskz_rsqrt14_pd(m, r2), r2, n_newton);
+}
+
+AVX512 static void eval_avx512(int ker, int digits, const double* r_trg, long nt, const double* r_src,
+                               const double* n_src, const double* v_src, long ns, double* v_trg, int nthreads) {
+  const int K0 = KD0[ker], K1 = KD1[ker], ND = NDIM[ker];
+  const double sf = oracle_scale(ker);
+  const int nn = oracle_newton_iters_avx512(digits);
+  const long NNt = (nt + 7) / 8 * 8;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (long t0 = 0; t0 < NNt; t0 += 8) {
+    double xt_[3][8];
+    for (int k = 0; k < 3; k++)
+      for (int l = 0; l < 8; l++) xt_[k][l] = (t0 + l < nt ? r_trg[(t0 + l) * 3 + k] : 0.0);
+    __m512d xt[3], vt[3];
+    for (int k = 0; k < 3; k++) { xt[k] = _mm512_loadu_pd(xt_[k]); vt[k] = _mm512_setzero_pd(); }
+    for (long s = 0; s < ns; s++) {
+      __m512d r[3], n[3], f[3];
+      for (int k = 0; k < 3; k++) r[k] = _mm512_sub_pd(xt[k], _mm512_set1_pd(r_src[s * 3 + k]));
+      for (int k = 0; k < ND; k++) n[k] = _mm512_set1_pd(n_src[s * 3 + k]);
+      for (int k = 0; k < K0; k++) f[k] = _mm512_set1_pd(v_src[s * K0 + k]);
+      const __m512d r2 = _mm512_add_pd(_mm512_add_pd(_mm512_mul_pd(r[0], r[0]), _mm512_mul_pd(r[1], r[1])),
+                                       _mm512_mul_pd(r[2], r[2]));
+      const __m512d rinv = rinv_avx512(r2, nn);
+      switch (ker) {
+        case LAP_FXU: vt[0] = _mm512_fmadd_pd(rinv, f[0], vt[0]); break;
+        case LAP_DXU: {
+          const __m512d rdotn = _mm512_add_pd(_mm512_add_pd(_mm512_mul_pd(r[0], n[0]), _mm512_mul_pd(r[1], n[1])),
+                                              _mm512_mul_pd(r[2], n[2]));
+          const __m512d rinv3 = _mm512_mul_pd(_mm512_mul_pd(rinv, rinv), rinv);
+          vt[0] = _mm512_fmadd_pd(_mm512_mul_pd(rdotn, rinv3), f[0], vt[0]);
+        } break;
+        case STK_FXU: {
+          const __m512d rinv3 = _mm512_mul_pd(_mm512_mul_pd(rinv, rinv), rinv);
+          const __m512d fr = _mm512_add_pd(_mm512_add_pd(_mm512_mul_pd(f[0], r[0]), _mm512_mul_pd(f[1], r[1])),
+                                           _mm512_mul_pd(f[2], r[2]));
+          const __m512d frr3 = _mm512_mul_pd(fr, rinv3);
+          for (int i = 0; i < 3; i++)
+            vt[i] = _mm512_add_pd(vt[i], _mm512_add_pd(_mm512_mul_pd(f[i], rinv), _mm512_mul_pd(r[i], frr3)));
+        } break;
+        case STK_DXU: {
+          const __m512d rinv2 = _mm512_mul_pd(rinv, rinv);
+          const __m512d rinv5 = _mm512_mul_pd(_mm512_mul_pd(rinv2, rinv2), rinv);
+          const __m512d rn = _mm512_add_pd(_mm512_add_pd(_mm512_mul_pd(r[0], n[0]), _mm512_mul_pd(r[1], n[1])),
+                                           _mm512_mul_pd(r[2], n[2]));
+          const __m512d rn5 = _mm512_mul_pd(rn, rinv5);
+          const __m512d fr = _mm512_add_pd(_mm512_add_pd(_mm512_mul_pd(f[0], r[0]), _mm512_mul_pd(f[1], r[1])),
+                                           _mm512_mul_pd(f[2], r[2]));
+          const __m512d q = _mm512_mul_pd(fr, rn5);
+          for (int i = 0; i < 3; i++) vt[i] = _mm512_add_pd(vt[i], _mm512_mul_pd(r[i], q));
+        } break;
+      }
+    }
+    double out[3][8];
+    for (int k = 0; k < K1; k++) _mm512_storeu_pd(out[k], vt[k]);
+    for (int k = 0; k < K1; k++)
+      for (int l = 0; l < 8 && t0 + l < nt; l++) v_trg[(t0 + l) * K1 + k] += out[k][l] * sf;
+  }
+}
+
+AVX512 static void rsqrt_avx512(int digits, const double* x, long n, double* y) {
+  const int nn = oracle_newton_iters_avx512(digits);
+  for (long i = 0; i < n; i += 8) {
+    double b[8], o[8];
+    for (int l = 0; l < 8; l++) b[l] = (i + l < n ? x[i + l] : 1.0);
+    _mm512_storeu_pd(o, rinv_avx512(_mm512_loadu_pd(b), nn));
+    for (int l = 0; l < 8 && i + l < n; l++) y[i + l] = o[l];
+  }
+}
+#endif
+
+int oracle_have_avx512(void) {
+#if defined(__x86_64__)
+  __builtin_cpu_init();
+  return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512dq") &&
+         __builtin_cpu_supports("avx512vl");
+#else
+  return 0;
+#endif
+}
+
+/* ------------------------------------------------------------ public API
+ * Return 0 on success, -1 on a bad kernel id or mode, -2 when the AVX-512
+ * mode is requested on a CPU without AVX-512.
+ */
+int oracle_eval(int ker, int mode, int digits, const double* r_trg, long nt, const double* r_src,
+                const double* n_src, const double* v_src, long ns, double* v_trg, int nthreads) {
+  if (ker < 0 || ker >= NKER) return -1;
+  if (nthreads < 1) nthreads = 1;
+  switch (mode) {
+    case ORACLE_EXACT: eval_exact(ker, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads); return 0;
+    case ORACLE_LDOUBLE: eval_ldouble(ker, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads); return 0;
+    case ORACLE_AVX512:
+#if defined(__x86_64__)
+      if (!oracle_have_avx512()) return -2;
+      eval_avx512(ker, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads);
+      return 0;
+#else
+      return -2;
+#endif
+  }
+  return -1;
+}
+
+/* KernelMatrix, exact mode: M is (ns*K0) x (nt*K1) row-major, overwritten. */
+int oracle_kernel_matrix(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
+                         long ns, double* M, int nthreads) {
+  if (ker < 0 || ker >= NKER) return -1;
+  const int K0 = KD0[ker], K1 = KD1[ker], ND = NDIM[ker];
+  const double sf = oracle_scale(ker);
+  const double zero3[3] = {0, 0, 0};
+  const long ld = nt * K1;
+#pragma omp parallel for schedule(static) num_threads(nthreads < 1 ? 1 : nthreads)
+  for (long s = 0; s < ns; s++) {
+    for (long t = 0; t < nt; t++) {
+      const double r[3] = {r_trg[t * 3 + 0] - r_src[s * 3 + 0], r_trg[t * 3 + 1] - r_src[s * 3 + 1],
+                           r_trg[t * 3 + 2] - r_src[s * 3 + 2]};
+      double u[3][3];
+      matrix_exact(ker, u, r, ND ? n_src + s * 3 : zero3);
+      for (int k0 = 0; k0 < K0; k0++)
+        for (int k1 = 0; k1 < K1; k1++) M[(s * K0 + k0) * ld + t * K1 + k1] = u[k0][k1] * sf;
+    }
+  }
+  return 0;
+}
+
+/* Masked rsqrt of one array: mode 0 exact, mode 1 AVX-512 estimate + Newton. */
+int oracle_rsqrt(int mode, int digits, const double* x, long n, double* y) {
+  if (mode == ORACLE_EXACT) {
+    for (long i = 0; i < n; i++) y[i] = rinv_exact(x[i]);
+    return 0;
+  }
+#if defined(__x86_64__)
+  if (mode == ORACLE_AVX512) {
+    if (!oracle_have_avx512()) return -2;
+    rsqrt_avx512(digits, x, n, y);
+    return 0;
+  }
+#endif
+  return -1;
+}

@@ -1,0 +1,148 @@
+// Test infrastructure only: a C-ABI wrapper that instantiates the reference
+// library's own GenericKernel code so that it can be called from the test
+// suite and from the CPU-baseline timer.  It is compiled from the reference
+// headers in place (see oracle/Makefile) into oracle/_ref/, never shipped as
+// part of the product and never called by it.
+//
+// Wrapped reference entry points:
+//   GenericKernel<uKernel>::Eval<Real,omp>(..., digits, self)   generic-kernel.txx:72-100
+//   GenericKernel<uKernel>::KernelMatrix<Real,false,digits>     generic-kernel.txx:213-329
+//   approx_rsqrt<digits>(x, mask)                               vec.txx:379-382
+// for the four micro-kernels of kernel_functions.hpp:32-79,125-218.
+#include <cstring>
+#include <string>
+
+#include "sctl/common.hpp"
+#include "sctl/iterator.hpp"
+#include "sctl/iterator.txx"
+#include "sctl/vector.hpp"
+#include "sctl/vector.txx"
+#include "sctl/matrix.hpp"
+#include "sctl/matrix.txx"
+#include "sctl/generic-kernel.hpp"
+#include "sctl/generic-kernel.txx"
+#include "sctl/kernel_functions.hpp"
+
+using sctl::Long;
+using sctl::Integer;
+
+namespace {
+
+template <class Real> using Vec = sctl::Vector<Real>;
+
+template <class K, class Real>
+void eval_one(int omp, int digits, const Real* r_trg, long nt, const Real* r_src, const Real* n_src,
+              const Real* v_src, long ns, Real* v_trg) {
+  K ker;
+  const Long nd = K::NormalDim();
+  Vec<Real> Xt(nt * 3, (sctl::Iterator<Real>)r_trg, false);
+  Vec<Real> Xs(ns * 3, (sctl::Iterator<Real>)r_src, false);
+  Vec<Real> Xn(nd ? ns * nd : 0, (sctl::Iterator<Real>)(nd ? n_src : nullptr), false);
+  Vec<Real> Vs(ns * K::SrcDim(), (sctl::Iterator<Real>)v_src, false);
+  Vec<Real> Vt(nt * K::TrgDim(), v_trg, false);  // pre-sized: Eval accumulates into it
+  const sctl::ConstIterator<char> self = (sctl::ConstIterator<char>)(const char*)&ker;
+  if (omp) K::template Eval<Real, true>(Vt, Xt, Xs, Xn, Vs, digits, self);
+  else     K::template Eval<Real, false>(Vt, Xt, Xs, Xn, Vs, digits, self);
+}
+
+template <class K>
+void kmat_one(int digits, const double* r_trg, long nt, const double* r_src, const double* n_src,
+              long ns, double* M_out) {
+  K ker;
+  const Long nd = K::NormalDim();
+  Vec<double> Xt(nt * 3, (sctl::Iterator<double>)r_trg, false);
+  Vec<double> Xs(ns * 3, (sctl::Iterator<double>)r_src, false);
+  Vec<double> Xn(nd ? ns * nd : 0, (sctl::Iterator<double>)(nd ? n_src : nullptr), false);
+  sctl::Matrix<double> M;
+  switch (digits) {
+    case 3:  ker.template KernelMatrix<double, false, 3>(M, Xt, Xs, Xn); break;
+    case 7:  ker.template KernelMatrix<double, false, 7>(M, Xt, Xs, Xn); break;
+    case 11: ker.template KernelMatrix<double, false, 11>(M, Xt, Xs, Xn); break;
+    default: ker.template KernelMatrix<double, false, -1>(M, Xt, Xs, Xn); break;
+  }
+  const long n = (long)M.Dim(0) * (long)M.Dim(1);
+  if (n) std::memcpy(M_out, &M[0][0], sizeof(double) * n);
+}
+
+template <Integer digits> void rsqrt_digits(const double* x, long n, double* y) {
+  constexpr Integer VL = sctl::DefaultVecLen<double>();
+  using V = sctl::Vec<double, VL>;
+  for (long i = 0; i < n; i += VL) {
+    alignas(64) double buf[VL], out[VL];
+    for (Integer k = 0; k < VL; k++) buf[k] = (i + k < n ? x[i + k] : 1.0);
+    V xv = V::LoadAligned(buf);
+    V yv = sctl::approx_rsqrt<digits>(xv, xv > V::Zero());
+    yv.StoreAligned(out);
+    for (Integer k = 0; k < VL && i + k < n; k++) y[i + k] = out[k];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// kernel ids: 0 Laplace3D_FxU, 1 Laplace3D_DxU, 2 Stokes3D_FxU, 3 Stokes3D_DxU
+int sctlref_info(int kernel, int* dims, double* scale, char* name, int name_len) {
+  auto fill = [&](auto ker, double sf) {
+    using K = decltype(ker);
+    dims[0] = K::CoordDim(); dims[1] = K::SrcDim(); dims[2] = K::TrgDim(); dims[3] = K::NormalDim();
+    dims[4] = (int)K::FLOPS();
+    *scale = sf;
+    std::strncpy(name, ker.Name().c_str(), name_len);
+    return 0;
+  };
+  switch (kernel) {
+    case 0: return fill(sctl::Laplace3D_FxU(), sctl::kernel_impl::Laplace3D_FxU::uKerScaleFactor<double>());
+    case 1: return fill(sctl::Laplace3D_DxU(), sctl::kernel_impl::Laplace3D_DxU::uKerScaleFactor<double>());
+    case 2: return fill(sctl::Stokes3D_FxU(), sctl::kernel_impl::Stokes3D_FxU::uKerScaleFactor<double>());
+    case 3: return fill(sctl::Stokes3D_DxU(), sctl::kernel_impl::Stokes3D_DxU::uKerScaleFactor<double>());
+  }
+  return -1;
+}
+
+int sctlref_veclen(void) { return (int)sctl::DefaultVecLen<double>(); }
+
+int sctlref_eval(int kernel, int digits, int omp, const double* r_trg, long nt, const double* r_src,
+                 const double* n_src, const double* v_src, long ns, double* v_trg) {
+  switch (kernel) {
+    case 0: eval_one<sctl::Laplace3D_FxU, double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
+    case 1: eval_one<sctl::Laplace3D_DxU, double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
+    case 2: eval_one<sctl::Stokes3D_FxU, double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
+    case 3: eval_one<sctl::Stokes3D_DxU, double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
+  }
+  return -1;
+}
+
+int sctlref_eval_f32(int kernel, int digits, int omp, const float* r_trg, long nt, const float* r_src,
+                     const float* n_src, const float* v_src, long ns, float* v_trg) {
+  switch (kernel) {
+    case 0: eval_one<sctl::Laplace3D_FxU, float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
+    case 1: eval_one<sctl::Laplace3D_DxU, float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
+    case 2: eval_one<sctl::Stokes3D_FxU, float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
+    case 3: eval_one<sctl::Stokes3D_DxU, float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
+  }
+  return -1;
+}
+
+int sctlref_kernel_matrix(int kernel, int digits, const double* r_trg, long nt, const double* r_src,
+                          const double* n_src, long ns, double* M) {
+  switch (kernel) {
+    case 0: kmat_one<sctl::Laplace3D_FxU>(digits, r_trg, nt, r_src, n_src, ns, M); return 0;
+    case 1: kmat_one<sctl::Laplace3D_DxU>(digits, r_trg, nt, r_src, n_src, ns, M); return 0;
+    case 2: kmat_one<sctl::Stokes3D_FxU>(digits, r_trg, nt, r_src, n_src, ns, M); return 0;
+    case 3: kmat_one<sctl::Stokes3D_DxU>(digits, r_trg, nt, r_src, n_src, ns, M); return 0;
+  }
+  return -1;
+}
+
+int sctlref_rsqrt(int digits, const double* x, long n, double* y) {
+  switch (digits) {
+    case 4: rsqrt_digits<4>(x, n, y); return 0;
+    case 7: rsqrt_digits<7>(x, n, y); return 0;
+    case 11: rsqrt_digits<11>(x, n, y); return 0;
+    case -1: rsqrt_digits<-1>(x, n, y); return 0;
+  }
+  return -1;
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// Host-side internals shared by the C-ABI and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/sctl_amd.h"
+
+namespace sctl_amd {
+
+// Per-thread message of the last failing call (sctl_amd_last_error()).
+void set_error(const std::string& msg);
+const char* last_error();
+
+// Status helpers: every HIP call on the product path is checked.
+#define SCTL_HIP_TRY(expr)                                                                      \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      ::sctl_amd::set_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));          \
+      return e_ == hipErrorOutOfMemory ? SCTL_AMD_ERROR_OUT_OF_MEMORY : SCTL_AMD_ERROR_HIP;    \
+    }                                                                                           \
+  } while (0)
+
+#define SCTL_TRY(expr)                   \
+  do {                                   \
+    int s_ = (expr);                     \
+    if (s_ != SCTL_AMD_SUCCESS) return s_; \
+  } while (0)
+
+#define SCTL_CHECK_ARG(cond, msg)                       \
+  do {                                                  \
+    if (!(cond)) {                                      \
+      ::sctl_amd::set_error(msg);                       \
+      return SCTL_AMD_ERROR_INVALID_VALUE;              \
+    }                                                   \
+  } while (0)
+
+// Device facts, looked up once per device.
+struct DeviceInfo {
+  int device = -1;
+  int num_cus = 0;
+  int max_threads = 0;
+  bool ok = false;
+};
+int device_info(DeviceInfo* out);  // for the current device
+
+// Stream-ordered device scratch: allocated from the device's default memory
+// pool (whose release threshold is raised at init, so memory is reused
+// across calls rather than returned to the driver), freed on the same stream.
+class Scratch {
+ public:
+  Scratch() = default;
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  ~Scratch() { release(); }
+  int alloc(size_t bytes, hipStream_t s) {
+    release();
+    stream_ = s;
+    if (!bytes) return SCTL_AMD_SUCCESS;
+    SCTL_HIP_TRY(hipMallocAsync(&ptr_, bytes, s));
+    bytes_ = bytes;
+    return SCTL_AMD_SUCCESS;
+  }
+  void release() {
+    if (ptr_) (void)hipFreeAsync(ptr_, stream_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(ptr_); }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+  hipStream_t stream_ = nullptr;
+};
+
+// Launch-shape knobs of the dense kernels (chosen by the tuner or defaults).
+struct DenseShape {
+  int tpl = 1;  // targets per lane
+};
+
+// Dense Eval launchers (device pointers, stream-ordered).  v_trg is
+// accumulated: v_trg[t*K1+k] += scale * sum.
+int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
+                    const double* n_src, const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape,
+                    hipStream_t stream);
+int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
+                      const double* n_src, const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape,
+                      hipStream_t stream);
+int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
+                         const double* n_src, int64_t ns, double* M, hipStream_t stream);
+
+// Morton ordering of points (30-bit keys over the points' bounding box);
+// writes a permutation of [0, n).  Implemented in sort.hip.
+int morton_order(const double* pts, int64_t n, uint32_t* perm, hipStream_t stream);
+
+// Shape defaults and tuning-database lookup (tune.cpp).
+DenseShape pick_dense_shape(int ker, int precision, int64_t nt, int64_t ns);
+
+int kernel_dims(int ker, int* K0, int* K1, int* ND);
+double kernel_scale(int ker);
+int newton_steps_f64(int digits);
+
+}  // namespace sctl_amd
