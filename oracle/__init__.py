@@ -39,13 +39,19 @@ def build(with_ref=None):
     subprocess.check_call(["make", "-s", "-C", HERE] + targets)
 
 
-_lib = None
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None and a.size else None
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    """One build of the C restatement (liboracle.so, or a mutant build)."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             build(with_ref=False)
         L = ctypes.CDLL(path)
@@ -57,69 +63,98 @@ def lib():
         L.oracle_scale.argtypes = [ctypes.c_int]
         L.oracle_scale.restype = ctypes.c_double
         L.oracle_newton_iters_avx512.argtypes = [ctypes.c_int]
-        _lib = L
-    return _lib
+        self.lib = L
+        self.path = path
+
+    def dims(self, kernel):
+        d = (ctypes.c_int * 5)()
+        if self.lib.oracle_dims(kernel, d) != 0:
+            raise ValueError("unknown kernel %r" % (kernel,))
+        return tuple(d)  # (coord, K0, K1, N_DIM, FLOPS)
+
+    def scale(self, kernel):
+        return self.lib.oracle_scale(kernel)
+
+    def have_avx512(self):
+        return bool(self.lib.oracle_have_avx512())
+
+    def eval(self, kernel, r_trg, r_src, n_src, v_src, v_trg=None, mode=EXACT, digits=-1, nthreads=None):
+        """v_trg += Eval(...); returns v_trg (a fresh zero array when v_trg is None)."""
+        _, K0, K1, ND, _ = self.dims(kernel)
+        r_trg, r_src, v_src = _c(r_trg), _c(r_src), _c(v_src)
+        n_src = _c(n_src) if ND else None
+        nt, ns = r_trg.size // 3, r_src.size // 3
+        assert r_trg.size == 3 * nt and r_src.size == 3 * ns and v_src.size == K0 * ns
+        assert not ND or n_src.size == 3 * ns
+        out = np.zeros(nt * K1) if v_trg is None else np.ascontiguousarray(v_trg, dtype=np.float64).copy()
+        nth = nthreads or os.cpu_count() or 1
+        rc = self.lib.oracle_eval(kernel, mode, digits, _p(r_trg), nt, _p(r_src), _p(n_src), _p(v_src), ns,
+                                  _p(out), nth)
+        if rc != 0:
+            raise RuntimeError("oracle_eval failed rc=%d" % rc)
+        return out
+
+    def kernel_matrix(self, kernel, r_trg, r_src, n_src, nthreads=None):
+        _, K0, K1, ND, _ = self.dims(kernel)
+        r_trg, r_src = _c(r_trg), _c(r_src)
+        n_src = _c(n_src) if ND else None
+        nt, ns = r_trg.size // 3, r_src.size // 3
+        M = np.zeros((ns * K0, nt * K1))
+        rc = self.lib.oracle_kernel_matrix(kernel, _p(r_trg), nt, _p(r_src), _p(n_src), ns, _p(M),
+                                           nthreads or os.cpu_count() or 1)
+        if rc != 0:
+            raise RuntimeError("oracle_kernel_matrix failed rc=%d" % rc)
+        return M
+
+    def rsqrt(self, x, mode=EXACT, digits=-1):
+        x = _c(x)
+        y = np.zeros_like(x)
+        rc = self.lib.oracle_rsqrt(mode, digits, _p(x), x.size, _p(y))
+        if rc != 0:
+            raise RuntimeError("oracle_rsqrt failed rc=%d" % rc)
+        return y
 
 
-def _p(a):
-    return a.ctypes.data_as(_dp) if a is not None and a.size else None
+_default = None
 
 
-def _c(a):
-    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+def default():
+    global _default
+    if _default is None:
+        _default = Oracle()
+    return _default
 
 
 def dims(kernel):
-    d = (ctypes.c_int * 5)()
-    if lib().oracle_dims(kernel, d) != 0:
-        raise ValueError("unknown kernel %r" % (kernel,))
-    return tuple(d)  # (coord, K0, K1, N_DIM, FLOPS)
+    return default().dims(kernel)
 
 
 def scale(kernel):
-    return lib().oracle_scale(kernel)
+    return default().scale(kernel)
 
 
 def have_avx512():
-    return bool(lib().oracle_have_avx512())
+    return default().have_avx512()
 
 
-def eval(kernel, r_trg, r_src, n_src, v_src, v_trg=None, mode=EXACT, digits=-1, nthreads=None):
-    """v_trg += Eval(...); returns v_trg (a fresh zero array when v_trg is None)."""
-    _, K0, K1, ND, _ = dims(kernel)
-    r_trg, r_src, v_src = _c(r_trg), _c(r_src), _c(v_src)
-    n_src = _c(n_src) if ND else None
-    nt, ns = r_trg.size // 3, r_src.size // 3
-    assert r_trg.size == 3 * nt and r_src.size == 3 * ns and v_src.size == K0 * ns
-    assert not ND or n_src.size == 3 * ns
-    out = np.zeros(nt * K1) if v_trg is None else np.ascontiguousarray(v_trg, dtype=np.float64).copy()
-    nth = nthreads or os.cpu_count() or 1
-    rc = lib().oracle_eval(kernel, mode, digits, _p(r_trg), nt, _p(r_src), _p(n_src), _p(v_src), ns, _p(out), nth)
-    if rc != 0:
-        raise RuntimeError("oracle_eval failed rc=%d" % rc)
+def eval(*args, **kw):
+    return default().eval(*args, **kw)
+
+
+def kernel_matrix(*args, **kw):
+    return default().kernel_matrix(*args, **kw)
+
+
+def rsqrt(*args, **kw):
+    return default().rsqrt(*args, **kw)
+
+
+def build_mutant(index, out_dir):
+    """Compiles the oracle with ORACLE_MUTANT=index into out_dir; returns the path."""
+    out = os.path.join(out_dir, "liboracle_mutant%d.so" % index)
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-std=c11",
+                           "-DORACLE_MUTANT=%d" % index, "-o", out, os.path.join(HERE, "direct_sum.c")])
     return out
-
-
-def kernel_matrix(kernel, r_trg, r_src, n_src, nthreads=None):
-    _, K0, K1, ND, _ = dims(kernel)
-    r_trg, r_src = _c(r_trg), _c(r_src)
-    n_src = _c(n_src) if ND else None
-    nt, ns = r_trg.size // 3, r_src.size // 3
-    M = np.zeros((ns * K0, nt * K1))
-    rc = lib().oracle_kernel_matrix(kernel, _p(r_trg), nt, _p(r_src), _p(n_src), ns, _p(M),
-                                    nthreads or os.cpu_count() or 1)
-    if rc != 0:
-        raise RuntimeError("oracle_kernel_matrix failed rc=%d" % rc)
-    return M
-
-
-def rsqrt(x, mode=EXACT, digits=-1):
-    x = _c(x)
-    y = np.zeros_like(x)
-    rc = lib().oracle_rsqrt(mode, digits, _p(x), x.size, _p(y))
-    if rc != 0:
-        raise RuntimeError("oracle_rsqrt failed rc=%d" % rc)
-    return y
 
 
 # ------------------------------------------------------------------ reference

@@ -42,6 +42,15 @@
 #include <stdint.h>
 #include <string.h>
 
+/* ORACLE_MUTANT != 0 builds a deliberately wrong oracle; the test suite
+ * compiles each mutant and checks that the pinning checks reject it:
+ *   1 Stokes FxU without its rank-one term      2 Laplace DxU with r = x_s - x_t
+ *   3 KernelMatrix written transposed           4 Stokes DxU scaled by 1/(4 pi)
+ *   5 one Newton step too few (AVX-512 mode)    6 Eval skips the last source */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
 #if defined(__x86_64__)
 #include <immintrin.h>
 #endif
@@ -70,7 +79,7 @@ double oracle_scale(int ker) {
     case LAP_FXU: return 1 / (4 * pi_d());
     case LAP_DXU: return 1 / (4 * pi_d());
     case STK_FXU: return 1 / (8 * pi_d());
-    case STK_DXU: return 3 / (4 * pi_d());
+    case STK_DXU: return (ORACLE_MUTANT == 4 ? 1 : 3) / (4 * pi_d());
   }
   return 0;
 }
@@ -95,7 +104,10 @@ static int resolve_digits(int digits) { return digits == -1 ? (int)(52 * 0.30102
 /* mylog2, intrin-wrapper.hpp:560-562 */
 static int mylog2(int x) { return x < 1 ? 0 : 1 + mylog2(x / 2); }
 /* Newton count for the rsqrt14 estimate, intrin-wrapper.hpp:3399 */
-int oracle_newton_iters_avx512(int digits) { return mylog2((int)(resolve_digits(digits) / 4.2144199393)); }
+int oracle_newton_iters_avx512(int digits) {
+  const int n = mylog2((int)(resolve_digits(digits) / 4.2144199393));
+  return ORACLE_MUTANT == 5 && n > 0 ? n - 1 : n;
+}
 
 /* ---------------------------------------------------------------- exact mode
  * Scalar arithmetic in the reference's operation order.  rinv() is the masked
@@ -117,14 +129,14 @@ static inline void interact_exact(int ker, double* vt, const double* r, const do
       vt[0] = u * f[0] + vt[0];
     } break;
     case LAP_DXU: {
-      const double rdotn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
+      const double rdotn = (ORACLE_MUTANT == 2 ? -1 : 1) * (r[0] * n[0] + r[1] * n[1] + r[2] * n[2]);
       const double rinv3 = rinv * rinv * rinv;
       const double u = rdotn * rinv3;
       vt[0] = u * f[0] + vt[0];
     } break;
     case STK_FXU: {
       const double rinv3 = rinv * rinv * rinv;
-      const double fdotr_rinv3 = (f[0] * r[0] + f[1] * r[1] + f[2] * r[2]) * rinv3;
+      const double fdotr_rinv3 = (ORACLE_MUTANT == 1 ? 0.0 : 1.0) * (f[0] * r[0] + f[1] * r[1] + f[2] * r[2]) * rinv3;
       for (int i = 0; i < 3; i++) vt[i] += f[i] * rinv + r[i] * fdotr_rinv3;
     } break;
     case STK_DXU: {
@@ -144,14 +156,14 @@ static inline void matrix_exact(int ker, double u[3][3], const double* r, const 
   switch (ker) {
     case LAP_FXU: u[0][0] = rinv; break;
     case LAP_DXU: {
-      const double rdotn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
+      const double rdotn = (ORACLE_MUTANT == 2 ? -1 : 1) * (r[0] * n[0] + r[1] * n[1] + r[2] * n[2]);
       const double rinv3 = rinv * rinv * rinv;
       u[0][0] = rdotn * rinv3;
     } break;
     case STK_FXU: {
       const double rinv3 = rinv * rinv * rinv;
       for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) u[i][j] = (i == j ? rinv : 0.0) + r[i] * r[j] * rinv3;
+        for (int j = 0; j < 3; j++) u[i][j] = (i == j ? rinv : 0.0) + (ORACLE_MUTANT == 1 ? 0.0 : r[i] * r[j] * rinv3);
     } break;
     case STK_DXU: {
       const double rinv2 = rinv * rinv;
@@ -171,7 +183,7 @@ static void eval_exact(int ker, const double* r_trg, long nt, const double* r_sr
 #pragma omp parallel for schedule(static) num_threads(nthreads)
   for (long t = 0; t < nt; t++) {
     double vt[3] = {0, 0, 0};
-    for (long s = 0; s < ns; s++) {
+    for (long s = 0; s < ns - (ORACLE_MUTANT == 6 ? 1 : 0); s++) {
       const double r[3] = {r_trg[t * 3 + 0] - r_src[s * 3 + 0], r_trg[t * 3 + 1] - r_src[s * 3 + 1],
                            r_trg[t * 3 + 2] - r_src[s * 3 + 2]};
       interact_exact(ker, vt, r, ND ? n_src + s * 3 : zero3, v_src + s * K0);
@@ -362,7 +374,10 @@ int oracle_kernel_matrix(int ker, const double* r_trg, long nt, const double* r_
       double u[3][3];
       matrix_exact(ker, u, r, ND ? n_src + s * 3 : zero3);
       for (int k0 = 0; k0 < K0; k0++)
-        for (int k1 = 0; k1 < K1; k1++) M[(s * K0 + k0) * ld + t * K1 + k1] = u[k0][k1] * sf;
+        for (int k1 = 0; k1 < K1; k1++) {
+          if (ORACLE_MUTANT == 3) M[(t * K1 + k1) * (ns * K0) + s * K0 + k0] = u[k0][k1] * sf;
+          else M[(s * K0 + k0) * ld + t * K1 + k1] = u[k0][k1] * sf;
+        }
     }
   }
   return 0;

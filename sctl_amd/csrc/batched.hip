@@ -1,16 +1,599 @@
-// Batched near-field blocks (plan API).  Implementation follows.
+// Batched near-field blocks: the plan API of include/sctl_amd.h.
+//
+// BoundaryIntegralOp evaluates thousands of small, uneven (target-set,
+// source-set) blocks (boundary_integral.txx:1009-1032 builds them, :1169-1238
+// and :1342-1364 evaluate them one OpenMP task each).  Here all blocks of a
+// plan run in ONE persistent launch:
+//
+//  * Work items are (block, chunk of <= 64 targets).  Items are ordered by
+//    cost Ns*Nt, largest first (longest-processing-time order), and every
+//    wave of the persistent grid takes the next item from a global atomic
+//    counter, so the tail of the launch is made of the smallest items.
+//  * A wave owns an item.  Chunks of <= 16 or <= 32 targets use G = 4 or 2
+//    lane groups (64/G lanes each, one target per lane): group g takes every
+//    G-th source and the groups' sums are combined with cross-lane shuffles,
+//    so a 16-target block still keeps all 64 lanes busy.
+//  * The block's sources are staged through a wave-private LDS region,
+//    64*SPL records at a time (no workgroup barrier: waves run independently).
+//  * Mixed precision: sources and targets are stored once per geometry as
+//    FP32 offsets from the block's source bounding-box centre (differenced in
+//    FP64); each stage's FP32 partial is added into FP64 sums with Kahan
+//    compensation.  FP64: plain FP64 records, as in the dense path.
+//
+// Every wave leaves the loop when the counter passes the number of items, so
+// the grid always drains.  An apply allocates nothing and issues no memset
+// (the counter is cleared by a one-lane kernel), so it can be captured in a
+// hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <string>
+#include <vector>
+
 #include "internal.hpp"
+#include "kernels.hpp"
+
+namespace sctl_amd {
+
+namespace {
+
+constexpr int BWG = 256;  // 4 waves per workgroup, each wave independent
+constexpr float PADF = 1e18f;
+constexpr double PADD = 1e100;
+
+struct Item {
+  int32_t block;  // block index
+  int32_t t_off;  // first target of the chunk, relative to the block
+  int32_t t_cnt;  // targets in the chunk (<= 64)
+  int32_t lg;     // log2 of the number of lane groups
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int next_item(int* counter, int lane) {
+  int it = 0;
+  if (lane == 0) it = atomicAdd(counter, 1);
+  return __builtin_amdgcn_readfirstlane(it);
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) {
+  const int lane = threadIdx.x & 63;
+  return __shfl(v, lane ^ m, 64);
+}
+
+__global__ void k_reset_counter(int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+// ------------------------------------------------------------- FP64 blocks
+template <int KER, int NEWTON, int SPL>
+__global__ void __launch_bounds__(BWG)
+k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ counter,
+             const int64_t* __restrict__ trg_dsp, const int64_t* __restrict__ src_dsp,
+             const int32_t* __restrict__ src_cnt, const double* __restrict__ xt, const double* __restrict__ xs,
+             const double* __restrict__ xn, const double* __restrict__ fs, double* __restrict__ out, double scale) {
+  using K = Ker<KER>;
+  constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, R = rec_f64<KER>(), STAGE = 64 * SPL;
+  __shared__ double lds_all[(BWG / 64) * STAGE * R];
+  const int lane = threadIdx.x & 63;
+  double* lds = lds_all + (threadIdx.x >> 6) * (STAGE * R);
+
+  for (int it = next_item(counter, lane); it < nitems; it = next_item(counter, lane)) {
+    const Item I = items[it];
+    const int lg = I.lg, G = 1 << lg, L = 64 >> lg;
+    const int g = lane >> (6 - lg), tl = lane & (L - 1);
+    const bool tv = tl < I.t_cnt;
+    const int64_t t = trg_dsp[I.block] + I.t_off + (tv ? tl : 0);
+    const double tx = xt[t * 3 + 0], ty = xt[t * 3 + 1], tz = xt[t * 3 + 2];
+    const int64_t sb = src_dsp[I.block];
+    const int nsb = src_cnt[I.block];
+    double acc[K1];
+#pragma unroll
+    for (int k = 0; k < K1; k++) acc[k] = 0.0;
+
+    for (int s0 = 0; s0 < nsb; s0 += STAGE) {
+      const int len = min(STAGE, nsb - s0);
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < SPL; q++) {
+        const int i = lane + 64 * q;
+        double* rec = lds + i * R;
+        if (i < len) {
+          const int64_t s = sb + s0 + i;
+          rec[0] = xs[s * 3 + 0];
+          rec[1] = xs[s * 3 + 1];
+          rec[2] = xs[s * 3 + 2];
+#pragma unroll
+          for (int k = 0; k < ND; k++) rec[3 + k] = xn[s * 3 + k];
+#pragma unroll
+          for (int k = 0; k < K0; k++) rec[3 + ND + k] = fs[s * K0 + k];
+        } else {
+          rec[0] = rec[1] = rec[2] = PADD;
+#pragma unroll
+          for (int k = 3; k < R; k++) rec[k] = 0.0;
+        }
+      }
+      wave_sync();
+      double saved[K1];
+#pragma unroll
+      for (int k = 0; k < K1; k++) saved[k] = acc[k];
+#pragma unroll 2
+      for (int i = g; i < STAGE; i += G) {
+        const double* rec = lds + i * R;
+        double n[3] = {0, 0, 0}, f[3];
+#pragma unroll
+        for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
+#pragma unroll
+        for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
+        interact<KER, NEWTON, false>(acc, tx - rec[0], ty - rec[1], tz - rec[2], n, f);
+      }
+      bool bad = false;
+#pragma unroll
+      for (int k = 0; k < K1; k++) bad |= !__builtin_isfinite(acc[k]);
+      if (bad) {
+#pragma unroll
+        for (int k = 0; k < K1; k++) acc[k] = saved[k];
+        for (int i = g; i < len; i += G) {
+          const double* rec = lds + i * R;
+          double n[3] = {0, 0, 0}, f[3];
+          for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
+          for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
+          interact<KER, NEWTON, true>(acc, tx - rec[0], ty - rec[1], tz - rec[2], n, f);
+        }
+      }
+    }
+    for (int m = L; m < 64; m <<= 1)
+#pragma unroll
+      for (int k = 0; k < K1; k++) acc[k] += shfl_xor_d(acc[k], m);
+    if (g == 0 && tv) {
+#pragma unroll
+      for (int k = 0; k < K1; k++) out[t * K1 + k] += acc[k] * scale;
+    }
+  }
+}
+
+// ------------------------------------------------------------ mixed blocks
+// xs32: per source {x-c, y-c, z-c, 0[, nx, ny, nz, 0]} (Q float4 per source);
+// xt32: per target {x-c, y-c, z-c, 0} relative to its block's centre.
+template <int KER, int NEWTON, int SPL>
+__global__ void __launch_bounds__(BWG)
+k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ counter,
+               const int64_t* __restrict__ trg_dsp, const int64_t* __restrict__ src_dsp,
+               const int32_t* __restrict__ src_cnt, const float4* __restrict__ xt32,
+               const float4* __restrict__ xs32, const double* __restrict__ fs, double* __restrict__ out,
+               double scale) {
+  using K = Ker<KER>;
+  constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, R = rec_f32<KER>(), STAGE = 64 * SPL;
+  constexpr int QG = ND ? 2 : 1;  // float4 per source in xs32
+  __shared__ float lds_all[(BWG / 64) * STAGE * R];
+  const int lane = threadIdx.x & 63;
+  float* lds = lds_all + (threadIdx.x >> 6) * (STAGE * R);
+
+  for (int it = next_item(counter, lane); it < nitems; it = next_item(counter, lane)) {
+    const Item I = items[it];
+    const int lg = I.lg, G = 1 << lg, L = 64 >> lg;
+    const int g = lane >> (6 - lg), tl = lane & (L - 1);
+    const bool tv = tl < I.t_cnt;
+    const int64_t t = trg_dsp[I.block] + I.t_off + (tv ? tl : 0);
+    const float4 tp = xt32[t];
+    const int64_t sb = src_dsp[I.block];
+    const int nsb = src_cnt[I.block];
+    double acc[K1], comp[K1];
+#pragma unroll
+    for (int k = 0; k < K1; k++) acc[k] = comp[k] = 0.0;
+
+    for (int s0 = 0; s0 < nsb; s0 += STAGE) {
+      const int len = min(STAGE, nsb - s0);
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < SPL; q++) {
+        const int i = lane + 64 * q;
+        float* rec = lds + i * R;
+        if (i < len) {
+          const int64_t s = sb + s0 + i;
+          const float4 a = xs32[s * QG];
+          rec[0] = a.x;
+          rec[1] = a.y;
+          rec[2] = a.z;
+          if constexpr (ND) {
+            const float4 b = xs32[s * QG + 1];
+            rec[3] = b.x;
+            rec[4] = b.y;
+            rec[5] = b.z;
+          }
+#pragma unroll
+          for (int k = 0; k < K0; k++) rec[3 + ND + k] = (float)fs[s * K0 + k];
+        } else {
+          rec[0] = rec[1] = rec[2] = PADF;
+#pragma unroll
+          for (int k = 3; k < R; k++) rec[k] = 0.0f;
+        }
+      }
+      wave_sync();
+      float p[K1];
+#pragma unroll
+      for (int k = 0; k < K1; k++) p[k] = 0.0f;
+#pragma unroll 4
+      for (int i = g; i < STAGE; i += G) {
+        const float* rec = lds + i * R;
+        float n[3] = {0, 0, 0}, f[3];
+#pragma unroll
+        for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
+#pragma unroll
+        for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
+        interact<KER, NEWTON, false>(p, tp.x - rec[0], tp.y - rec[1], tp.z - rec[2], n, f);
+      }
+      bool bad = false;
+#pragma unroll
+      for (int k = 0; k < K1; k++) bad |= !__builtin_isfinite(p[k]);
+      if (bad) {
+#pragma unroll
+        for (int k = 0; k < K1; k++) p[k] = 0.0f;
+        for (int i = g; i < STAGE; i += G) {
+          const float* rec = lds + i * R;
+          float n[3] = {0, 0, 0}, f[3];
+          for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
+          for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
+          interact<KER, NEWTON, true>(p, tp.x - rec[0], tp.y - rec[1], tp.z - rec[2], n, f);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K1; k++) {  // Kahan: acc += p
+        const double y = (double)p[k] - comp[k];
+        const double s = acc[k] + y;
+        comp[k] = (s - acc[k]) - y;
+        acc[k] = s;
+      }
+    }
+    double v[K1];
+#pragma unroll
+    for (int k = 0; k < K1; k++) v[k] = acc[k] - comp[k];
+    for (int m = L; m < 64; m <<= 1)
+#pragma unroll
+      for (int k = 0; k < K1; k++) v[k] += shfl_xor_d(v[k], m);
+    if (g == 0 && tv) {
+#pragma unroll
+      for (int k = 0; k < K1; k++) out[t * K1 + k] += v[k] * scale;
+    }
+  }
+}
+
+// Per block: bounding-box centre of its sources (FP64), then FP32 offsets of
+// its sources and targets.  One workgroup per block.
+__global__ void __launch_bounds__(BWG)
+k_pack_blocks(int64_t nblocks, const int64_t* __restrict__ trg_dsp, const int64_t* __restrict__ trg_cnt,
+              const int64_t* __restrict__ src_dsp, const int32_t* __restrict__ src_cnt,
+              const double* __restrict__ xt, const double* __restrict__ xs, const double* __restrict__ xn,
+              int has_normal, float4* __restrict__ xt32, float4* __restrict__ xs32) {
+  __shared__ double red[6][BWG];
+  const int tid = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  if (b >= nblocks) return;
+  const int64_t sb = src_dsp[b];
+  const int ns = src_cnt[b];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = tid; i < ns; i += BWG)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double v = xs[(sb + i) * 3 + k];
+      lo[k] = fmin(lo[k], v);
+      hi[k] = fmax(hi[k], v);
+    }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    red[k][tid] = lo[k];
+    red[3 + k][tid] = hi[k];
+  }
+  __syncthreads();
+  for (int w = BWG / 2; w > 0; w >>= 1) {
+    if (tid < w)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        red[k][tid] = fmin(red[k][tid], red[k][tid + w]);
+        red[3 + k][tid] = fmax(red[3 + k][tid], red[3 + k][tid + w]);
+      }
+    __syncthreads();
+  }
+  double c[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) c[k] = ns > 0 ? 0.5 * (red[k][0] + red[3 + k][0]) : 0.0;
+  const int QG = has_normal ? 2 : 1;
+  for (int i = tid; i < ns; i += BWG) {
+    const int64_t s = sb + i;
+    xs32[s * QG] = make_float4((float)(xs[s * 3 + 0] - c[0]), (float)(xs[s * 3 + 1] - c[1]),
+                               (float)(xs[s * 3 + 2] - c[2]), 0.0f);
+    if (has_normal)
+      xs32[s * QG + 1] = make_float4((float)xn[s * 3 + 0], (float)xn[s * 3 + 1], (float)xn[s * 3 + 2], 0.0f);
+  }
+  const int64_t tb = trg_dsp[b], nt = trg_cnt[b];
+  for (int64_t i = tid; i < nt; i += BWG) {
+    const int64_t t = tb + i;
+    xt32[t] = make_float4((float)(xt[t * 3 + 0] - c[0]), (float)(xt[t * 3 + 1] - c[1]),
+                          (float)(xt[t * 3 + 2] - c[2]), 0.0f);
+  }
+}
+
+using BlocksF64Fn = void (*)(const Item*, int, int*, const int64_t*, const int64_t*, const int32_t*, const double*,
+                             const double*, const double*, const double*, double*, double);
+using BlocksMixFn = void (*)(const Item*, int, int*, const int64_t*, const int64_t*, const int32_t*, const float4*,
+                             const float4*, const double*, double*, double);
+
+template <int KER, int NEWTON> BlocksF64Fn f64_blocks(int spl) {
+  return spl >= 4 ? k_blocks_f64<KER, NEWTON, 4> : (spl >= 2 ? k_blocks_f64<KER, NEWTON, 2> : k_blocks_f64<KER, NEWTON, 1>);
+}
+template <int KER> BlocksF64Fn f64_blocks(int newton, int spl) {
+  return newton == 0 ? f64_blocks<KER, 0>(spl) : (newton == 1 ? f64_blocks<KER, 1>(spl) : f64_blocks<KER, 2>(spl));
+}
+BlocksF64Fn f64_blocks(int ker, int newton, int spl) {
+  switch (ker) {
+    case LAP_FXU: return f64_blocks<LAP_FXU>(newton, spl);
+    case LAP_DXU: return f64_blocks<LAP_DXU>(newton, spl);
+    case STK_FXU: return f64_blocks<STK_FXU>(newton, spl);
+    default: return f64_blocks<STK_DXU>(newton, spl);
+  }
+}
+template <int KER, int NEWTON> BlocksMixFn mix_blocks(int spl) {
+  return spl >= 4 ? k_blocks_mixed<KER, NEWTON, 4>
+                  : (spl >= 2 ? k_blocks_mixed<KER, NEWTON, 2> : k_blocks_mixed<KER, NEWTON, 1>);
+}
+BlocksMixFn mix_blocks(int ker, int newton, int spl) {
+  switch (ker) {
+    case LAP_FXU: return newton ? mix_blocks<LAP_FXU, 1>(spl) : mix_blocks<LAP_FXU, 0>(spl);
+    case LAP_DXU: return newton ? mix_blocks<LAP_DXU, 1>(spl) : mix_blocks<LAP_DXU, 0>(spl);
+    case STK_FXU: return newton ? mix_blocks<STK_FXU, 1>(spl) : mix_blocks<STK_FXU, 0>(spl);
+    default: return newton ? mix_blocks<STK_DXU, 1>(spl) : mix_blocks<STK_DXU, 0>(spl);
+  }
+}
+double fold(int ker, int newton) {
+  switch (ker) {
+    case LAP_FXU: return newton == 0 ? 1.0 : (newton == 1 ? fold_factor<LAP_FXU, 1>() : fold_factor<LAP_FXU, 2>());
+    case LAP_DXU: return newton == 0 ? 1.0 : (newton == 1 ? fold_factor<LAP_DXU, 1>() : fold_factor<LAP_DXU, 2>());
+    case STK_DXU: return newton == 0 ? 1.0 : (newton == 1 ? fold_factor<STK_DXU, 1>() : fold_factor<STK_DXU, 2>());
+    default: return 1.0;
+  }
+}
+
+template <class T> int dev_upload(T** dst, const std::vector<T>& v) {
+  *dst = nullptr;
+  if (v.empty()) return SCTL_AMD_SUCCESS;
+  SCTL_HIP_TRY(hipMalloc(dst, sizeof(T) * v.size()));
+  SCTL_HIP_TRY(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return SCTL_AMD_SUCCESS;
+}
+
+}  // namespace
+}  // namespace sctl_amd
+
+using namespace sctl_amd;
+
+struct sctl_amd_plan_s {
+  int kernel = 0, precision = SCTL_AMD_FP64, newton = 2, K0 = 1, K1 = 1, ND = 0;
+  int device = 0;
+  int64_t nblocks = 0, nt_total = 0, ns_total = 0, interactions = 0;
+  int nitems = 0;
+  int spl = 2;          // sources per lane per stage
+  int grid = 0;         // persistent grid (workgroups)
+  bool has_geometry = false;
+  Item* d_items = nullptr;
+  int64_t *d_trg_dsp = nullptr, *d_trg_cnt = nullptr, *d_src_dsp = nullptr;
+  int32_t* d_src_cnt = nullptr;
+  int* d_counter = nullptr;
+  double *d_xt = nullptr, *d_xs = nullptr, *d_xn = nullptr;
+  float4 *d_xt32 = nullptr, *d_xs32 = nullptr;
+
+  ~sctl_amd_plan_s() {
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    if (cur != device) (void)hipSetDevice(device);
+    void* ptrs[] = {d_items, d_trg_dsp, d_trg_cnt, d_src_dsp, d_src_cnt, d_counter, d_xt, d_xs, d_xn, d_xt32, d_xs32};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+  }
+};
+
+namespace sctl_amd {
+
+// Plan knobs: sources per lane per stage and the persistent grid size
+// (workgroups per CU from the occupancy calculator).
+int plan_configure(sctl_amd_plan_s* p, int spl) {
+  DeviceInfo dev;
+  SCTL_TRY(device_info(&dev));
+  p->spl = spl;
+  int per_cu = 0;
+  if (p->precision == SCTL_AMD_MIXED) {
+    SCTL_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)mix_blocks(p->kernel, p->newton, spl), BWG, 0));
+  } else {
+    SCTL_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)f64_blocks(p->kernel, p->newton, spl), BWG, 0));
+  }
+  p->grid = std::max(1, per_cu) * dev.num_cus;
+  return SCTL_AMD_SUCCESS;
+}
+
+int plan_launch(sctl_amd_plan_s* p, const double* v_src, double* v_trg, hipStream_t s) {
+  if (p->nitems == 0) return SCTL_AMD_SUCCESS;
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, p->d_counter);
+  SCTL_HIP_TRY(hipGetLastError());
+  const double scale = kernel_scale(p->kernel) * fold(p->kernel, p->newton);
+  const int grid = std::min<int64_t>(p->grid, ((int64_t)p->nitems + 3) / 4);
+  if (p->precision == SCTL_AMD_MIXED) {
+    hipLaunchKernelGGL(mix_blocks(p->kernel, p->newton, p->spl), dim3(grid), dim3(BWG), 0, s, p->d_items, p->nitems,
+                       p->d_counter, p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt32, p->d_xs32, v_src, v_trg,
+                       scale);
+  } else {
+    hipLaunchKernelGGL(f64_blocks(p->kernel, p->newton, p->spl), dim3(grid), dim3(BWG), 0, s, p->d_items, p->nitems,
+                       p->d_counter, p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, v_src,
+                       v_trg, scale);
+  }
+  SCTL_HIP_TRY(hipGetLastError());
+  return SCTL_AMD_SUCCESS;
+}
+
+}  // namespace sctl_amd
 
 extern "C" {
-int sctl_amd_plan_create(sctl_amd_plan*, int, int, int, int64_t, const int64_t*, const int64_t*, const int64_t*,
-                         const int64_t*, int64_t, int64_t) {
-  sctl_amd::set_error("plan API not built yet");
-  return SCTL_AMD_ERROR_UNSUPPORTED;
+
+int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int digits, int64_t nblocks,
+                         const int64_t* trg_dsp, const int64_t* trg_cnt, const int64_t* src_dsp,
+                         const int64_t* src_cnt, int64_t nt_total, int64_t ns_total) {
+  SCTL_CHECK_ARG(plan, "plan is NULL");
+  *plan = nullptr;
+  int K0, K1, ND;
+  if (kernel_dims(kernel, &K0, &K1, &ND) != SCTL_AMD_SUCCESS) {
+    set_error("unknown kernel id " + std::to_string(kernel));
+    return SCTL_AMD_ERROR_UNKNOWN_KERNEL;
+  }
+  SCTL_CHECK_ARG(precision >= SCTL_AMD_FP64 && precision <= SCTL_AMD_AUTO, "unknown precision");
+  SCTL_CHECK_ARG(digits >= -1, "digits must be -1 or >= 0");
+  SCTL_CHECK_ARG(nblocks >= 0 && nt_total >= 0 && ns_total >= 0, "negative size");
+  SCTL_CHECK_ARG(nblocks == 0 || (trg_dsp && trg_cnt && src_dsp && src_cnt), "NULL block array");
+  if (precision == SCTL_AMD_AUTO) precision = (digits >= 0 && digits <= 7) ? SCTL_AMD_MIXED : SCTL_AMD_FP64;
+
+  std::vector<int64_t> order(nblocks);
+  std::iota(order.begin(), order.end(), 0);
+  for (int64_t b = 0; b < nblocks; b++) {
+    SCTL_CHECK_ARG(trg_cnt[b] >= 0 && src_cnt[b] >= 0 && trg_dsp[b] >= 0 && src_dsp[b] >= 0,
+                   "negative block offset or count (block " + std::to_string(b) + ")");
+    SCTL_CHECK_ARG(trg_dsp[b] + trg_cnt[b] <= nt_total, "block target range exceeds nt_total (block " + std::to_string(b) + ")");
+    SCTL_CHECK_ARG(src_dsp[b] + src_cnt[b] <= ns_total, "block source range exceeds ns_total (block " + std::to_string(b) + ")");
+    SCTL_CHECK_ARG(src_cnt[b] <= 0x7fffffffLL && trg_cnt[b] <= 0x7fffffffLL, "block too large");
+  }
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return trg_dsp[a] < trg_dsp[b]; });
+  for (size_t i = 1; i < order.size(); i++) {
+    const int64_t a = order[i - 1], b = order[i];
+    if (trg_cnt[a] && trg_cnt[b])
+      SCTL_CHECK_ARG(trg_dsp[a] + trg_cnt[a] <= trg_dsp[b], "target ranges of blocks overlap");
+  }
+
+  // Work items: chunks of <= 64 targets, lane groups for small chunks,
+  // longest first.
+  std::vector<Item> items;
+  std::vector<int64_t> cost;
+  int64_t inter = 0;
+  for (int64_t b = 0; b < nblocks; b++) {
+    if (!trg_cnt[b] || !src_cnt[b]) continue;
+    inter += trg_cnt[b] * src_cnt[b];
+    for (int64_t t0 = 0; t0 < trg_cnt[b]; t0 += 64) {
+      const int cnt = (int)std::min<int64_t>(64, trg_cnt[b] - t0);
+      const int lg = cnt <= 16 ? 2 : (cnt <= 32 ? 1 : 0);
+      items.push_back(Item{(int32_t)b, (int32_t)t0, cnt, lg});
+      cost.push_back((int64_t)(64 >> lg) * src_cnt[b]);
+    }
+  }
+  SCTL_CHECK_ARG(items.size() < 0x7fffffffULL, "too many work items");
+  std::vector<size_t> idx(items.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+  std::vector<Item> sorted(items.size());
+  for (size_t i = 0; i < idx.size(); i++) sorted[i] = items[idx[i]];
+
+  DeviceInfo dev;
+  SCTL_TRY(device_info(&dev));
+  auto* p = new sctl_amd_plan_s();
+  p->kernel = kernel;
+  p->precision = precision;
+  p->newton = precision == SCTL_AMD_MIXED ? 0 : newton_steps_f64(digits);
+  p->K0 = K0;
+  p->K1 = K1;
+  p->ND = ND;
+  p->device = dev.device;
+  p->nblocks = nblocks;
+  p->nt_total = nt_total;
+  p->ns_total = ns_total;
+  p->interactions = inter;
+  p->nitems = (int)sorted.size();
+  std::vector<int64_t> v_td(trg_dsp, trg_dsp + nblocks), v_tc(trg_cnt, trg_cnt + nblocks), v_sd(src_dsp, src_dsp + nblocks);
+  std::vector<int32_t> v_sc(nblocks);
+  for (int64_t b = 0; b < nblocks; b++) v_sc[b] = (int32_t)src_cnt[b];
+  int rc = SCTL_AMD_SUCCESS;
+  if (!rc) rc = dev_upload(&p->d_items, sorted);
+  if (!rc) rc = dev_upload(&p->d_trg_dsp, v_td);
+  if (!rc) rc = dev_upload(&p->d_trg_cnt, v_tc);
+  if (!rc) rc = dev_upload(&p->d_src_dsp, v_sd);
+  if (!rc) rc = dev_upload(&p->d_src_cnt, v_sc);
+  if (!rc) {
+    hipError_t e = hipMalloc(&p->d_counter, sizeof(int));
+    if (e != hipSuccess) {
+      set_error(std::string("hipMalloc(counter): ") + hipGetErrorString(e));
+      rc = SCTL_AMD_ERROR_OUT_OF_MEMORY;
+    }
+  }
+  if (!rc) rc = plan_configure(p, p->precision == SCTL_AMD_MIXED ? 2 : 1);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  *plan = p;
+  return SCTL_AMD_SUCCESS;
 }
-int sctl_amd_plan_set_geometry(sctl_amd_plan, const double*, const double*, const double*, uint32_t, void*) {
-  return SCTL_AMD_ERROR_UNSUPPORTED;
+
+int sctl_amd_plan_set_geometry(sctl_amd_plan p, const double* r_trg, const double* r_src, const double* n_src,
+                               uint32_t flags, void* stream) {
+  SCTL_CHECK_ARG(p, "plan is NULL");
+  SCTL_CHECK_ARG((flags & ~SCTL_AMD_HOST_PTRS) == 0, "unknown flag bits");
+  SCTL_CHECK_ARG(p->nt_total == 0 || r_trg, "r_trg is NULL");
+  SCTL_CHECK_ARG(p->ns_total == 0 || r_src, "r_src is NULL");
+  SCTL_CHECK_ARG(p->ns_total == 0 || !p->ND || n_src, "n_src is NULL for a kernel with a normal");
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread;
+  const hipMemcpyKind kind = (flags & SCTL_AMD_HOST_PTRS) ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  auto copy = [&](double** dst, const double* src, int64_t n) -> int {
+    if (n <= 0) return SCTL_AMD_SUCCESS;
+    if (!*dst) SCTL_HIP_TRY(hipMalloc(dst, sizeof(double) * n));
+    SCTL_HIP_TRY(hipMemcpyAsync(*dst, src, sizeof(double) * n, kind, s));
+    return SCTL_AMD_SUCCESS;
+  };
+  SCTL_TRY(copy(&p->d_xt, r_trg, p->nt_total * 3));
+  SCTL_TRY(copy(&p->d_xs, r_src, p->ns_total * 3));
+  if (p->ND) SCTL_TRY(copy(&p->d_xn, n_src, p->ns_total * 3));
+  if (p->precision == SCTL_AMD_MIXED && p->nblocks > 0) {
+    if (!p->d_xt32 && p->nt_total) SCTL_HIP_TRY(hipMalloc(&p->d_xt32, sizeof(float4) * p->nt_total));
+    if (!p->d_xs32 && p->ns_total) SCTL_HIP_TRY(hipMalloc(&p->d_xs32, sizeof(float4) * p->ns_total * (p->ND ? 2 : 1)));
+    hipLaunchKernelGGL(k_pack_blocks, dim3((unsigned)p->nblocks), dim3(BWG), 0, s, p->nblocks, p->d_trg_dsp,
+                       p->d_trg_cnt, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, p->ND ? 1 : 0, p->d_xt32,
+                       p->d_xs32);
+    SCTL_HIP_TRY(hipGetLastError());
+  }
+  if (flags & SCTL_AMD_HOST_PTRS) SCTL_HIP_TRY(hipStreamSynchronize(s));
+  p->has_geometry = true;
+  return SCTL_AMD_SUCCESS;
 }
-int sctl_amd_plan_apply(sctl_amd_plan, const double*, double*, uint32_t, void*) { return SCTL_AMD_ERROR_UNSUPPORTED; }
-int sctl_amd_plan_interactions(sctl_amd_plan, int64_t*) { return SCTL_AMD_ERROR_UNSUPPORTED; }
-int sctl_amd_plan_destroy(sctl_amd_plan) { return SCTL_AMD_SUCCESS; }
+
+int sctl_amd_plan_apply(sctl_amd_plan p, const double* v_src, double* v_trg, uint32_t flags, void* stream) {
+  SCTL_CHECK_ARG(p, "plan is NULL");
+  SCTL_CHECK_ARG(p->has_geometry, "plan has no geometry (call sctl_amd_plan_set_geometry first)");
+  SCTL_CHECK_ARG((flags & ~SCTL_AMD_HOST_PTRS) == 0, "unknown flag bits");
+  SCTL_CHECK_ARG(p->ns_total == 0 || v_src, "v_src is NULL");
+  SCTL_CHECK_ARG(p->nt_total == 0 || v_trg, "v_trg is NULL");
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread;
+  if (!(flags & SCTL_AMD_HOST_PTRS)) return plan_launch(p, v_src, v_trg, s);
+  Scratch bs, bt;
+  SCTL_TRY(bs.alloc(sizeof(double) * p->ns_total * p->K0, s));
+  SCTL_TRY(bt.alloc(sizeof(double) * p->nt_total * p->K1, s));
+  if (p->ns_total) SCTL_HIP_TRY(hipMemcpyAsync(bs.as<double>(), v_src, bs.bytes(), hipMemcpyHostToDevice, s));
+  if (p->nt_total) SCTL_HIP_TRY(hipMemcpyAsync(bt.as<double>(), v_trg, bt.bytes(), hipMemcpyHostToDevice, s));
+  SCTL_TRY(plan_launch(p, bs.as<double>(), bt.as<double>(), s));
+  if (p->nt_total) SCTL_HIP_TRY(hipMemcpyAsync(v_trg, bt.as<double>(), bt.bytes(), hipMemcpyDeviceToHost, s));
+  bs.release();
+  bt.release();
+  SCTL_HIP_TRY(hipStreamSynchronize(s));
+  return SCTL_AMD_SUCCESS;
 }
+
+int sctl_amd_plan_interactions(sctl_amd_plan p, int64_t* interactions) {
+  SCTL_CHECK_ARG(p && interactions, "NULL argument");
+  *interactions = p->interactions;
+  return SCTL_AMD_SUCCESS;
+}
+
+int sctl_amd_plan_destroy(sctl_amd_plan p) {
+  delete p;
+  return SCTL_AMD_SUCCESS;
+}
+
+}  // extern "C"

@@ -37,8 +37,6 @@ constexpr int TILE = 256;
 constexpr double PAD_F64 = 1e100;
 constexpr float PAD_F32 = 1e18f;
 
-template <int KER> constexpr int rec_f64() { return (src_words<KER>() + 1) & ~1; }  // 16-byte records
-template <int KER> constexpr int rec_f32() { return (src_words<KER>() + 3) & ~3; }  // 16-byte records
 
 // ------------------------------------------------------------------ FP64 Eval
 template <int KER, int TPL, int NEWTON>

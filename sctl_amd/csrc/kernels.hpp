@@ -36,6 +36,9 @@ template <> struct Ker<STK_DXU> { static constexpr int K0 = 3, K1 = 3, ND = 3, P
 
 // Words per packed source record: position, normal (if any), density.
 template <int KER> constexpr int src_words() { return 3 + Ker<KER>::ND + Ker<KER>::K0; }
+// Record widths padded to 16 bytes (LDS reads of 128 bits).
+template <int KER> constexpr int rec_f64() { return (src_words<KER>() + 1) & ~1; }
+template <int KER> constexpr int rec_f32() { return (src_words<KER>() + 3) & ~3; }
 
 __device__ __forceinline__ double hw_rsq(double x) { return __builtin_amdgcn_rsq(x); }
 __device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }

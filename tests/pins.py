@@ -1,0 +1,267 @@
+"""Checks that pin the oracle (oracle/direct_sum.c) to the reference.
+
+Each check takes an ``oracle.Oracle`` and returns (ok, detail).  The test
+suite runs every check on the real oracle (all must pass) and on each
+deliberately mutated build (each mutant must fail at least one), so a
+plausible mistake -- a dropped term, a wrong sign, a transposed layout, a
+wrong constant, a missing Newton step, a skipped source -- cannot slip past.
+
+Pins, with what each rests on:
+  golden_exact      bit-exact against the reference's scalar build (golden
+                    vectors written by tests/golden/make_golden.py)
+  golden_avx512     the AVX-512 restatement against the reference's AVX-512
+                    build, <= 2e-15 normwise, digits -1, 3, 7, 11
+  golden_rsqrt      approx_rsqrt<digits> bit-exact against the reference
+  rsqrt_bounds      vec-test.hpp:431-465: |err| < 2^-11 (digits 4), < 2^-22
+                    (digits 7), masked lanes exactly 0
+  closed_forms      test-generic-kernel.cpp:55-103, test-kernel_functions.cpp:35-131
+  long_double       long-double brute force (independent of SCTL rounding)
+  eval_vs_matrix    Eval == KernelMatrix^T v, test-kernel_functions.cpp:487-526
+  surface_identities Gauss / Stokes layer identities on a spectral sphere
+                    quadrature: Laplace DxU(1) = -1 inside, 0 outside;
+                    Laplace FxU(1) = 1 inside, 1/|x| outside; Stokes DxU(f) =
+                    -f inside, 0 outside; Stokes FxU(f) = (2/3) f inside
+  linearity         Eval(a f + b g) == a Eval(f) + b Eval(g), superposition
+                    of two sources (test-generic-kernel.cpp:89-103)
+"""
+import math
+import os
+
+import numpy as np
+
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sctl_ref_vectors.npz")
+PI = math.pi
+
+
+def golden():
+    return np.load(GOLDEN, allow_pickle=False)
+
+
+def _nrm(x, ref):
+    d = np.abs(np.asarray(x) - np.asarray(ref)).max()
+    den = np.abs(ref).max()
+    return d / den if den > 0 else d
+
+
+def small_inputs(K0, ND):
+    Ns, Nt = 7, 5
+    Xs = 0.1 * (np.arange(Ns * 3) + 1) - 0.35
+    Vs = 0.3 * (np.arange(Ns * K0) % 5) - 0.4
+    Xt = 1.7 + 0.2 * np.arange(Nt * 3)
+    Xn = np.tile([0.0, 1.0, 0.0], Ns) if ND else np.zeros(0)
+    return Xt, Xs, Xn, Vs
+
+
+def golden_exact(orc):
+    G = golden()
+    bad = []
+    for k in range(4):
+        _, K0, K1, ND, _ = orc.dims(k)
+        Xt, Xs, Xn, Vs = small_inputs(K0, ND)
+        if not np.array_equal(orc.eval(k, Xt, Xs, Xn, Vs), G["k%d_small_eval_scalar" % k]):
+            bad.append("k%d small eval" % k)
+        if not np.array_equal(orc.kernel_matrix(k, Xt, Xs, Xn).ravel(), G["k%d_small_kmat_scalar" % k].ravel()):
+            bad.append("k%d small kmat" % k)
+        xt, xs, n, v = (G["k%d_rand_%s" % (k, a)] for a in ("xt", "xs", "n", "v"))
+        if not np.array_equal(orc.eval(k, xt, xs, n, v), G["k%d_rand_eval_scalar" % k]):
+            bad.append("k%d rand eval" % k)
+        vt = orc.eval(k, G["k%d_veclen_xt" % k], xs, n, v, v_trg=G["k%d_veclen_vt0" % k])
+        if not np.array_equal(vt, G["k%d_veclen_eval_scalar" % k]):
+            bad.append("k%d veclen accumulate" % k)
+        xt11, xs13, n13 = (G["k%d_kmat_%s" % (k, a)] for a in ("xt", "xs", "n"))
+        for tag, a, b, c in (("kmat", xt11, xs13, n13), ("kmat1t", xt11[:3], xs13, n13),
+                             ("kmat1s", xt11, xs13[:3], n13[:3])):
+            M = orc.kernel_matrix(k, a, b, c)
+            ref = G["k%d_%s_scalar" % (k, tag)]
+            if M.shape != ref.shape or not np.array_equal(M, ref):
+                bad.append("k%d %s" % (k, tag))
+    return not bad, ", ".join(bad)
+
+
+def golden_avx512(orc):
+    if not orc.have_avx512():
+        return True, "skipped: no AVX-512 on this CPU"
+    G = golden()
+    worst, bad = 0.0, []
+    for k in range(4):
+        xt, xs, n, v = (G["k%d_rand_%s" % (k, a)] for a in ("xt", "xs", "n", "v"))
+        for d, key in ((-1, "k%d_rand_eval_v4" % k), (3, "k%d_rand_eval_v4_d3" % k), (7, "k%d_rand_eval_v4_d7" % k),
+                       (11, "k%d_rand_eval_v4_d11" % k)):
+            e = _nrm(orc.eval(k, xt, xs, n, v, mode=oracle.AVX512, digits=d), G[key])
+            worst = max(worst, e)
+            if e > 2e-15:
+                bad.append("k%d digits %d: %.2e" % (k, d, e))
+    return not bad, "; ".join(bad) or "max %.2e" % worst
+
+
+def golden_rsqrt(orc):
+    if not orc.have_avx512():
+        return True, "skipped: no AVX-512 on this CPU"
+    G = golden()
+    x = G["rsqrt_x"]
+    bad = [d for d in (4, 7, 11, -1) if not np.array_equal(orc.rsqrt(x, mode=oracle.AVX512, digits=d),
+                                                           G["rsqrt_v4_d%d" % d])]
+    return not bad, "digits %s differ" % bad if bad else ""
+
+
+def rsqrt_bounds(orc):
+    if not orc.have_avx512():
+        return True, "skipped: no AVX-512 on this CPU"
+    rng = np.random.default_rng(7)
+    x = rng.integers(1, 2 ** 31, 4096).astype(np.float64)  # rand() values as in vec-test.hpp:432-436
+    exact = 1 / np.sqrt(x)
+    bad = []
+    for d, bound in ((4, 2.0 ** -11), (7, 2.0 ** -22)):
+        y = orc.rsqrt(x, mode=oracle.AVX512, digits=d)
+        e = np.max(np.abs(y - exact) / exact)
+        if not e < bound:
+            bad.append("digits %d: %.2e >= %.2e" % (d, e, bound))
+    z = orc.rsqrt(np.array([0.0, -1.0, 0.0]), mode=oracle.AVX512, digits=7)
+    if np.any(z != 0):
+        bad.append("masked lanes not zero")
+    z = orc.rsqrt(np.array([0.0, -1.0]), mode=oracle.EXACT)
+    if np.any(z != 0):
+        bad.append("exact masked lanes not zero")
+    return not bad, "; ".join(bad)
+
+
+def closed_forms(orc):
+    tol = 1e-9
+    bad = []
+    sf = 1 / (4 * PI)
+
+    def ok(a, b):
+        return abs(a - b) <= tol * max(1.0, abs(b))
+
+    # test-generic-kernel.cpp:55-103
+    M = orc.kernel_matrix(0, [1, 0, 0], [0, 0, 0], None)
+    if M.shape != (1, 1) or not ok(M[0, 0], sf):
+        bad.append("Laplace FxU 1x1 matrix")
+    v = orc.eval(0, [1, 0, 0, 2, 0, 0, 3, 0, 0, 5, 0, 0], [0, 0, 0], None, [1.0])
+    if not all(ok(v[i], sf / d) for i, d in enumerate((1, 2, 3, 5))):
+        bad.append("Laplace FxU distances")
+    v = orc.eval(0, [2, 0, 0], [0, 0, 0, 4, 0, 0], None, [1.0, 1.0])
+    if not ok(v[0], sf / 2 + sf / 2):
+        bad.append("Laplace FxU superposition")
+    # test-kernel_functions.cpp:35-131
+    if not ok(orc.eval(0, [2, 0, 0], [0, 0, 0], None, [3.0])[0], 3.0 / (4 * PI * 2.0)):
+        bad.append("Laplace FxU 3/(8 pi)")
+    if not ok(orc.eval(1, [2, 0, 0], [0, 0, 0], [1, 0, 0], [1.0])[0], 1.0 / (16 * PI)):
+        bad.append("Laplace DxU 1/(16 pi)")
+    u = orc.eval(2, [2.0, 0, 0], [0, 0, 0], None, [1.0, 0, 0])
+    if not (ok(u[0], 1.0 / (4 * PI * 2.0)) and abs(u[1]) < 1e-12 and abs(u[2]) < 1e-12):
+        bad.append("Stokeslet u_x = 1/(4 pi r)")
+    # Stokes DxU closed form: 3/(4pi) r (f.r)(r.n)/r^5 at r=(1,2,2), n=(0,0,1), f=(1,0,0)
+    r = np.array([1.0, 2.0, 2.0])
+    u = orc.eval(3, r, [0, 0, 0], [0, 0, 1], [1.0, 0, 0])
+    want = 3 / (4 * PI) * r * (r[0]) * (r[2]) / np.linalg.norm(r) ** 5
+    if not np.allclose(u, want, rtol=tol, atol=0):
+        bad.append("Stokes DxU closed form")
+    for k, K0, ND in ((0, 1, 0), (1, 1, 3), (2, 3, 0), (3, 3, 3)):
+        if orc.dims(k)[1:4] != (K0, K0, ND):
+            bad.append("dims of kernel %d" % k)
+    for k, s in ((0, 1 / (4 * PI)), (1, 1 / (4 * PI)), (2, 1 / (8 * PI)), (3, 3 / (4 * PI))):
+        if not abs(orc.scale(k) - s) <= 1e-15 * s:
+            bad.append("scale of kernel %d" % k)
+    return not bad, "; ".join(bad)
+
+
+def long_double(orc):
+    rng = np.random.default_rng(11)
+    bad, worst = [], 0.0
+    for k in range(4):
+        _, K0, K1, ND, _ = orc.dims(k)
+        xs, xt = rng.random(400 * 3), rng.random(300 * 3)
+        xt[:30] = xs[:30]
+        n = rng.standard_normal(400 * 3)
+        v = rng.uniform(-1, 1, 400 * K0)
+        a = orc.eval(k, xt, xs, n, v)
+        b = orc.eval(k, xt, xs, n, v, mode=oracle.LDOUBLE)
+        e = _nrm(a, b)
+        worst = max(worst, e)
+        if e > 1e-13:
+            bad.append("k%d %.2e" % (k, e))
+    return not bad, "; ".join(bad) or "max %.2e" % worst
+
+
+def eval_vs_matrix(orc):
+    bad = []
+    for k in range(4):
+        _, K0, K1, ND, _ = orc.dims(k)
+        Xt, Xs, Xn, Vs = small_inputs(K0, ND)
+        vt = orc.eval(k, Xt, Xs, Xn, Vs)
+        M = orc.kernel_matrix(k, Xt, Xs, Xn)
+        if M.shape != (7 * K0, 5 * K1) or not np.allclose(vt, M.T @ Vs, rtol=1e-10, atol=1e-14):
+            bad.append("k%d" % k)
+    return not bad, ", ".join(bad)
+
+
+def sphere_quadrature(p):
+    """Gauss-Legendre in cos(theta) x trapezoid in phi on the unit sphere."""
+    z, w = np.polynomial.legendre.leggauss(p)
+    m = 2 * p
+    phi = 2 * np.pi * np.arange(m) / m
+    Z = np.repeat(z, m)
+    PHI = np.tile(phi, p)
+    s = np.sqrt(1 - Z * Z)
+    X = np.stack([s * np.cos(PHI), s * np.sin(PHI), Z], axis=1)
+    W = np.repeat(w, m) * (2 * np.pi / m)
+    return X, W
+
+
+def surface_identities(orc, p=40):
+    X, W = sphere_quadrature(p)
+    xs = X.ravel()
+    n = X.ravel()
+    rng = np.random.default_rng(3)
+    d = rng.standard_normal((6, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    inside = (d * np.array([0.0, 0.1, 0.2, 0.3, 0.4, 0.5])[:, None]).ravel()
+    outside = (d * np.array([2.0, 2.5, 3.0, 4.0, 6.0, 10.0])[:, None]).ravel()
+    rad_out = np.linalg.norm(outside.reshape(-1, 3), axis=1)
+    tol = 1e-10
+    bad = []
+    u = orc.eval(1, inside, xs, n, W)
+    if np.abs(u + 1).max() > tol:
+        bad.append("Laplace DxU inside %.1e" % np.abs(u + 1).max())
+    u = orc.eval(1, outside, xs, n, W)
+    if np.abs(u).max() > tol:
+        bad.append("Laplace DxU outside")
+    u = orc.eval(0, inside, xs, None, W)
+    if np.abs(u - 1).max() > tol:
+        bad.append("Laplace FxU inside")
+    u = orc.eval(0, outside, xs, None, W)
+    if np.abs(u - 1 / rad_out).max() > tol:
+        bad.append("Laplace FxU outside")
+    f = np.array([0.3, -1.1, 0.7])
+    F = (W[:, None] * f[None, :]).ravel()
+    u = orc.eval(3, inside, xs, n, F).reshape(-1, 3)
+    if np.abs(u + f).max() > tol:
+        bad.append("Stokes DxU inside")
+    u = orc.eval(3, outside, xs, n, F).reshape(-1, 3)
+    if np.abs(u).max() > tol:
+        bad.append("Stokes DxU outside")
+    u = orc.eval(2, inside, xs, None, F).reshape(-1, 3)
+    if np.abs(u - 2.0 / 3.0 * f).max() > tol:
+        bad.append("Stokes FxU inside")
+    return not bad, "; ".join(bad)
+
+
+def linearity(orc):
+    rng = np.random.default_rng(5)
+    bad = []
+    for k in range(4):
+        _, K0, K1, ND, _ = orc.dims(k)
+        xs, xt, n = rng.random(64 * 3), rng.random(50 * 3), rng.standard_normal(64 * 3)
+        f, g = rng.uniform(-1, 1, 64 * K0), rng.uniform(-1, 1, 64 * K0)
+        a = orc.eval(k, xt, xs, n, 2.0 * f - 0.5 * g)
+        b = 2.0 * orc.eval(k, xt, xs, n, f) - 0.5 * orc.eval(k, xt, xs, n, g)
+        if _nrm(a, b) > 1e-13:
+            bad.append("k%d" % k)
+    return not bad, ", ".join(bad)
+
+
+CHECKS = [golden_exact, golden_avx512, golden_rsqrt, rsqrt_bounds, closed_forms, long_double, eval_vs_matrix,
+          surface_identities, linearity]
