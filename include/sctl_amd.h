@@ -143,6 +143,40 @@ int sctl_amd_plan_interactions(sctl_amd_plan plan, int64_t* interactions);
 
 int sctl_amd_plan_destroy(sctl_amd_plan plan);
 
+/* ------------------------------------------------------ repeated applies
+ * A Krylov solve applies the same operator once per iteration with new
+ * densities (lin-solve.txx:146-156 calling BoundaryIntegralOp::
+ * ComputePotential, boundary_integral.txx:716-722).  This captures nsteps
+ * applications of a plan as one hipGraph: for step s, v_src is overwritten
+ * with N(0,1) draws of the counter-based generator for (seed + s, stream_id)
+ * (as sctl_amd_gen_normal), then v_trg += apply(v_src).  v_src (ns_total*K0)
+ * and v_trg (nt_total*K1) are device buffers that the graph keeps using on
+ * every launch.  The graph allocates nothing and contains no memset. */
+typedef struct sctl_amd_graph_s* sctl_amd_graph;
+
+int sctl_amd_plan_graph_create(sctl_amd_graph* graph, sctl_amd_plan plan, int nsteps, uint64_t seed,
+                               uint64_t stream_id, double* v_src, double* v_trg);
+int sctl_amd_graph_launch(sctl_amd_graph graph, void* stream);
+int sctl_amd_graph_interactions(sctl_amd_graph graph, int64_t* interactions);
+int sctl_amd_graph_destroy(sctl_amd_graph graph);
+
+/* Device counter-based generator: out[i] = i-th N(0,1) draw of
+ * (seed, stream_id) -- SplitMix64 counters, Box-Muller pairs. */
+int sctl_amd_gen_normal(uint64_t seed, uint64_t stream_id, int64_t n, double* out, void* stream);
+
+/* -------------------------------------------------------------- profiling
+ * Counterpart of the reference's Profile FLOP counter, which Eval and
+ * KernelMatrix increment by Ns*Nt*FLOPS() on every call
+ * (generic-kernel.txx:210,224; profile.txx:466-473).  The library always
+ * counts the interactions and the reference-convention FLOPs of the work it
+ * launches.  While timing is enabled it also records a pair of HIP events, on
+ * the caller's stream, around every main summation kernel (dense Eval,
+ * KernelMatrix, plan apply); sctl_amd_profile_read waits for them and adds
+ * their elapsed times. */
+int sctl_amd_profile_enable(int enable);
+int sctl_amd_profile_read(int64_t* interactions, int64_t* flops, int64_t* timed_launches, double* kernel_ms);
+int sctl_amd_profile_reset(void);
+
 /* -------------------------------------------------------- device plumbing
  * Minimal device-memory and stream helpers for hosts that have no HIP
  * allocator of their own (bindings, tests).  kind: 0 host->device,

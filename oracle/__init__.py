@@ -29,14 +29,14 @@ _fp = ctypes.POINTER(ctypes.c_float)
 _L = ctypes.c_long
 
 
-def build(with_ref=None):
+def build(with_ref=None, force=False):
     """Compile liboracle.so (and the _ref libraries when the reference is present)."""
     targets = ["all"]
     if with_ref is None:
         with_ref = os.path.isdir("/root/reference/include")
     if with_ref:
         targets.append("ref")
-    subprocess.check_call(["make", "-s", "-C", HERE] + targets)
+    subprocess.check_call(["make", "-s", "-C", HERE] + (["-B"] if force else []) + targets)
 
 
 def _p(a):

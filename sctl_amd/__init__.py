@@ -36,7 +36,9 @@ SYMBOLS = [
     "sctl_amd_init", "sctl_amd_last_error", "sctl_amd_version", "sctl_amd_kernel_info", "sctl_amd_eval",
     "sctl_amd_kernel_matrix", "sctl_amd_plan_create", "sctl_amd_plan_set_geometry", "sctl_amd_plan_apply",
     "sctl_amd_plan_interactions", "sctl_amd_plan_destroy", "sctl_amd_malloc", "sctl_amd_free", "sctl_amd_memcpy",
-    "sctl_amd_stream_create", "sctl_amd_stream_destroy", "sctl_amd_stream_synchronize",
+    "sctl_amd_stream_create", "sctl_amd_stream_destroy", "sctl_amd_stream_synchronize", "sctl_amd_profile_enable",
+    "sctl_amd_profile_read", "sctl_amd_profile_reset", "sctl_amd_plan_graph_create", "sctl_amd_graph_launch",
+    "sctl_amd_graph_interactions", "sctl_amd_graph_destroy", "sctl_amd_gen_normal",
 ]
 
 
@@ -82,6 +84,13 @@ def lib():
     L.sctl_amd_stream_create.argtypes = [ctypes.POINTER(_vp)]
     L.sctl_amd_stream_destroy.argtypes = [_vp]
     L.sctl_amd_stream_synchronize.argtypes = [_vp]
+    L.sctl_amd_plan_graph_create.argtypes = [ctypes.POINTER(_vp), _vp, i32, ctypes.c_uint64, ctypes.c_uint64, _vp, _vp]
+    L.sctl_amd_graph_launch.argtypes = [_vp, _vp]
+    L.sctl_amd_graph_interactions.argtypes = [_vp, _i64p]
+    L.sctl_amd_graph_destroy.argtypes = [_vp]
+    L.sctl_amd_gen_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, _vp, _vp]
+    L.sctl_amd_profile_enable.argtypes = [i32]
+    L.sctl_amd_profile_read.argtypes = [_i64p, _i64p, _i64p, ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 
@@ -141,6 +150,22 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def profile_enable(on=True):
+    check(lib().sctl_amd_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(lib().sctl_amd_profile_reset())
+
+
+def profile_read():
+    """dict(interactions, flops, timed_launches, kernel_ms); waits for pending timing events."""
+    a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    d = ctypes.c_double()
+    check(lib().sctl_amd_profile_read(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
+    return dict(interactions=a.value, flops=b.value, timed_launches=c.value, kernel_ms=d.value)
 
 
 def synchronize(stream=None):
@@ -291,6 +316,10 @@ class Plan:
         check(lib().sctl_amd_plan_interactions(self.handle, ctypes.byref(n)))
         return n.value
 
+    def graph(self, nsteps, seed, stream_id, v_src, v_trg):
+        """Captures nsteps applies with densities regenerated on the device (see Graph)."""
+        return Graph(self, nsteps, seed, stream_id, v_src, v_trg)
+
     def destroy(self):
         if self.handle:
             lib().sctl_amd_plan_destroy(self.handle)
@@ -301,3 +330,41 @@ class Plan:
             self.destroy()
         except Exception:
             pass
+
+
+class Graph:
+    """nsteps applications of a plan captured as one hipGraph: for step s the
+    device buffer v_src is refilled with N(0,1) draws of (seed + s, stream_id),
+    then v_trg += apply(v_src)."""
+
+    def __init__(self, plan, nsteps, seed, stream_id, v_src, v_trg):
+        h = _vp()
+        check(lib().sctl_amd_plan_graph_create(ctypes.byref(h), plan.handle, nsteps, seed, stream_id, v_src.ptr,
+                                               v_trg.ptr))
+        self.handle = h
+        self._keep = (plan, v_src, v_trg)
+
+    def launch(self, stream=None):
+        check(lib().sctl_amd_graph_launch(self.handle, stream))
+
+    def interactions(self):
+        n = ctypes.c_int64()
+        check(lib().sctl_amd_graph_interactions(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def destroy(self):
+        if self.handle:
+            lib().sctl_amd_graph_destroy(self.handle)
+            self.handle = _vp()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def gen_normal(seed, stream_id, out, stream=None):
+    """Device generator: fills DeviceArray `out` with N(0,1) draws of (seed, stream_id)."""
+    check(lib().sctl_amd_gen_normal(seed, stream_id, out.nbytes // 8, out.ptr, stream))
+    return out

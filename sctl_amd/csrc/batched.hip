@@ -15,10 +15,10 @@
 //    so a 16-target block still keeps all 64 lanes busy.
 //  * The block's sources are staged through a wave-private LDS region,
 //    64*SPL records at a time (no workgroup barrier: waves run independently).
-//  * Mixed precision: sources and targets are stored once per geometry as
-//    FP32 offsets from the block's source bounding-box centre (differenced in
-//    FP64); each stage's FP32 partial is added into FP64 sums with Kahan
-//    compensation.  FP64: plain FP64 records, as in the dense path.
+//  * Mixed precision: each pair is differenced in FP64 and the difference
+//    rounded to FP32, the rest of the interaction is FP32; each stage's FP32
+//    partial is added into FP64 sums with Kahan compensation.  FP64: plain
+//    FP64 records, as in the dense path.
 //
 // Every wave leaves the loop when the counter passes the number of items, so
 // the grid always drains.  An apply allocates nothing and issues no memset
@@ -40,7 +40,6 @@ namespace sctl_amd {
 namespace {
 
 constexpr int BWG = 256;  // 4 waves per workgroup, each wave independent
-constexpr float PADF = 1e18f;
 constexpr double PADD = 1e100;
 
 struct Item {
@@ -159,21 +158,27 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
 }
 
 // ------------------------------------------------------------ mixed blocks
-// xs32: per source {x-c, y-c, z-c, 0[, nx, ny, nz, 0]} (Q float4 per source);
-// xt32: per target {x-c, y-c, z-c, 0} relative to its block's centre.
+// Near-field blocks put most targets close to their block's sources, so the
+// mixed path differences every pair in FP64 and rounds the difference to
+// FP32 (accurate to 2^-24 relative to r however small r is); the rest of the
+// interaction is FP32 with v_rsq_f32.  The stage's FP32 partial is added into
+// FP64 sums with Kahan compensation; a non-finite partial (r == 0 in the
+// stage) is redone with the r^2 > 0 mask.
 template <int KER, int NEWTON, int SPL>
 __global__ void __launch_bounds__(BWG)
 k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ counter,
                const int64_t* __restrict__ trg_dsp, const int64_t* __restrict__ src_dsp,
-               const int32_t* __restrict__ src_cnt, const float4* __restrict__ xt32,
-               const float4* __restrict__ xs32, const double* __restrict__ fs, double* __restrict__ out,
+               const int32_t* __restrict__ src_cnt, const double* __restrict__ xt, const double* __restrict__ xs,
+               const double* __restrict__ xn, const double* __restrict__ fs, double* __restrict__ out,
                double scale) {
   using K = Ker<KER>;
-  constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, R = rec_f32<KER>(), STAGE = 64 * SPL;
-  constexpr int QG = ND ? 2 : 1;  // float4 per source in xs32
-  __shared__ float lds_all[(BWG / 64) * STAGE * R];
+  constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, STAGE = 64 * SPL;
+  constexpr int AW = ((ND + K0) + 3) & ~3;  // float words of normal + density
+  __shared__ double pos_all[(BWG / 64) * STAGE * 4];
+  __shared__ float aux_all[(BWG / 64) * STAGE * AW];
   const int lane = threadIdx.x & 63;
-  float* lds = lds_all + (threadIdx.x >> 6) * (STAGE * R);
+  double* pos = pos_all + (threadIdx.x >> 6) * (STAGE * 4);
+  float* aux = aux_all + (threadIdx.x >> 6) * (STAGE * AW);
 
   for (int it = next_item(counter, lane); it < nitems; it = next_item(counter, lane)) {
     const Item I = items[it];
@@ -181,7 +186,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
     const int g = lane >> (6 - lg), tl = lane & (L - 1);
     const bool tv = tl < I.t_cnt;
     const int64_t t = trg_dsp[I.block] + I.t_off + (tv ? tl : 0);
-    const float4 tp = xt32[t];
+    const double tx = xt[t * 3 + 0], ty = xt[t * 3 + 1], tz = xt[t * 3 + 2];
     const int64_t sb = src_dsp[I.block];
     const int nsb = src_cnt[I.block];
     double acc[K1], comp[K1];
@@ -194,25 +199,21 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
 #pragma unroll
       for (int q = 0; q < SPL; q++) {
         const int i = lane + 64 * q;
-        float* rec = lds + i * R;
+        double* pr = pos + i * 4;
+        float* ar = aux + i * AW;
         if (i < len) {
           const int64_t s = sb + s0 + i;
-          const float4 a = xs32[s * QG];
-          rec[0] = a.x;
-          rec[1] = a.y;
-          rec[2] = a.z;
-          if constexpr (ND) {
-            const float4 b = xs32[s * QG + 1];
-            rec[3] = b.x;
-            rec[4] = b.y;
-            rec[5] = b.z;
-          }
+          pr[0] = xs[s * 3 + 0];
+          pr[1] = xs[s * 3 + 1];
+          pr[2] = xs[s * 3 + 2];
 #pragma unroll
-          for (int k = 0; k < K0; k++) rec[3 + ND + k] = (float)fs[s * K0 + k];
+          for (int k = 0; k < ND; k++) ar[k] = (float)xn[s * 3 + k];
+#pragma unroll
+          for (int k = 0; k < K0; k++) ar[ND + k] = (float)fs[s * K0 + k];
         } else {
-          rec[0] = rec[1] = rec[2] = PADF;
+          pr[0] = pr[1] = pr[2] = PADD;
 #pragma unroll
-          for (int k = 3; k < R; k++) rec[k] = 0.0f;
+          for (int k = 0; k < AW; k++) ar[k] = 0.0f;
         }
       }
       wave_sync();
@@ -221,13 +222,14 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
       for (int k = 0; k < K1; k++) p[k] = 0.0f;
 #pragma unroll 4
       for (int i = g; i < STAGE; i += G) {
-        const float* rec = lds + i * R;
+        const double* pr = pos + i * 4;
+        const float* ar = aux + i * AW;
         float n[3] = {0, 0, 0}, f[3];
 #pragma unroll
-        for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
+        for (int k = 0; k < ND; k++) n[k] = ar[k];
 #pragma unroll
-        for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
-        interact<KER, NEWTON, false>(p, tp.x - rec[0], tp.y - rec[1], tp.z - rec[2], n, f);
+        for (int k = 0; k < K0; k++) f[k] = ar[ND + k];
+        interact<KER, NEWTON, false>(p, (float)(tx - pr[0]), (float)(ty - pr[1]), (float)(tz - pr[2]), n, f);
       }
       bool bad = false;
 #pragma unroll
@@ -235,12 +237,13 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
       if (bad) {
 #pragma unroll
         for (int k = 0; k < K1; k++) p[k] = 0.0f;
-        for (int i = g; i < STAGE; i += G) {
-          const float* rec = lds + i * R;
+        for (int i = g; i < len; i += G) {
+          const double* pr = pos + i * 4;
+          const float* ar = aux + i * AW;
           float n[3] = {0, 0, 0}, f[3];
-          for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
-          for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
-          interact<KER, NEWTON, true>(p, tp.x - rec[0], tp.y - rec[1], tp.z - rec[2], n, f);
+          for (int k = 0; k < ND; k++) n[k] = ar[k];
+          for (int k = 0; k < K0; k++) f[k] = ar[ND + k];
+          interact<KER, NEWTON, true>(p, (float)(tx - pr[0]), (float)(ty - pr[1]), (float)(tz - pr[2]), n, f);
         }
       }
 #pragma unroll
@@ -264,65 +267,9 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
   }
 }
 
-// Per block: bounding-box centre of its sources (FP64), then FP32 offsets of
-// its sources and targets.  One workgroup per block.
-__global__ void __launch_bounds__(BWG)
-k_pack_blocks(int64_t nblocks, const int64_t* __restrict__ trg_dsp, const int64_t* __restrict__ trg_cnt,
-              const int64_t* __restrict__ src_dsp, const int32_t* __restrict__ src_cnt,
-              const double* __restrict__ xt, const double* __restrict__ xs, const double* __restrict__ xn,
-              int has_normal, float4* __restrict__ xt32, float4* __restrict__ xs32) {
-  __shared__ double red[6][BWG];
-  const int tid = threadIdx.x;
-  const int64_t b = blockIdx.x;
-  if (b >= nblocks) return;
-  const int64_t sb = src_dsp[b];
-  const int ns = src_cnt[b];
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int i = tid; i < ns; i += BWG)
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const double v = xs[(sb + i) * 3 + k];
-      lo[k] = fmin(lo[k], v);
-      hi[k] = fmax(hi[k], v);
-    }
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    red[k][tid] = lo[k];
-    red[3 + k][tid] = hi[k];
-  }
-  __syncthreads();
-  for (int w = BWG / 2; w > 0; w >>= 1) {
-    if (tid < w)
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        red[k][tid] = fmin(red[k][tid], red[k][tid + w]);
-        red[3 + k][tid] = fmax(red[3 + k][tid], red[3 + k][tid + w]);
-      }
-    __syncthreads();
-  }
-  double c[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) c[k] = ns > 0 ? 0.5 * (red[k][0] + red[3 + k][0]) : 0.0;
-  const int QG = has_normal ? 2 : 1;
-  for (int i = tid; i < ns; i += BWG) {
-    const int64_t s = sb + i;
-    xs32[s * QG] = make_float4((float)(xs[s * 3 + 0] - c[0]), (float)(xs[s * 3 + 1] - c[1]),
-                               (float)(xs[s * 3 + 2] - c[2]), 0.0f);
-    if (has_normal)
-      xs32[s * QG + 1] = make_float4((float)xn[s * 3 + 0], (float)xn[s * 3 + 1], (float)xn[s * 3 + 2], 0.0f);
-  }
-  const int64_t tb = trg_dsp[b], nt = trg_cnt[b];
-  for (int64_t i = tid; i < nt; i += BWG) {
-    const int64_t t = tb + i;
-    xt32[t] = make_float4((float)(xt[t * 3 + 0] - c[0]), (float)(xt[t * 3 + 1] - c[1]),
-                          (float)(xt[t * 3 + 2] - c[2]), 0.0f);
-  }
-}
-
 using BlocksF64Fn = void (*)(const Item*, int, int*, const int64_t*, const int64_t*, const int32_t*, const double*,
                              const double*, const double*, const double*, double*, double);
-using BlocksMixFn = void (*)(const Item*, int, int*, const int64_t*, const int64_t*, const int32_t*, const float4*,
-                             const float4*, const double*, double*, double);
+using BlocksMixFn = BlocksF64Fn;
 
 template <int KER, int NEWTON> BlocksF64Fn f64_blocks(int spl) {
   return spl >= 4 ? k_blocks_f64<KER, NEWTON, 4> : (spl >= 2 ? k_blocks_f64<KER, NEWTON, 2> : k_blocks_f64<KER, NEWTON, 1>);
@@ -385,13 +332,12 @@ struct sctl_amd_plan_s {
   int32_t* d_src_cnt = nullptr;
   int* d_counter = nullptr;
   double *d_xt = nullptr, *d_xs = nullptr, *d_xn = nullptr;
-  float4 *d_xt32 = nullptr, *d_xs32 = nullptr;
 
   ~sctl_amd_plan_s() {
     int cur = -1;
     (void)hipGetDevice(&cur);
     if (cur != device) (void)hipSetDevice(device);
-    void* ptrs[] = {d_items, d_trg_dsp, d_trg_cnt, d_src_dsp, d_src_cnt, d_counter, d_xt, d_xs, d_xn, d_xt32, d_xs32};
+    void* ptrs[] = {d_items, d_trg_dsp, d_trg_cnt, d_src_dsp, d_src_cnt, d_counter, d_xt, d_xs, d_xn};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
@@ -416,16 +362,31 @@ int plan_configure(sctl_amd_plan_s* p, int spl) {
   return SCTL_AMD_SUCCESS;
 }
 
+int plan_sizes(sctl_amd_plan p, int64_t* ns, int64_t* nt, int* K0, int* K1, int64_t* interactions) {
+  SCTL_CHECK_ARG(p, "plan is NULL");
+  *ns = p->ns_total;
+  *nt = p->nt_total;
+  *K0 = p->K0;
+  *K1 = p->K1;
+  *interactions = p->interactions;
+  return SCTL_AMD_SUCCESS;
+}
+int plan_kernel(sctl_amd_plan p) { return p ? p->kernel : -1; }
+
 int plan_launch(sctl_amd_plan_s* p, const double* v_src, double* v_trg, hipStream_t s) {
   if (p->nitems == 0) return SCTL_AMD_SUCCESS;
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, p->d_counter);
   SCTL_HIP_TRY(hipGetLastError());
   const double scale = kernel_scale(p->kernel) * fold(p->kernel, p->newton);
   const int grid = std::min<int64_t>(p->grid, ((int64_t)p->nitems + 3) / 4);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  SCTL_HIP_TRY(hipStreamIsCapturing(s, &cs));
+  if (cs == hipStreamCaptureStatusNone) profile_count(p->kernel, p->interactions);  // graphs count per launch
+  ProfileScope prof(s);
   if (p->precision == SCTL_AMD_MIXED) {
     hipLaunchKernelGGL(mix_blocks(p->kernel, p->newton, p->spl), dim3(grid), dim3(BWG), 0, s, p->d_items, p->nitems,
-                       p->d_counter, p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt32, p->d_xs32, v_src, v_trg,
-                       scale);
+                       p->d_counter, p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, v_src,
+                       v_trg, scale);
   } else {
     hipLaunchKernelGGL(f64_blocks(p->kernel, p->newton, p->spl), dim3(grid), dim3(BWG), 0, s, p->d_items, p->nitems,
                        p->d_counter, p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, v_src,
@@ -551,14 +512,6 @@ int sctl_amd_plan_set_geometry(sctl_amd_plan p, const double* r_trg, const doubl
   SCTL_TRY(copy(&p->d_xt, r_trg, p->nt_total * 3));
   SCTL_TRY(copy(&p->d_xs, r_src, p->ns_total * 3));
   if (p->ND) SCTL_TRY(copy(&p->d_xn, n_src, p->ns_total * 3));
-  if (p->precision == SCTL_AMD_MIXED && p->nblocks > 0) {
-    if (!p->d_xt32 && p->nt_total) SCTL_HIP_TRY(hipMalloc(&p->d_xt32, sizeof(float4) * p->nt_total));
-    if (!p->d_xs32 && p->ns_total) SCTL_HIP_TRY(hipMalloc(&p->d_xs32, sizeof(float4) * p->ns_total * (p->ND ? 2 : 1)));
-    hipLaunchKernelGGL(k_pack_blocks, dim3((unsigned)p->nblocks), dim3(BWG), 0, s, p->nblocks, p->d_trg_dsp,
-                       p->d_trg_cnt, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, p->ND ? 1 : 0, p->d_xt32,
-                       p->d_xs32);
-    SCTL_HIP_TRY(hipGetLastError());
-  }
   if (flags & SCTL_AMD_HOST_PTRS) SCTL_HIP_TRY(hipStreamSynchronize(s));
   p->has_geometry = true;
   return SCTL_AMD_SUCCESS;

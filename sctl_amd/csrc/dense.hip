@@ -1,7 +1,6 @@
-// Dense direct summation on gfx950: Eval (FP64 and mixed precision) and
-// KernelMatrix.
+// Dense direct summation on gfx950: FP64 Eval and KernelMatrix.
 //
-// Eval (reference: generic-kernel.txx:102-211).  A workgroup of 256 lanes
+// FP64 Eval (reference: generic-kernel.txx:102-211).  A workgroup of 256 lanes
 // owns 256*TPL targets (TPL targets per lane, held in registers) and walks a
 // contiguous range of sources in tiles of TILE sources staged through LDS;
 // every lane of a wave reads the same source record (an LDS broadcast).  When
@@ -14,11 +13,7 @@
 // non-finite (some r2 == 0 in that tile) it is restored and the tile redone
 // with the r2 > 0 mask.
 //
-// Mixed path: sources are put in Morton order and packed, per tile, as FP32
-// offsets from the tile's bounding-box centre (the differencing happens in
-// FP64).  A target subtracts the centre in FP64 once per tile and rounds to
-// FP32; the pairwise work is FP32 with v_rsq_f32; each tile's FP32 partial is
-// added into FP64 per-target sums with Kahan compensation.
+// The mixed-precision Eval is in mixed.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,7 +30,6 @@ constexpr int TILE = 256;
 // Far-away padding for the unused slots of a partial tile: the contribution is
 // a finite, vanishing kernel value times a zero density.
 constexpr double PAD_F64 = 1e100;
-constexpr float PAD_F32 = 1e18f;
 
 
 // ------------------------------------------------------------------ FP64 Eval
@@ -151,168 +145,6 @@ k_reduce_splits(const double* __restrict__ part, int64_t n, int nsplit, double* 
   out[i] += s * scale;
 }
 
-// --------------------------------------------------------------- mixed Eval
-// Packs the sources, in the order perm[], into FP32 tile records relative to
-// each tile's bounding-box centre.  One workgroup per tile.
-template <int KER>
-__global__ void __launch_bounds__(WG)
-k_pack_mixed(const double* __restrict__ xs, const double* __restrict__ xn, const double* __restrict__ fs, int64_t ns,
-             const uint32_t* __restrict__ perm, float4* __restrict__ packed, double* __restrict__ centre) {
-  using K = Ker<KER>;
-  constexpr int K0 = K::K0, ND = K::ND, R = rec_f32<KER>(), Q = R / 4;
-  static_assert(TILE == WG, "one source per lane");
-  __shared__ double red[6][WG];
-  const int tid = threadIdx.x;
-  const int64_t s = (int64_t)blockIdx.x * TILE + tid;
-  const bool ok = s < ns;
-  const int64_t src = ok ? (int64_t)perm[s] : 0;
-  double p[3] = {0, 0, 0};
-  if (ok) {
-    p[0] = xs[src * 3 + 0];
-    p[1] = xs[src * 3 + 1];
-    p[2] = xs[src * 3 + 2];
-  }
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    red[k][tid] = ok ? p[k] : INFINITY;
-    red[3 + k][tid] = ok ? p[k] : -INFINITY;
-  }
-  __syncthreads();
-  for (int w = WG / 2; w > 0; w >>= 1) {
-    if (tid < w) {
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        red[k][tid] = fmin(red[k][tid], red[k][tid + w]);
-        red[3 + k][tid] = fmax(red[3 + k][tid], red[3 + k][tid + w]);
-      }
-    }
-    __syncthreads();
-  }
-  const double c[3] = {0.5 * (red[0][0] + red[3][0]), 0.5 * (red[1][0] + red[4][0]), 0.5 * (red[2][0] + red[5][0])};
-  if (tid == 0) {
-    centre[blockIdx.x * 3 + 0] = c[0];
-    centre[blockIdx.x * 3 + 1] = c[1];
-    centre[blockIdx.x * 3 + 2] = c[2];
-  }
-  float rec[R];
-#pragma unroll
-  for (int k = 0; k < R; k++) rec[k] = 0.0f;
-  if (ok) {
-#pragma unroll
-    for (int k = 0; k < 3; k++) rec[k] = (float)(p[k] - c[k]);
-#pragma unroll
-    for (int k = 0; k < ND; k++) rec[3 + k] = (float)xn[src * 3 + k];
-#pragma unroll
-    for (int k = 0; k < K0; k++) rec[3 + ND + k] = (float)fs[src * K0 + k];
-  } else {
-    rec[0] = rec[1] = rec[2] = PAD_F32;
-  }
-  float4* dst = packed + ((int64_t)blockIdx.x * TILE + tid) * Q;
-#pragma unroll
-  for (int q = 0; q < Q; q++) dst[q] = make_float4(rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], rec[4 * q + 3]);
-}
-
-template <int KER, int TPL, int NEWTON>
-__global__ void __launch_bounds__(WG)
-k_eval_mixed(const double* __restrict__ xt, int64_t nt, const float4* __restrict__ packed,
-             const double* __restrict__ centre, int64_t ntiles, int64_t tiles_per_split, double* __restrict__ out,
-             int direct, double scale) {
-  using K = Ker<KER>;
-  constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, R = rec_f32<KER>(), Q = R / 4;
-  __shared__ float4 lds[TILE * Q];
-
-  const int tid = threadIdx.x;
-  const int64_t tile_begin = (int64_t)blockIdx.y * tiles_per_split;
-  const int64_t tile_end = tile_begin + tiles_per_split < ntiles ? tile_begin + tiles_per_split : ntiles;
-  const int64_t t0 = (int64_t)blockIdx.x * (WG * TPL) + tid;
-
-  double tx[TPL], ty[TPL], tz[TPL], acc[TPL][K1], comp[TPL][K1];
-#pragma unroll
-  for (int j = 0; j < TPL; j++) {
-    const int64_t t = t0 + (int64_t)j * WG;
-    const bool ok = t < nt;
-    tx[j] = ok ? xt[t * 3 + 0] : 0.0;
-    ty[j] = ok ? xt[t * 3 + 1] : 0.0;
-    tz[j] = ok ? xt[t * 3 + 2] : 0.0;
-#pragma unroll
-    for (int k = 0; k < K1; k++) acc[j][k] = comp[j][k] = 0.0;
-  }
-
-  for (int64_t tile = tile_begin; tile < tile_end; tile++) {
-    __syncthreads();
-    const float4* src = packed + tile * (TILE * Q);
-#pragma unroll
-    for (int q = 0; q < Q; q++) lds[q * WG + tid] = src[q * WG + tid];
-    const double cx = centre[tile * 3 + 0], cy = centre[tile * 3 + 1], cz = centre[tile * 3 + 2];
-    __syncthreads();
-
-    float ox[TPL], oy[TPL], oz[TPL], p[TPL][K1];
-#pragma unroll
-    for (int j = 0; j < TPL; j++) {
-      ox[j] = (float)(tx[j] - cx);
-      oy[j] = (float)(ty[j] - cy);
-      oz[j] = (float)(tz[j] - cz);
-#pragma unroll
-      for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
-    }
-    const float* recs = reinterpret_cast<const float*>(lds);
-#pragma unroll 4
-    for (int i = 0; i < TILE; i++) {
-      const float* rec = recs + i * R;
-      float n[3] = {0, 0, 0}, f[3];
-#pragma unroll
-      for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
-#pragma unroll
-      for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
-#pragma unroll
-      for (int j = 0; j < TPL; j++)
-        interact<KER, NEWTON, false>(p[j], ox[j] - rec[0], oy[j] - rec[1], oz[j] - rec[2], n, f);
-    }
-    bool bad = false;
-#pragma unroll
-    for (int j = 0; j < TPL; j++)
-#pragma unroll
-      for (int k = 0; k < K1; k++) bad |= !__builtin_isfinite(p[j][k]);
-    if (bad) {
-#pragma unroll
-      for (int j = 0; j < TPL; j++)
-#pragma unroll
-        for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
-      for (int i = 0; i < TILE; i++) {
-        const float* rec = recs + i * R;
-        float n[3] = {0, 0, 0}, f[3];
-        for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
-        for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
-#pragma unroll
-        for (int j = 0; j < TPL; j++)
-          interact<KER, NEWTON, true>(p[j], ox[j] - rec[0], oy[j] - rec[1], oz[j] - rec[2], n, f);
-      }
-    }
-    // Kahan: acc += p
-#pragma unroll
-    for (int j = 0; j < TPL; j++)
-#pragma unroll
-      for (int k = 0; k < K1; k++) {
-        const double y = (double)p[j][k] - comp[j][k];
-        const double s = acc[j][k] + y;
-        comp[j][k] = (s - acc[j][k]) - y;
-        acc[j][k] = s;
-      }
-  }
-
-#pragma unroll
-  for (int j = 0; j < TPL; j++) {
-    const int64_t t = t0 + (int64_t)j * WG;
-    if (t < nt) {
-#pragma unroll
-      for (int k = 0; k < K1; k++) {
-        if (direct) out[t * K1 + k] += acc[j][k] * scale;
-        else out[(int64_t)blockIdx.y * nt * K1 + t * K1 + k] = acc[j][k];
-      }
-    }
-  }
-}
-
 // ------------------------------------------------------------- KernelMatrix
 // M[(s*K0+k0)*ld + t*K1+k1] = scale * u[k0][k1]; one lane per target, the
 // workgroup's gridDim.y slices stride over the sources.
@@ -346,9 +178,6 @@ template <class F> struct Table;
 
 using EvalF64Fn = void (*)(const double*, int64_t, const double*, const double*, const double*, int64_t, int64_t,
                            double*, int, double);
-using EvalMixFn = void (*)(const double*, int64_t, const float4*, const double*, int64_t, int64_t, double*, int,
-                           double);
-using PackFn = void (*)(const double*, const double*, const double*, int64_t, const uint32_t*, float4*, double*);
 using KMatFn = void (*)(const double*, int64_t, const double*, const double*, int64_t, double*, double);
 
 constexpr int TPLS[3] = {1, 2, 4};
@@ -377,40 +206,6 @@ EvalF64Fn f64_fn(int ker, int newton, int ti) {
   }
 }
 
-template <int KER, int NEWTON> EvalMixFn mix_fn(int ti) {
-  switch (ti) {
-    case 0: return k_eval_mixed<KER, 1, NEWTON>;
-    case 1: return k_eval_mixed<KER, 2, NEWTON>;
-    default: return k_eval_mixed<KER, 4, NEWTON>;
-  }
-}
-template <int KER> EvalMixFn mix_fn(int newton, int ti) {
-  return newton ? mix_fn<KER, 1>(ti) : mix_fn<KER, 0>(ti);
-}
-EvalMixFn mix_fn(int ker, int newton, int ti) {
-  switch (ker) {
-    case LAP_FXU: return mix_fn<LAP_FXU>(newton, ti);
-    case LAP_DXU: return mix_fn<LAP_DXU>(newton, ti);
-    case STK_FXU: return mix_fn<STK_FXU>(newton, ti);
-    default: return mix_fn<STK_DXU>(newton, ti);
-  }
-}
-PackFn pack_fn(int ker) {
-  switch (ker) {
-    case LAP_FXU: return k_pack_mixed<LAP_FXU>;
-    case LAP_DXU: return k_pack_mixed<LAP_DXU>;
-    case STK_FXU: return k_pack_mixed<STK_FXU>;
-    default: return k_pack_mixed<STK_DXU>;
-  }
-}
-int rec_words_f32(int ker) {
-  switch (ker) {
-    case LAP_FXU: return rec_f32<LAP_FXU>();
-    case LAP_DXU: return rec_f32<LAP_DXU>();
-    case STK_FXU: return rec_f32<STK_FXU>();
-    default: return rec_f32<STK_DXU>();
-  }
-}
 template <int KER> KMatFn kmat_fn(int newton) {
   switch (newton) {
     case 0: return k_kernel_matrix<KER, 0>;
@@ -426,10 +221,9 @@ KMatFn kmat_fn(int ker, int newton) {
     default: return kmat_fn<STK_DXU>(newton);
   }
 }
-double fold(int ker, int newton, bool mixed) {
+double fold(int ker, int newton) {
   auto f = [&](auto kc) -> double {
     constexpr int K = decltype(kc)::value;
-    if (mixed) return newton ? fold_factor<K, 1>() : fold_factor<K, 0>();
     return newton == 0 ? fold_factor<K, 0>() : (newton == 1 ? fold_factor<K, 1>() : fold_factor<K, 2>());
   };
   switch (ker) {
@@ -468,56 +262,16 @@ int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const 
   const int64_t ntiles = (ns + TILE - 1) / TILE;
   int64_t nsplit, tps;
   split_sources(tb, ntiles, dev.num_cus, &nsplit, &tps);
-  const double scale = kernel_scale(ker) * fold(ker, newton, false);
+  const double scale = kernel_scale(ker) * fold(ker, newton);
   Scratch part;
   if (nsplit > 1) SCTL_TRY(part.alloc(sizeof(double) * nsplit * nt * K1, stream));
   dim3 grid((unsigned)tb, (unsigned)nsplit);
-  hipLaunchKernelGGL(f64_fn(ker, newton, ti), grid, dim3(WG), 0, stream, r_trg, nt, r_src, n_src, v_src, ns,
-                     tps * TILE, nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale);
-  SCTL_HIP_TRY(hipGetLastError());
-  if (nsplit > 1) {
-    const int64_t n = nt * K1;
-    hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part.as<double>(),
-                       n, (int)nsplit, v_trg, scale);
-    SCTL_HIP_TRY(hipGetLastError());
+  profile_count(ker, nt * ns);
+  {
+    ProfileScope prof(stream);
+    hipLaunchKernelGGL(f64_fn(ker, newton, ti), grid, dim3(WG), 0, stream, r_trg, nt, r_src, n_src, v_src, ns,
+                       tps * TILE, nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale);
   }
-  return SCTL_AMD_SUCCESS;
-}
-
-int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
-                      const double* n_src, const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape,
-                      hipStream_t stream) {
-  if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
-  DeviceInfo dev;
-  SCTL_TRY(device_info(&dev));
-  int K0, K1, ND;
-  kernel_dims(ker, &K0, &K1, &ND);
-  SCTL_CHECK_ARG(ns <= 0xffffffffLL, "mixed precision supports at most 2^32-1 sources per call");
-  const int ti = tpl_index(shape.tpl), tpl = TPLS[ti];
-  const int64_t tb = (nt + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
-  SCTL_CHECK_ARG(tb <= 0x7fffffffLL, "too many targets for one launch");
-  const int64_t ntiles = (ns + TILE - 1) / TILE;
-  SCTL_CHECK_ARG(ntiles <= 0x7fffffffLL, "too many sources for one launch");
-  const int R = rec_words_f32(ker);
-
-  // Morton order, then FP32 tile packing relative to each tile's centre.
-  Scratch perm, packed, centre;
-  SCTL_TRY(perm.alloc(sizeof(uint32_t) * ns, stream));
-  SCTL_TRY(morton_order(r_src, ns, perm.as<uint32_t>(), stream));
-  SCTL_TRY(packed.alloc(sizeof(float) * R * TILE * ntiles, stream));
-  SCTL_TRY(centre.alloc(sizeof(double) * 3 * ntiles, stream));
-  hipLaunchKernelGGL(pack_fn(ker), dim3((unsigned)ntiles), dim3(WG), 0, stream, r_src, n_src, v_src, ns,
-                     perm.as<uint32_t>(), packed.as<float4>(), centre.as<double>());
-  SCTL_HIP_TRY(hipGetLastError());
-
-  int64_t nsplit, tps;
-  split_sources(tb, ntiles, dev.num_cus, &nsplit, &tps);
-  const double scale = kernel_scale(ker) * fold(ker, newton, true);
-  Scratch part;
-  if (nsplit > 1) SCTL_TRY(part.alloc(sizeof(double) * nsplit * nt * K1, stream));
-  hipLaunchKernelGGL(mix_fn(ker, newton, ti), dim3((unsigned)tb, (unsigned)nsplit), dim3(WG), 0, stream, r_trg, nt,
-                     packed.as<float4>(), centre.as<double>(), ntiles, tps, nsplit > 1 ? part.as<double>() : v_trg,
-                     nsplit > 1 ? 0 : 1, scale);
   SCTL_HIP_TRY(hipGetLastError());
   if (nsplit > 1) {
     const int64_t n = nt * K1;
@@ -535,6 +289,8 @@ int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, c
   SCTL_CHECK_ARG(tb <= 0x7fffffffLL, "too many targets for one launch");
   const int64_t gy = std::min<int64_t>(ns, 8192);
   const double scale = kernel_scale(ker);
+  profile_count(ker, nt * ns);
+  ProfileScope prof(stream);
   hipLaunchKernelGGL(kmat_fn(ker, newton), dim3((unsigned)tb, (unsigned)gy), dim3(WG), 0, stream, r_trg, nt, r_src,
                      n_src, ns, M, scale);
   SCTL_HIP_TRY(hipGetLastError());

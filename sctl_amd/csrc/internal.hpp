@@ -100,6 +100,23 @@ int morton_order(const double* pts, int64_t n, uint32_t* perm, hipStream_t strea
 // Shape defaults and tuning-database lookup (tune.cpp).
 DenseShape pick_dense_shape(int ker, int precision, int64_t nt, int64_t ns);
 
+// Profiling (profile.cpp): count(...) is always on; a ProfileScope records
+// HIP events around one launch while timing is enabled.
+void profile_count(int ker, int64_t interactions);
+class ProfileScope {
+ public:
+  explicit ProfileScope(hipStream_t s);
+  ~ProfileScope();
+
+ private:
+  hipEvent_t a_ = nullptr;
+  hipStream_t s_ = nullptr;
+};
+
+// Plan sizes (batched.hip): source/target totals, K0/K1, interactions per apply.
+int plan_sizes(sctl_amd_plan p, int64_t* ns, int64_t* nt, int* K0, int* K1, int64_t* interactions);
+int plan_kernel(sctl_amd_plan p);
+
 int kernel_dims(int ker, int* K0, int* K1, int* ND);
 double kernel_scale(int ker);
 int newton_steps_f64(int digits);

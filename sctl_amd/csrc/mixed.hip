@@ -1,0 +1,427 @@
+// Mixed-precision dense Eval on gfx950: FP64 tile-centred differencing, FP32
+// pairwise arithmetic, Kahan-compensated FP64 accumulation.
+//
+// Sources and targets are both put in Morton order.  The sources are packed,
+// TILE at a time, relative to each tile's bounding-box centre c: for every
+// source the FP64 offset s - c is split into an FP32 value s' and its FP32
+// residual s'' (s - c = s' + s'' to about 2^-48 relative), and the tile keeps
+// H^2 = max |s'|^2.  A target (held in FP64) forms t - c once per tile the same
+// way, t' + t''.
+//
+// Per (wave, tile) one of two loops runs:
+//   far  -- every target of the wave has |t'| >= 3H, so every pair has
+//           r >= 2H.  Laplace kernels use r^2 = |t'|^2 + |s'|^2 - 2 t'.s'
+//           (one add and three FMAs; the rounding error is at most a few ulp
+//           of r^2 because r^2 >= (|t'| - H)^2 is never small against the
+//           terms).  Stokes kernels use dx = t' - s'.
+//   near -- some target is within 3H of the tile centre: positions are
+//           differenced from both halves, dx = (t' - s') + (t'' - s''), which
+//           is accurate to about 2^-23 relative to dx however small r is, and
+//           r^2 == 0 is masked as in the reference (kernel_functions.hpp:45).
+// Targets in Morton order keep a wave's targets together, so only tiles next
+// to the wave take the near loop.  A far tile whose FP32 partial is not
+// finite is redone by the near loop.  Each tile's FP32 partial is added into
+// the FP64 per-target sum with Kahan compensation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace sctl_amd {
+
+namespace {
+
+constexpr int WG = 256;
+constexpr int TILE = 256;
+constexpr float PADF = 1e18f;
+
+// Record layout (floats) per source; far-path words first.
+//   LAP_FXU  [s'x s'y s'z S | q s''x s''y s''z]                         S = |s'|^2
+//   LAP_DXU  [s'x s'y s'z S | nq_x nq_y nq_z -s'.nq | s''x s''y s''z 0]   nq = q n
+//   STK_FXU  [s'x s'y s'z f0 | f1 f2 s''x s''y | s''z 0 0 0]
+//   STK_DXU  [s'x s'y s'z f0 | f1 f2 n_x n_y | n_z s''x s''y s''z]
+template <int KER> struct MixLayout;
+template <> struct MixLayout<LAP_FXU> { static constexpr int Q = 2, LO = 5; };
+template <> struct MixLayout<LAP_DXU> { static constexpr int Q = 3, LO = 8; };
+template <> struct MixLayout<STK_FXU> { static constexpr int Q = 3, LO = 6; };
+template <> struct MixLayout<STK_DXU> { static constexpr int Q = 3, LO = 9; };
+
+template <int KER>
+__device__ __forceinline__ void fill_record(float* r, const double* d, const double* n, const double* f) {
+  float hi[3], lo[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    hi[k] = (float)d[k];
+    lo[k] = (float)(d[k] - (double)hi[k]);
+  }
+  const double S = (double)hi[0] * hi[0] + (double)hi[1] * hi[1] + (double)hi[2] * hi[2];
+  constexpr int LO = MixLayout<KER>::LO;
+  r[0] = hi[0];
+  r[1] = hi[1];
+  r[2] = hi[2];
+  if constexpr (KER == LAP_FXU) {
+    r[3] = (float)S;
+    r[4] = (float)f[0];
+  } else if constexpr (KER == LAP_DXU) {
+    const float nq[3] = {(float)(n[0] * f[0]), (float)(n[1] * f[0]), (float)(n[2] * f[0])};
+    r[3] = (float)S;
+    r[4] = nq[0];
+    r[5] = nq[1];
+    r[6] = nq[2];
+    r[7] = (float)(-((double)hi[0] * nq[0] + (double)hi[1] * nq[1] + (double)hi[2] * nq[2]));
+    r[11] = 0.0f;
+  } else if constexpr (KER == STK_FXU) {
+    r[3] = (float)f[0];
+    r[4] = (float)f[1];
+    r[5] = (float)f[2];
+    r[9] = r[10] = r[11] = 0.0f;
+  } else {
+    r[3] = (float)f[0];
+    r[4] = (float)f[1];
+    r[5] = (float)f[2];
+    r[6] = (float)n[0];
+    r[7] = (float)n[1];
+    r[8] = (float)n[2];
+  }
+  r[LO + 0] = lo[0];
+  r[LO + 1] = lo[1];
+  r[LO + 2] = lo[2];
+}
+
+// Far-field interaction.  th: t' ; a: -2 t' ; T: |t'|^2 ; r: the record.
+template <int KER, int NEWTON>
+__device__ __forceinline__ void mix_far(float* p, const float* th, const float* a, float T, const float* r) {
+  if constexpr (KER == LAP_FXU) {
+    const float r2 = fmad(a[2], r[2], fmad(a[1], r[1], fmad(a[0], r[0], T + r[3])));
+    p[0] = fmad(rsq_deferred<NEWTON>(r2), r[4], p[0]);
+  } else if constexpr (KER == LAP_DXU) {
+    const float r2 = fmad(a[2], r[2], fmad(a[1], r[1], fmad(a[0], r[0], T + r[3])));
+    const float rnq = fmad(th[2], r[6], fmad(th[1], r[5], fmad(th[0], r[4], r[7])));
+    const float z = rsq_deferred<NEWTON>(r2);
+    p[0] = fmad(rnq, z * z * z, p[0]);
+  } else if constexpr (KER == STK_FXU) {
+    const float f[3] = {r[3], r[4], r[5]};
+    interact<STK_FXU, NEWTON, false>(p, th[0] - r[0], th[1] - r[1], th[2] - r[2], f, f);
+  } else {
+    const float f[3] = {r[3], r[4], r[5]}, n[3] = {r[6], r[7], r[8]};
+    interact<STK_DXU, NEWTON, false>(p, th[0] - r[0], th[1] - r[1], th[2] - r[2], n, f);
+  }
+}
+
+// Near-field interaction: two-part differencing, r^2 == 0 masked.
+template <int KER, int NEWTON>
+__device__ __forceinline__ void mix_near(float* p, const float* th, const float* tl, const float* r) {
+  constexpr int LO = MixLayout<KER>::LO;
+  const float dx = (th[0] - r[0]) + (tl[0] - r[LO + 0]);
+  const float dy = (th[1] - r[1]) + (tl[1] - r[LO + 1]);
+  const float dz = (th[2] - r[2]) + (tl[2] - r[LO + 2]);
+  if constexpr (KER == LAP_FXU) {
+    const float f[1] = {r[4]};
+    interact<LAP_FXU, NEWTON, true>(p, dx, dy, dz, f, f);
+  } else if constexpr (KER == LAP_DXU) {
+    const float one[1] = {1.0f}, nq[3] = {r[4], r[5], r[6]};
+    interact<LAP_DXU, NEWTON, true>(p, dx, dy, dz, nq, one);
+  } else if constexpr (KER == STK_FXU) {
+    const float f[3] = {r[3], r[4], r[5]};
+    interact<STK_FXU, NEWTON, true>(p, dx, dy, dz, f, f);
+  } else {
+    const float f[3] = {r[3], r[4], r[5]}, n[3] = {r[6], r[7], r[8]};
+    interact<STK_DXU, NEWTON, true>(p, dx, dy, dz, n, f);
+  }
+}
+
+// One workgroup per tile: tile centre and H^2, then the records, sources in
+// the order perm[].  meta[tile] = {c_x, c_y, c_z, H^2}.
+template <int KER>
+__global__ void __launch_bounds__(WG)
+k_pack_tiles(const double* __restrict__ xs, const double* __restrict__ xn, const double* __restrict__ fs, int64_t ns,
+             const uint32_t* __restrict__ perm, float4* __restrict__ packed, double4* __restrict__ meta) {
+  using K = Ker<KER>;
+  constexpr int K0 = K::K0, ND = K::ND, Q = MixLayout<KER>::Q;
+  static_assert(TILE == WG, "one source per lane");
+  __shared__ double red[6][WG];
+  __shared__ float hred[WG];
+  const int tid = threadIdx.x;
+  const int64_t s = (int64_t)blockIdx.x * TILE + tid;
+  const bool ok = s < ns;
+  const int64_t src = ok ? (int64_t)perm[s] : 0;
+  double p[3] = {0, 0, 0};
+  if (ok)
+#pragma unroll
+    for (int k = 0; k < 3; k++) p[k] = xs[src * 3 + k];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    red[k][tid] = ok ? p[k] : INFINITY;
+    red[3 + k][tid] = ok ? p[k] : -INFINITY;
+  }
+  __syncthreads();
+  for (int w = WG / 2; w > 0; w >>= 1) {
+    if (tid < w)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        red[k][tid] = fmin(red[k][tid], red[k][tid + w]);
+        red[3 + k][tid] = fmax(red[3 + k][tid], red[3 + k][tid + w]);
+      }
+    __syncthreads();
+  }
+  const double c[3] = {0.5 * (red[0][0] + red[3][0]), 0.5 * (red[1][0] + red[4][0]), 0.5 * (red[2][0] + red[5][0])};
+  float rec[4 * Q];
+#pragma unroll
+  for (int k = 0; k < 4 * Q; k++) rec[k] = 0.0f;
+  float h2 = 0.0f;
+  if (ok) {
+    const double d[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+    double n[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < ND; k++) n[k] = xn[src * 3 + k];
+#pragma unroll
+    for (int k = 0; k < K0; k++) f[k] = fs[src * K0 + k];
+    fill_record<KER>(rec, d, n, f);
+    h2 = rec[0] * rec[0] + rec[1] * rec[1] + rec[2] * rec[2];
+  } else {
+    rec[0] = rec[1] = rec[2] = PADF;
+    if constexpr (KER == LAP_FXU || KER == LAP_DXU) rec[3] = 3.0f * PADF * PADF;
+  }
+  hred[tid] = h2;
+  __syncthreads();
+  for (int w = WG / 2; w > 0; w >>= 1) {
+    if (tid < w) hred[tid] = fmaxf(hred[tid], hred[tid + w]);
+    __syncthreads();
+  }
+  if (tid == 0) meta[blockIdx.x] = make_double4(c[0], c[1], c[2], (double)hred[0] * 1.0000001);
+  float4* dst = packed + ((int64_t)blockIdx.x * TILE + tid) * Q;
+#pragma unroll
+  for (int q = 0; q < Q; q++) dst[q] = make_float4(rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], rec[4 * q + 3]);
+}
+
+template <int KER, int TPL, int NEWTON>
+__global__ void __launch_bounds__(WG)
+k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restrict__ tperm,
+             const float4* __restrict__ packed, const double4* __restrict__ meta, int64_t ntiles,
+             int64_t tiles_per_split, double* __restrict__ out, int direct, double scale) {
+  using K = Ker<KER>;
+  constexpr int K1 = K::K1, Q = MixLayout<KER>::Q, R = 4 * Q;
+  __shared__ float4 lds[TILE * Q];
+
+  const int tid = threadIdx.x;
+  const int64_t tile_begin = (int64_t)blockIdx.y * tiles_per_split;
+  const int64_t tile_end = tile_begin + tiles_per_split < ntiles ? tile_begin + tiles_per_split : ntiles;
+  const int64_t q0 = (int64_t)blockIdx.x * (WG * TPL) + tid;
+
+  double t64[TPL][3], acc[TPL][K1], comp[TPL][K1];
+  int64_t tidx[TPL];
+#pragma unroll
+  for (int j = 0; j < TPL; j++) {
+    const int64_t q = q0 + (int64_t)j * WG;
+    const bool ok = q < nt;
+    tidx[j] = ok ? (int64_t)tperm[q] : -1;
+#pragma unroll
+    for (int k = 0; k < 3; k++) t64[j][k] = ok ? xt[tidx[j] * 3 + k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < K1; k++) acc[j][k] = comp[j][k] = 0.0;
+  }
+
+  for (int64_t tile = tile_begin; tile < tile_end; tile++) {
+    __syncthreads();
+    const float4* src = packed + tile * (TILE * Q);
+#pragma unroll
+    for (int q = 0; q < Q; q++) lds[q * WG + tid] = src[q * WG + tid];
+    const double4 m = meta[tile];
+    __syncthreads();
+
+    const float near_r2 = 9.0f * (float)m.w;
+    float th[TPL][3], a[TPL][3], T[TPL], p[TPL][K1];
+    bool near = false;
+#pragma unroll
+    for (int j = 0; j < TPL; j++) {
+      th[j][0] = (float)(t64[j][0] - m.x);
+      th[j][1] = (float)(t64[j][1] - m.y);
+      th[j][2] = (float)(t64[j][2] - m.z);
+#pragma unroll
+      for (int k = 0; k < 3; k++) a[j][k] = -2.0f * th[j][k];
+      T[j] = fmad(th[j][2], th[j][2], fmad(th[j][1], th[j][1], th[j][0] * th[j][0]));
+      near |= T[j] < near_r2;
+#pragma unroll
+      for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
+    }
+    const float* recs = reinterpret_cast<const float*>(lds);
+    bool use_near = __builtin_amdgcn_ballot_w64(near) != 0;
+    if (!use_near) {
+#pragma unroll 4
+      for (int i = 0; i < TILE; i++) {
+        const float* rec = recs + i * R;
+#pragma unroll
+        for (int j = 0; j < TPL; j++) mix_far<KER, NEWTON>(p[j], th[j], a[j], T[j], rec);
+      }
+      bool bad = false;
+#pragma unroll
+      for (int j = 0; j < TPL; j++)
+#pragma unroll
+        for (int k = 0; k < K1; k++) bad |= !__builtin_isfinite(p[j][k]);
+      use_near = __builtin_amdgcn_ballot_w64(bad) != 0;
+    }
+    if (use_near) {
+      float tl[TPL][3];
+#pragma unroll
+      for (int j = 0; j < TPL; j++) {
+        tl[j][0] = (float)((t64[j][0] - m.x) - (double)th[j][0]);
+        tl[j][1] = (float)((t64[j][1] - m.y) - (double)th[j][1]);
+        tl[j][2] = (float)((t64[j][2] - m.z) - (double)th[j][2]);
+#pragma unroll
+        for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
+      }
+#pragma unroll 2
+      for (int i = 0; i < TILE; i++) {
+        const float* rec = recs + i * R;
+#pragma unroll
+        for (int j = 0; j < TPL; j++) mix_near<KER, NEWTON>(p[j], th[j], tl[j], rec);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < TPL; j++)
+#pragma unroll
+      for (int k = 0; k < K1; k++) {  // Kahan: acc += p
+        const double y = (double)p[j][k] - comp[j][k];
+        const double s = acc[j][k] + y;
+        comp[j][k] = (s - acc[j][k]) - y;
+        acc[j][k] = s;
+      }
+  }
+
+#pragma unroll
+  for (int j = 0; j < TPL; j++) {
+    const int64_t q = q0 + (int64_t)j * WG;
+    if (q < nt) {
+#pragma unroll
+      for (int k = 0; k < K1; k++) {
+        const double v = acc[j][k] - comp[j][k];
+        if (direct) out[tidx[j] * K1 + k] += v * scale;
+        else out[(int64_t)blockIdx.y * nt * K1 + q * K1 + k] = v;
+      }
+    }
+  }
+}
+
+// v_trg[tperm[q]*K1+k] += scale * sum_y part[y][q*K1+k], in split order.
+__global__ void __launch_bounds__(WG)
+k_reduce_splits_perm(const double* __restrict__ part, int64_t nt, int K1, int nsplit, const uint32_t* __restrict__ tperm,
+                     double* __restrict__ out, double scale) {
+  const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
+  const int64_t n = nt * K1;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int y = 0; y < nsplit; y++) s += part[(int64_t)y * n + i];
+  const int64_t q = i / K1, k = i - q * K1;
+  out[(int64_t)tperm[q] * K1 + k] += s * scale;
+}
+
+using EvalMixFn = void (*)(const double*, int64_t, const uint32_t*, const float4*, const double4*, int64_t, int64_t,
+                           double*, int, double);
+using PackFn = void (*)(const double*, const double*, const double*, int64_t, const uint32_t*, float4*, double4*);
+
+constexpr int TPLS[3] = {1, 2, 4};
+int tpl_index(int tpl) { return tpl >= 4 ? 2 : (tpl >= 2 ? 1 : 0); }
+
+template <int KER, int NEWTON> EvalMixFn mix_fn(int ti) {
+  switch (ti) {
+    case 0: return k_eval_mixed<KER, 1, NEWTON>;
+    case 1: return k_eval_mixed<KER, 2, NEWTON>;
+    default: return k_eval_mixed<KER, 4, NEWTON>;
+  }
+}
+template <int KER> EvalMixFn mix_fn(int newton, int ti) { return newton ? mix_fn<KER, 1>(ti) : mix_fn<KER, 0>(ti); }
+EvalMixFn mix_fn(int ker, int newton, int ti) {
+  switch (ker) {
+    case LAP_FXU: return mix_fn<LAP_FXU>(newton, ti);
+    case LAP_DXU: return mix_fn<LAP_DXU>(newton, ti);
+    case STK_FXU: return mix_fn<STK_FXU>(newton, ti);
+    default: return mix_fn<STK_DXU>(newton, ti);
+  }
+}
+PackFn pack_fn(int ker) {
+  switch (ker) {
+    case LAP_FXU: return k_pack_tiles<LAP_FXU>;
+    case LAP_DXU: return k_pack_tiles<LAP_DXU>;
+    case STK_FXU: return k_pack_tiles<STK_FXU>;
+    default: return k_pack_tiles<STK_DXU>;
+  }
+}
+int layout_q(int ker) {
+  switch (ker) {
+    case LAP_FXU: return MixLayout<LAP_FXU>::Q;
+    case LAP_DXU: return MixLayout<LAP_DXU>::Q;
+    case STK_FXU: return MixLayout<STK_FXU>::Q;
+    default: return MixLayout<STK_DXU>::Q;
+  }
+}
+double mixed_fold(int ker, int newton) {
+  if (!newton) return 1.0;
+  switch (ker) {
+    case LAP_FXU: return fold_factor<LAP_FXU, 1>();
+    case LAP_DXU: return fold_factor<LAP_DXU, 1>();
+    case STK_FXU: return fold_factor<STK_FXU, 1>();
+    default: return fold_factor<STK_DXU, 1>();
+  }
+}
+
+}  // namespace
+
+int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
+                      const double* n_src, const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape,
+                      hipStream_t stream) {
+  if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
+  DeviceInfo dev;
+  SCTL_TRY(device_info(&dev));
+  int K0, K1, ND;
+  kernel_dims(ker, &K0, &K1, &ND);
+  SCTL_CHECK_ARG(ns <= 0xffffffffLL && nt <= 0xffffffffLL, "mixed precision supports at most 2^32-1 points");
+  const int ti = tpl_index(shape.tpl), tpl = TPLS[ti];
+  const int64_t tb = (nt + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
+  SCTL_CHECK_ARG(tb <= 0x7fffffffLL, "too many targets for one launch");
+  const int64_t ntiles = (ns + TILE - 1) / TILE;
+  SCTL_CHECK_ARG(ntiles <= 0x7fffffffLL, "too many sources for one launch");
+  const int Q = layout_q(ker);
+
+  Scratch sperm, tperm, packed, meta;
+  SCTL_TRY(sperm.alloc(sizeof(uint32_t) * ns, stream));
+  SCTL_TRY(tperm.alloc(sizeof(uint32_t) * nt, stream));
+  SCTL_TRY(morton_order(r_src, ns, sperm.as<uint32_t>(), stream));
+  SCTL_TRY(morton_order(r_trg, nt, tperm.as<uint32_t>(), stream));
+  SCTL_TRY(packed.alloc(sizeof(float4) * Q * TILE * ntiles, stream));
+  SCTL_TRY(meta.alloc(sizeof(double4) * ntiles, stream));
+  hipLaunchKernelGGL(pack_fn(ker), dim3((unsigned)ntiles), dim3(WG), 0, stream, r_src, n_src, v_src, ns,
+                     sperm.as<uint32_t>(), packed.as<float4>(), meta.as<double4>());
+  SCTL_HIP_TRY(hipGetLastError());
+
+  // Split the sources over gridDim.y when the target blocks cannot fill the
+  // device (>= 8 workgroups per CU, each split >= 4 tiles).
+  const int64_t want = 8LL * std::max(dev.num_cus, 1);
+  int64_t nsplit = tb < want ? (want + tb - 1) / tb : 1;
+  nsplit = std::min<int64_t>(std::min<int64_t>(nsplit, std::max<int64_t>(1, ntiles / 4)), 65535);
+  const int64_t tps = (ntiles + nsplit - 1) / nsplit;
+  nsplit = (ntiles + tps - 1) / tps;
+
+  const double scale = kernel_scale(ker) * mixed_fold(ker, newton);
+  Scratch part;
+  if (nsplit > 1) SCTL_TRY(part.alloc(sizeof(double) * nsplit * nt * K1, stream));
+  profile_count(ker, nt * ns);
+  {
+    ProfileScope prof(stream);
+    hipLaunchKernelGGL(mix_fn(ker, newton, ti), dim3((unsigned)tb, (unsigned)nsplit), dim3(WG), 0, stream, r_trg,
+                       nt, tperm.as<uint32_t>(), packed.as<float4>(), meta.as<double4>(), ntiles, tps,
+                       nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale);
+  }
+  SCTL_HIP_TRY(hipGetLastError());
+  if (nsplit > 1) {
+    const int64_t n = nt * K1;
+    hipLaunchKernelGGL(k_reduce_splits_perm, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream,
+                       part.as<double>(), nt, K1, (int)nsplit, tperm.as<uint32_t>(), v_trg, scale);
+    SCTL_HIP_TRY(hipGetLastError());
+  }
+  return SCTL_AMD_SUCCESS;
+}
+
+}  // namespace sctl_amd
