@@ -103,6 +103,126 @@ def cpu_baseline(kernel, xt, xs, n, f, seconds):
                 what, m, xt.size // 3, ns, tmed)}
 
 
+def time_once(fn, stream, reps=1):
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    S.synchronize(stream)
+    return e0.elapsed_ms(e1) / reps
+
+
+def dense_pair(kernel, xt, xs, n, f, stream, reps, digits=-1):
+    """FP64 and mixed Eval of one dense configuration: rates and mixed-vs-FP64 errors."""
+    _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
+    nt, ns = xt.size // 3, xs.size // 3
+    d = [S.DeviceArray.from_host(a) for a in (xt, xs, n if ND else np.zeros(3), f)]
+    dn = d[2] if ND else None
+    res = {}
+    outs = {}
+    for name, prec in (("fp64", S.FP64), ("mixed", S.MIXED)):
+        o = S.DeviceArray.from_host(np.zeros(nt * K1))
+        S.eval(kernel, d[0], d[1], dn, d[3], o, precision=prec, digits=digits, stream=stream)  # warm-up
+        S.synchronize(stream)
+        outs[name] = o.download()
+        ms = time_once(lambda: S.eval(kernel, d[0], d[1], dn, d[3], o, precision=prec, digits=digits,
+                                      stream=stream), stream, reps)
+        res[name] = {"ms": round(ms, 3), "G_int_per_s": round(nt * ns / ms / 1e6, 2)}
+    e_inf, e_rms = W.rel_errors(outs["mixed"], outs["fp64"])
+    res["mixed_vs_fp64"] = {"max_rel": float("%.3g" % e_inf), "rms_rel": float("%.3g" % e_rms)}
+    res["speedup"] = round(res["fp64"]["ms"] / res["mixed"]["ms"], 3)
+    return res, outs
+
+
+def plan_pair(kernel, B, stream, reps):
+    _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
+    nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
+    dv = S.DeviceArray.from_host(B["v_src"])
+    geo = [S.DeviceArray.from_host(B[a]) for a in ("r_trg", "r_src", "n_src")]
+    res, outs = {}, {}
+    for name, prec in (("fp64", S.FP64), ("mixed", S.MIXED)):
+        P = S.Plan(kernel, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=prec)
+        P.set_geometry(geo[0], geo[1], geo[2] if ND else None, stream=stream)
+        o = S.DeviceArray.from_host(np.zeros(nt * K1))
+        P.apply(dv, o, stream=stream)
+        S.synchronize(stream)
+        outs[name] = o.download()
+        ms = time_once(lambda: P.apply(dv, o, stream=stream), stream, reps)
+        res[name] = {"ms_per_apply": round(ms, 3), "G_int_per_s": round(P.interactions() / ms / 1e6, 2)}
+        res["interactions_per_apply"] = P.interactions()
+        P.destroy()
+    e_inf, e_rms = W.rel_errors(outs["mixed"], outs["fp64"])
+    res["mixed_vs_fp64"] = {"max_rel": float("%.3g" % e_inf), "rms_rel": float("%.3g" % e_rms)}
+    res["speedup"] = round(res["fp64"]["ms_per_apply"] / res["mixed"]["ms_per_apply"], 3)
+    return res, outs
+
+
+def graph_pair(kernel, B, stream, nsteps, seed=0):
+    _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
+    nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
+    geo = [S.DeviceArray.from_host(B[a]) for a in ("r_trg", "r_src", "n_src")]
+    res, outs = {}, {}
+    for name, prec in (("fp64", S.FP64), ("mixed", S.MIXED)):
+        P = S.Plan(kernel, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=prec)
+        P.set_geometry(geo[0], geo[1], geo[2] if ND else None, stream=stream)
+        vs = S.DeviceArray(ns * K0)
+        vt = S.DeviceArray.from_host(np.zeros(nt * K1))
+        G = P.graph(nsteps, seed, 25, vs, vt)
+        G.launch(stream)  # warm-up replay
+        S.synchronize(stream)
+        vt.upload(np.zeros(nt * K1))
+        ms = time_once(lambda: G.launch(stream), stream, 1)
+        outs[name] = vt.download()
+        res[name] = {"ms_per_graph": round(ms, 3), "ms_per_apply": round(ms / nsteps, 4),
+                     "G_int_per_s": round(G.interactions() / ms / 1e6, 2)}
+        res["interactions_per_apply"] = P.interactions()
+        G.destroy()
+        P.destroy()
+    e_inf, e_rms = W.rel_errors(outs["mixed"], outs["fp64"])
+    res["mixed_vs_fp64"] = {"max_rel": float("%.3g" % e_inf), "rms_rel": float("%.3g" % e_rms)}
+    res["applies"] = nsteps
+    res["speedup"] = round(res["fp64"]["ms_per_graph"] / res["mixed"]["ms_per_graph"], 3)
+    return res
+
+
+def other_configs(stream, quick=False):
+    """The other configurations: measured, not the headline."""
+    import oracle
+    out = {}
+    log("config 1: Laplace FxU FP64, 20000 x 20000 uniform cube, parity vs the oracle")
+    xt, xs, f = W.cube(20000, 20000, seed=0)
+    r, o = dense_pair(S.LAPLACE3D_FXU, xt, xs, None, f, stream, reps=5)
+    ref = oracle.eval(S.LAPLACE3D_FXU, xt, xs, None, f)
+    r["fp64_vs_oracle"] = dict(zip(("max_rel", "rms_rel"), (float("%.3g" % e) for e in W.rel_errors(o["fp64"], ref))))
+    r["mixed_vs_oracle"] = dict(zip(("max_rel", "rms_rel"), (float("%.3g" % e) for e in W.rel_errors(o["mixed"], ref))))
+    out["c1_laplace_fxu_cube_20k"] = r
+    N = 1 << 20
+    log("config 2 (DxU leg): Laplace DxU, 2^20 sphere sources x 2^20 shell targets")
+    xt, xs, nrm, f = W.sphere(N, N, seed=0)
+    out["c2_laplace_dxu_sphere_1M"], _ = dense_pair(S.LAPLACE3D_DXU, xt, xs, nrm, f, stream, reps=1 if quick else 2)
+    log("config 3: Stokes FxU, 524288 points in 64 Gaussian clusters")
+    M = 524288
+    xt, xs, f = W.clusters(M, M, seed=0, k0=3)
+    r, o = dense_pair(S.STOKES3D_FXU, xt, xs, None, f, stream, reps=1 if quick else 2)
+    sub = 512
+    ref = oracle.eval(S.STOKES3D_FXU, xt[:3 * sub], xs, None, f)
+    q = np.arange(3 * sub)
+    r["mixed_vs_oracle_subset"] = dict(zip(("max_rel", "rms_rel"), (float("%.3g" % e) for e in W.rel_errors(
+        o["mixed"][q], ref))))
+    r["fp64_vs_oracle_subset"] = dict(zip(("max_rel", "rms_rel"), (float("%.3g" % e) for e in W.rel_errors(
+        o["fp64"][q], ref))))
+    r["oracle_subset_targets"] = sub
+    out["c3_stokes_fxu_clusters_512k"] = r
+    log("config 4: batched Laplace DxU, 4096 torus blocks")
+    B = W.torus_blocks(64, 64, seed=0, k0=1)
+    out["c4_batched_laplace_dxu_4096"], _ = plan_pair(S.LAPLACE3D_DXU, B, stream, reps=5)
+    log("config 5: batched Stokes DxU, 2048 torus blocks, 100 applies in one hipGraph")
+    B = W.torus_blocks(64, 32, seed=0, k0=3)
+    out["c5_stokes_dxu_graph_2048x100"] = graph_pair(S.STOKES3D_DXU, B, stream, nsteps=20 if quick else 100)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +231,7 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20, help="sources = targets")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work per baseline run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -154,6 +275,14 @@ def main():
     k64_ms = f64["kernel_ms"] / max(f64["launches"], 1)
     achieved64 = FLOP_PER_INT[K] * inter / (k64_ms * 1e-3) / 1e12
 
+    configs = None
+    if not args.no_configs:
+        try:
+            configs = other_configs(stream)
+        except Exception as ex:
+            log("other configurations failed:", repr(ex))
+            configs = {"error": repr(ex)}
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (~%.0f s per run)" % args.cpu_seconds)
@@ -194,6 +323,7 @@ def main():
                      "kernel": "k_eval_mixed<Laplace3D_FxU>", "mean_launch_ms": round(kern_ms, 3),
                      "flop_per_interaction": FLOP_PER_INT[K]},
         "cpu_baseline": cpu,
+        "configs": configs,
     }
     log("wall %.2fs for %d mixed steps (events %.1f ms)" % (mx["wall_s"], args.steps, mx["total_ms"]))
     print(json.dumps(line), flush=True)
