@@ -46,6 +46,18 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X published FP64 vector peak
 FLOP_PER_INT = {0: 11, 1: 19, 2: 29, 3: 30}  # algorithmic flop per interaction
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_eval_mixed_laplace_fxu_sphere_1M.json")
+
+
+def measured_traffic(n):
+    """HBM bytes per launch of the headline kernel from the committed FETCH_SIZE /
+    WRITE_SIZE counter runs (copy-calibrated), for the full-size workload only."""
+    if n != 1 << 20 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        return float(json.load(fh)["traffic_bytes"])
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -232,6 +244,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work per baseline run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the mixed headline steps (for hardware-counter runs): no FP64 legs, configs, CPU")
+    ap.add_argument("--headline-precision", choices=["mixed", "fp64"], default="mixed",
+                    help="with --headline-only: which path to run")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -250,8 +268,22 @@ def main():
     S.check(S.lib().sctl_amd_stream_create(ctypes.byref(stream)))
     out = S.DeviceArray.from_host(np.zeros(N))
 
+    if args.copy_mb:
+        nb = args.copy_mb << 20
+        a, b = S.DeviceArray(nb // 8), S.DeviceArray(nb // 8)
+        S.check(S.lib().sctl_amd_memcpy(b.ptr, a.ptr, nb, 2, stream))
+        S.synchronize(stream)
+        log("calibration copy: %d bytes device to device" % nb)
     log("mixed precision: %d warmup + %d timed steps" % (args.warmup, args.steps))
-    mx = timed_evals(K, dev, out, S.MIXED, -1, args.steps, args.warmup, stream)
+    hp = S.FP64 if (args.headline_only and args.headline_precision == "fp64") else S.MIXED
+    mx = timed_evals(K, dev, out, hp, -1, args.steps, args.warmup, stream)
+    if args.headline_only:
+        v = inter * args.steps / (mx["total_ms"] * 1e-3) / 1e9
+        print(json.dumps({"metric": METRIC, "value": round(v, 2), "unit": "G interactions/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": round(mx["total_ms"] / args.steps, 3), "headline_only": True,
+                          "precision": args.headline_precision}), flush=True)
+        return
     log("fp64 (digits 15): timed steps")
     f64 = timed_evals(K, dev, out, S.FP64, -1, args.steps, min(args.warmup, 1), stream)
     log("fp64 (digits 14): timed steps")
@@ -319,7 +351,7 @@ def main():
         "speedup_vs_fp64_digits14": round(v_mx / v_64b, 3),
         "pct_fp32_valu_peak": round(100 * step_tflops / PEAK_FP32_TFLOPS, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(N),
                      "kernel": "k_eval_mixed<Laplace3D_FxU>", "mean_launch_ms": round(kern_ms, 3),
                      "flop_per_interaction": FLOP_PER_INT[K]},
         "cpu_baseline": cpu,
