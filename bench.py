@@ -157,6 +157,9 @@ def plan_pair(kernel, B, stream, reps):
         P = S.Plan(kernel, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=prec)
         P.set_geometry(geo[0], geo[1], geo[2] if ND else None, stream=stream)
         o = S.DeviceArray.from_host(np.zeros(nt * K1))
+        spl, tpl, cap, _ = P.tune(dv, o, stream=stream)
+        res[name + "_shape"] = {"sources_per_lane": spl, "targets_per_lane": tpl, "wg_per_cu_cap": cap}
+        o.upload(np.zeros(nt * K1))
         P.apply(dv, o, stream=stream)
         S.synchronize(stream)
         outs[name] = o.download()
@@ -180,6 +183,10 @@ def graph_pair(kernel, B, stream, nsteps, seed=0):
         P.set_geometry(geo[0], geo[1], geo[2] if ND else None, stream=stream)
         vs = S.DeviceArray(ns * K0)
         vt = S.DeviceArray.from_host(np.zeros(nt * K1))
+        S.gen_normal(seed, 25, vs, stream=stream)
+        spl, tpl, cap, _ = P.tune(vs, vt, stream=stream)
+        res[name + "_shape"] = {"sources_per_lane": spl, "targets_per_lane": tpl, "wg_per_cu_cap": cap}
+        vt.upload(np.zeros(nt * K1))
         G = P.graph(nsteps, seed, 25, vs, vt)
         G.launch(stream)  # warm-up replay
         S.synchronize(stream)
@@ -244,6 +251,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work per baseline run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
+    ap.add_argument("--no-tune", action="store_true", help="use default launch shapes (no autotuning)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the mixed headline steps (for hardware-counter runs): no FP64 legs, configs, CPU")
     ap.add_argument("--headline-precision", choices=["mixed", "fp64"], default="mixed",
@@ -274,6 +282,13 @@ def main():
         S.check(S.lib().sctl_amd_memcpy(b.ptr, a.ptr, nb, 2, stream))
         S.synchronize(stream)
         log("calibration copy: %d bytes device to device" % nb)
+    tuned = {}
+    if not args.headline_only and not args.no_tune:
+        log("autotuning the launch shapes (not timed)")
+        for name, prec, dig in (("mixed", S.MIXED, -1), ("fp64", S.FP64, -1), ("fp64_digits14", S.FP64, 14)):
+            tpl, cap, ms = S.tune_eval(K, dev[0], dev[1], None, dev[3], out, precision=prec, digits=dig, stream=stream)
+            tuned[name] = {"targets_per_lane": tpl, "wg_per_cu_cap": cap, "ms": round(ms, 3)}
+        out.upload(np.zeros(N))
     log("mixed precision: %d warmup + %d timed steps" % (args.warmup, args.steps))
     hp = S.FP64 if (args.headline_only and args.headline_precision == "fp64") else S.MIXED
     mx = timed_evals(K, dev, out, hp, -1, args.steps, args.warmup, stream)
@@ -354,6 +369,7 @@ def main():
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(N),
                      "kernel": "k_eval_mixed<Laplace3D_FxU>", "mean_launch_ms": round(kern_ms, 3),
                      "flop_per_interaction": FLOP_PER_INT[K]},
+        "autotuned_shapes": tuned or None,
         "cpu_baseline": cpu,
         "configs": configs,
     }

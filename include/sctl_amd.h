@@ -143,6 +143,13 @@ int sctl_amd_plan_interactions(sctl_amd_plan plan, int64_t* interactions);
 
 int sctl_amd_plan_destroy(sctl_amd_plan plan);
 
+/* Launch knobs of a plan: sources staged through LDS per wave-stage
+ * (64 * sources_per_lane), targets per lane (1 or 2), and a cap on resident
+ * workgroups of 4 waves per CU (0 = no cap).  Chosen at creation from the
+ * tuning database, else defaults; sctl_amd_plan_tune searches them. */
+int sctl_amd_plan_get_shape(sctl_amd_plan plan, int* sources_per_lane, int* targets_per_lane, int* wg_per_cu_cap);
+int sctl_amd_plan_set_shape(sctl_amd_plan plan, int sources_per_lane, int targets_per_lane, int wg_per_cu_cap);
+
 /* ------------------------------------------------------ repeated applies
  * A Krylov solve applies the same operator once per iteration with new
  * densities (lin-solve.txx:146-156 calling BoundaryIntegralOp::
@@ -163,6 +170,27 @@ int sctl_amd_graph_destroy(sctl_amd_graph graph);
 /* Device counter-based generator: out[i] = i-th N(0,1) draw of
  * (seed, stream_id) -- SplitMix64 counters, Box-Muller pairs. */
 int sctl_amd_gen_normal(uint64_t seed, uint64_t stream_id, int64_t n, double* out, void* stream);
+
+/* ----------------------------------------------------------------- tuning
+ * The autotuner times candidate launch shapes on the caller's own data (HIP
+ * events on `stream`) and records the fastest in a database keyed by
+ * operation, kernel, precision and log2 size classes.  The database is kept in
+ * memory and, when a path is set (here or through the SCTL_AMD_TUNE_DB
+ * environment variable), read from and written to that file.  Later calls of
+ * sctl_amd_eval / sctl_amd_plan_create with a matching key use the result. */
+int sctl_amd_tune_db_path(const char* path);
+
+/* Dense Eval: targets per lane {1,2,4} x workgroups-per-CU cap {none,3,2}.
+ * Device pointers only; v_scratch (nt*K1) receives garbage. */
+int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_trg, int64_t nt, const double* r_src,
+                       const double* n_src, const double* v_src, int64_t ns, double* v_scratch, uint32_t flags,
+                       void* stream, int* best_tpl, int* best_occ, double* best_ms);
+
+/* Plan: sources per lane {1,2,4} x targets per lane {1,2} x cap {none,4,2};
+ * leaves the plan configured with the fastest.  v_scratch (nt_total*K1,
+ * device) receives garbage. */
+int sctl_amd_plan_tune(sctl_amd_plan plan, const double* v_src, double* v_scratch, void* stream, int* best_spl,
+                       int* best_tpl, int* best_occ, double* best_ms);
 
 /* -------------------------------------------------------------- profiling
  * Counterpart of the reference's Profile FLOP counter, which Eval and

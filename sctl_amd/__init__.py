@@ -38,7 +38,8 @@ SYMBOLS = [
     "sctl_amd_plan_interactions", "sctl_amd_plan_destroy", "sctl_amd_malloc", "sctl_amd_free", "sctl_amd_memcpy",
     "sctl_amd_stream_create", "sctl_amd_stream_destroy", "sctl_amd_stream_synchronize", "sctl_amd_profile_enable",
     "sctl_amd_profile_read", "sctl_amd_profile_reset", "sctl_amd_plan_graph_create", "sctl_amd_graph_launch",
-    "sctl_amd_graph_interactions", "sctl_amd_graph_destroy", "sctl_amd_gen_normal",
+    "sctl_amd_graph_interactions", "sctl_amd_graph_destroy", "sctl_amd_gen_normal", "sctl_amd_plan_get_shape",
+    "sctl_amd_plan_set_shape", "sctl_amd_tune_db_path", "sctl_amd_tune_eval", "sctl_amd_plan_tune",
 ]
 
 
@@ -89,6 +90,12 @@ def lib():
     L.sctl_amd_graph_interactions.argtypes = [_vp, _i64p]
     L.sctl_amd_graph_destroy.argtypes = [_vp]
     L.sctl_amd_gen_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, _vp, _vp]
+    ip = ctypes.POINTER(i32)
+    L.sctl_amd_plan_get_shape.argtypes = [_vp, ip, ip, ip]
+    L.sctl_amd_plan_set_shape.argtypes = [_vp, i32, i32, i32]
+    L.sctl_amd_tune_db_path.argtypes = [ctypes.c_char_p]
+    L.sctl_amd_tune_eval.argtypes = [i32, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp, ip, ip, _dp]
+    L.sctl_amd_plan_tune.argtypes = [_vp, _vp, _vp, _vp, ip, ip, ip, _dp]
     L.sctl_amd_profile_enable.argtypes = [i32]
     L.sctl_amd_profile_read.argtypes = [_i64p, _i64p, _i64p, ctypes.POINTER(ctypes.c_double)]
     _lib = L
@@ -316,6 +323,23 @@ class Plan:
         check(lib().sctl_amd_plan_interactions(self.handle, ctypes.byref(n)))
         return n.value
 
+    def shape(self):
+        """(sources_per_lane, targets_per_lane, wg_per_cu_cap)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().sctl_amd_plan_get_shape(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_shape(self, sources_per_lane, targets_per_lane, wg_per_cu_cap=0):
+        check(lib().sctl_amd_plan_set_shape(self.handle, sources_per_lane, targets_per_lane, wg_per_cu_cap))
+
+    def tune(self, v_src, v_scratch, stream=None):
+        """Autotunes the launch shape on these device densities; returns (spl, tpl, cap, ms)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        check(lib().sctl_amd_plan_tune(self.handle, v_src.ptr, v_scratch.ptr, stream, ctypes.byref(a), ctypes.byref(b),
+                                       ctypes.byref(c), ctypes.byref(ms)))
+        return a.value, b.value, c.value, ms.value
+
     def graph(self, nsteps, seed, stream_id, v_src, v_trg):
         """Captures nsteps applies with densities regenerated on the device (see Graph)."""
         return Graph(self, nsteps, seed, stream_id, v_src, v_trg)
@@ -362,6 +386,20 @@ class Graph:
             self.destroy()
         except Exception:
             pass
+
+
+def tune_db_path(path):
+    check(lib().sctl_amd_tune_db_path(None if path is None else str(path).encode()))
+
+
+def tune_eval(kernel, r_trg, r_src, n_src, v_src, v_scratch, precision=FP64, digits=-1, flags=0, stream=None):
+    """Autotunes the dense Eval shape for these device arrays; returns (tpl, cap, ms)."""
+    nt, ns = r_trg.nbytes // 24, r_src.nbytes // 24
+    a, b = ctypes.c_int(), ctypes.c_int()
+    ms = ctypes.c_double()
+    check(lib().sctl_amd_tune_eval(kernel, precision, digits, r_trg.ptr, nt, r_src.ptr, _ptr(n_src), v_src.ptr, ns,
+                                   v_scratch.ptr, flags, stream, ctypes.byref(a), ctypes.byref(b), ctypes.byref(ms)))
+    return a.value, b.value, ms.value
 
 
 def gen_normal(seed, stream_id, out, stream=None):

@@ -5,14 +5,14 @@
 // and :1342-1364 evaluate them one OpenMP task each).  Here all blocks of a
 // plan run in ONE persistent launch:
 //
-//  * Work items are (block, chunk of <= 64 targets).  Items are ordered by
+//  * Work items are (block, chunk of <= 64*TPL targets).  Items are ordered by
 //    cost Ns*Nt, largest first (longest-processing-time order), and every
 //    wave of the persistent grid takes the next item from a global atomic
 //    counter, so the tail of the launch is made of the smallest items.
-//  * A wave owns an item.  Chunks of <= 16 or <= 32 targets use G = 4 or 2
-//    lane groups (64/G lanes each, one target per lane): group g takes every
-//    G-th source and the groups' sums are combined with cross-lane shuffles,
-//    so a 16-target block still keeps all 64 lanes busy.
+//  * A wave owns an item; each lane holds TPL targets.  Chunks of <= 16*TPL or
+//    <= 32*TPL targets use G = 4 or 2 lane groups (64/G lanes each): group g
+//    takes every G-th source and the groups' sums are combined with
+//    cross-lane shuffles, so a 16-target block still keeps 64 lanes busy.
 //  * The block's sources are staged through a wave-private LDS region,
 //    64*SPL records at a time (no workgroup barrier: waves run independently).
 //  * Mixed precision: each pair is differenced in FP64 and the difference
@@ -20,6 +20,8 @@
 //    partial is added into FP64 sums with Kahan compensation.  FP64: plain
 //    FP64 records, as in the dense path.
 //
+// The knobs (SPL: LDS stage size, TPL: targets per lane, a cap on resident
+// workgroups per CU) are chosen per plan by the tuner (tune.cpp) or defaults.
 // Every wave leaves the loop when the counter passes the number of items, so
 // the grid always drains.  An apply allocates nothing and issues no memset
 // (the counter is cleared by a one-lane kernel), so it can be captured in a
@@ -45,7 +47,7 @@ constexpr double PADD = 1e100;
 struct Item {
   int32_t block;  // block index
   int32_t t_off;  // first target of the chunk, relative to the block
-  int32_t t_cnt;  // targets in the chunk (<= 64)
+  int32_t t_cnt;  // targets in the chunk (<= 64*TPL)
   int32_t lg;     // log2 of the number of lane groups
 };
 
@@ -71,7 +73,7 @@ __global__ void k_reset_counter(int* counter) {
 }
 
 // ------------------------------------------------------------- FP64 blocks
-template <int KER, int NEWTON, int SPL>
+template <int KER, int NEWTON, int SPL, int TPL>
 __global__ void __launch_bounds__(BWG)
 k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ counter,
              const int64_t* __restrict__ trg_dsp, const int64_t* __restrict__ src_dsp,
@@ -87,14 +89,22 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
     const Item I = items[it];
     const int lg = I.lg, G = 1 << lg, L = 64 >> lg;
     const int g = lane >> (6 - lg), tl = lane & (L - 1);
-    const bool tv = tl < I.t_cnt;
-    const int64_t t = trg_dsp[I.block] + I.t_off + (tv ? tl : 0);
-    const double tx = xt[t * 3 + 0], ty = xt[t * 3 + 1], tz = xt[t * 3 + 2];
+    const int64_t tb = trg_dsp[I.block] + I.t_off;
+    bool tv[TPL];
+    int64_t t[TPL];
+    double tx[TPL], ty[TPL], tz[TPL], acc[TPL][K1];
+#pragma unroll
+    for (int j = 0; j < TPL; j++) {
+      tv[j] = tl + j * L < I.t_cnt;
+      t[j] = tb + (tv[j] ? tl + j * L : 0);
+      tx[j] = xt[t[j] * 3 + 0];
+      ty[j] = xt[t[j] * 3 + 1];
+      tz[j] = xt[t[j] * 3 + 2];
+#pragma unroll
+      for (int k = 0; k < K1; k++) acc[j][k] = 0.0;
+    }
     const int64_t sb = src_dsp[I.block];
     const int nsb = src_cnt[I.block];
-    double acc[K1];
-#pragma unroll
-    for (int k = 0; k < K1; k++) acc[k] = 0.0;
 
     for (int s0 = 0; s0 < nsb; s0 += STAGE) {
       const int len = min(STAGE, nsb - s0);
@@ -119,9 +129,11 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
         }
       }
       wave_sync();
-      double saved[K1];
+      double saved[TPL][K1];
 #pragma unroll
-      for (int k = 0; k < K1; k++) saved[k] = acc[k];
+      for (int j = 0; j < TPL; j++)
+#pragma unroll
+        for (int k = 0; k < K1; k++) saved[j][k] = acc[j][k];
 #pragma unroll 2
       for (int i = g; i < STAGE; i += G) {
         const double* rec = lds + i * R;
@@ -130,29 +142,40 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
         for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
 #pragma unroll
         for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
-        interact<KER, NEWTON, false>(acc, tx - rec[0], ty - rec[1], tz - rec[2], n, f);
+#pragma unroll
+        for (int j = 0; j < TPL; j++)
+          interact<KER, NEWTON, false>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f);
       }
       bool bad = false;
 #pragma unroll
-      for (int k = 0; k < K1; k++) bad |= !__builtin_isfinite(acc[k]);
+      for (int j = 0; j < TPL; j++)
+#pragma unroll
+        for (int k = 0; k < K1; k++) bad |= !__builtin_isfinite(acc[j][k]);
       if (bad) {
 #pragma unroll
-        for (int k = 0; k < K1; k++) acc[k] = saved[k];
+        for (int j = 0; j < TPL; j++)
+#pragma unroll
+          for (int k = 0; k < K1; k++) acc[j][k] = saved[j][k];
         for (int i = g; i < len; i += G) {
           const double* rec = lds + i * R;
           double n[3] = {0, 0, 0}, f[3];
           for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
           for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
-          interact<KER, NEWTON, true>(acc, tx - rec[0], ty - rec[1], tz - rec[2], n, f);
+#pragma unroll
+          for (int j = 0; j < TPL; j++)
+            interact<KER, NEWTON, true>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f);
         }
       }
     }
-    for (int m = L; m < 64; m <<= 1)
 #pragma unroll
-      for (int k = 0; k < K1; k++) acc[k] += shfl_xor_d(acc[k], m);
-    if (g == 0 && tv) {
+    for (int j = 0; j < TPL; j++) {
+      for (int m = L; m < 64; m <<= 1)
 #pragma unroll
-      for (int k = 0; k < K1; k++) out[t * K1 + k] += acc[k] * scale;
+        for (int k = 0; k < K1; k++) acc[j][k] += shfl_xor_d(acc[j][k], m);
+      if (g == 0 && tv[j]) {
+#pragma unroll
+        for (int k = 0; k < K1; k++) out[t[j] * K1 + k] += acc[j][k] * scale;
+      }
     }
   }
 }
@@ -164,7 +187,7 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
 // interaction is FP32 with v_rsq_f32.  The stage's FP32 partial is added into
 // FP64 sums with Kahan compensation; a non-finite partial (r == 0 in the
 // stage) is redone with the r^2 > 0 mask.
-template <int KER, int NEWTON, int SPL>
+template <int KER, int NEWTON, int SPL, int TPL>
 __global__ void __launch_bounds__(BWG)
 k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ counter,
                const int64_t* __restrict__ trg_dsp, const int64_t* __restrict__ src_dsp,
@@ -184,14 +207,22 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
     const Item I = items[it];
     const int lg = I.lg, G = 1 << lg, L = 64 >> lg;
     const int g = lane >> (6 - lg), tl = lane & (L - 1);
-    const bool tv = tl < I.t_cnt;
-    const int64_t t = trg_dsp[I.block] + I.t_off + (tv ? tl : 0);
-    const double tx = xt[t * 3 + 0], ty = xt[t * 3 + 1], tz = xt[t * 3 + 2];
+    const int64_t tb = trg_dsp[I.block] + I.t_off;
+    bool tv[TPL];
+    int64_t t[TPL];
+    double tx[TPL], ty[TPL], tz[TPL], acc[TPL][K1], comp[TPL][K1];
+#pragma unroll
+    for (int j = 0; j < TPL; j++) {
+      tv[j] = tl + j * L < I.t_cnt;
+      t[j] = tb + (tv[j] ? tl + j * L : 0);
+      tx[j] = xt[t[j] * 3 + 0];
+      ty[j] = xt[t[j] * 3 + 1];
+      tz[j] = xt[t[j] * 3 + 2];
+#pragma unroll
+      for (int k = 0; k < K1; k++) acc[j][k] = comp[j][k] = 0.0;
+    }
     const int64_t sb = src_dsp[I.block];
     const int nsb = src_cnt[I.block];
-    double acc[K1], comp[K1];
-#pragma unroll
-    for (int k = 0; k < K1; k++) acc[k] = comp[k] = 0.0;
 
     for (int s0 = 0; s0 < nsb; s0 += STAGE) {
       const int len = min(STAGE, nsb - s0);
@@ -217,10 +248,12 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
         }
       }
       wave_sync();
-      float p[K1];
+      float p[TPL][K1];
 #pragma unroll
-      for (int k = 0; k < K1; k++) p[k] = 0.0f;
-#pragma unroll 4
+      for (int j = 0; j < TPL; j++)
+#pragma unroll
+        for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
+#pragma unroll 2
       for (int i = g; i < STAGE; i += G) {
         const double* pr = pos + i * 4;
         const float* ar = aux + i * AW;
@@ -229,73 +262,103 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
         for (int k = 0; k < ND; k++) n[k] = ar[k];
 #pragma unroll
         for (int k = 0; k < K0; k++) f[k] = ar[ND + k];
-        interact<KER, NEWTON, false>(p, (float)(tx - pr[0]), (float)(ty - pr[1]), (float)(tz - pr[2]), n, f);
+#pragma unroll
+        for (int j = 0; j < TPL; j++)
+          interact<KER, NEWTON, false>(p[j], (float)(tx[j] - pr[0]), (float)(ty[j] - pr[1]), (float)(tz[j] - pr[2]),
+                                       n, f);
       }
       bool bad = false;
 #pragma unroll
-      for (int k = 0; k < K1; k++) bad |= !__builtin_isfinite(p[k]);
+      for (int j = 0; j < TPL; j++)
+#pragma unroll
+        for (int k = 0; k < K1; k++) bad |= !__builtin_isfinite(p[j][k]);
       if (bad) {
 #pragma unroll
-        for (int k = 0; k < K1; k++) p[k] = 0.0f;
+        for (int j = 0; j < TPL; j++)
+#pragma unroll
+          for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
         for (int i = g; i < len; i += G) {
           const double* pr = pos + i * 4;
           const float* ar = aux + i * AW;
           float n[3] = {0, 0, 0}, f[3];
           for (int k = 0; k < ND; k++) n[k] = ar[k];
           for (int k = 0; k < K0; k++) f[k] = ar[ND + k];
-          interact<KER, NEWTON, true>(p, (float)(tx - pr[0]), (float)(ty - pr[1]), (float)(tz - pr[2]), n, f);
+#pragma unroll
+          for (int j = 0; j < TPL; j++)
+            interact<KER, NEWTON, true>(p[j], (float)(tx[j] - pr[0]), (float)(ty[j] - pr[1]), (float)(tz[j] - pr[2]),
+                                        n, f);
         }
       }
 #pragma unroll
-      for (int k = 0; k < K1; k++) {  // Kahan: acc += p
-        const double y = (double)p[k] - comp[k];
-        const double s = acc[k] + y;
-        comp[k] = (s - acc[k]) - y;
-        acc[k] = s;
+      for (int j = 0; j < TPL; j++)
+#pragma unroll
+        for (int k = 0; k < K1; k++) {  // Kahan: acc += p
+          const double y = (double)p[j][k] - comp[j][k];
+          const double s = acc[j][k] + y;
+          comp[j][k] = (s - acc[j][k]) - y;
+          acc[j][k] = s;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < TPL; j++) {
+      double v[K1];
+#pragma unroll
+      for (int k = 0; k < K1; k++) v[k] = acc[j][k] - comp[j][k];
+      for (int m = L; m < 64; m <<= 1)
+#pragma unroll
+        for (int k = 0; k < K1; k++) v[k] += shfl_xor_d(v[k], m);
+      if (g == 0 && tv[j]) {
+#pragma unroll
+        for (int k = 0; k < K1; k++) out[t[j] * K1 + k] += v[k] * scale;
       }
     }
-    double v[K1];
-#pragma unroll
-    for (int k = 0; k < K1; k++) v[k] = acc[k] - comp[k];
-    for (int m = L; m < 64; m <<= 1)
-#pragma unroll
-      for (int k = 0; k < K1; k++) v[k] += shfl_xor_d(v[k], m);
-    if (g == 0 && tv) {
-#pragma unroll
-      for (int k = 0; k < K1; k++) out[t * K1 + k] += v[k] * scale;
-    }
   }
 }
 
-using BlocksF64Fn = void (*)(const Item*, int, int*, const int64_t*, const int64_t*, const int32_t*, const double*,
-                             const double*, const double*, const double*, double*, double);
-using BlocksMixFn = BlocksF64Fn;
+using BlocksFn = void (*)(const Item*, int, int*, const int64_t*, const int64_t*, const int32_t*, const double*,
+                          const double*, const double*, const double*, double*, double);
 
-template <int KER, int NEWTON> BlocksF64Fn f64_blocks(int spl) {
-  return spl >= 4 ? k_blocks_f64<KER, NEWTON, 4> : (spl >= 2 ? k_blocks_f64<KER, NEWTON, 2> : k_blocks_f64<KER, NEWTON, 1>);
+template <int KER, int NEWTON, int SPL> BlocksFn f64_blocks_t(int tpl) {
+  return tpl >= 2 ? k_blocks_f64<KER, NEWTON, SPL, 2> : k_blocks_f64<KER, NEWTON, SPL, 1>;
 }
-template <int KER> BlocksF64Fn f64_blocks(int newton, int spl) {
-  return newton == 0 ? f64_blocks<KER, 0>(spl) : (newton == 1 ? f64_blocks<KER, 1>(spl) : f64_blocks<KER, 2>(spl));
+template <int KER, int NEWTON> BlocksFn f64_blocks_s(int spl, int tpl) {
+  return spl >= 4 ? f64_blocks_t<KER, NEWTON, 4>(tpl)
+                  : (spl >= 2 ? f64_blocks_t<KER, NEWTON, 2>(tpl) : f64_blocks_t<KER, NEWTON, 1>(tpl));
 }
-BlocksF64Fn f64_blocks(int ker, int newton, int spl) {
+template <int KER> BlocksFn f64_blocks_n(int newton, int spl, int tpl) {
+  return newton == 0 ? f64_blocks_s<KER, 0>(spl, tpl)
+                     : (newton == 1 ? f64_blocks_s<KER, 1>(spl, tpl) : f64_blocks_s<KER, 2>(spl, tpl));
+}
+BlocksFn f64_blocks(int ker, int newton, int spl, int tpl) {
   switch (ker) {
-    case LAP_FXU: return f64_blocks<LAP_FXU>(newton, spl);
-    case LAP_DXU: return f64_blocks<LAP_DXU>(newton, spl);
-    case STK_FXU: return f64_blocks<STK_FXU>(newton, spl);
-    default: return f64_blocks<STK_DXU>(newton, spl);
+    case LAP_FXU: return f64_blocks_n<LAP_FXU>(newton, spl, tpl);
+    case LAP_DXU: return f64_blocks_n<LAP_DXU>(newton, spl, tpl);
+    case STK_FXU: return f64_blocks_n<STK_FXU>(newton, spl, tpl);
+    default: return f64_blocks_n<STK_DXU>(newton, spl, tpl);
   }
 }
-template <int KER, int NEWTON> BlocksMixFn mix_blocks(int spl) {
-  return spl >= 4 ? k_blocks_mixed<KER, NEWTON, 4>
-                  : (spl >= 2 ? k_blocks_mixed<KER, NEWTON, 2> : k_blocks_mixed<KER, NEWTON, 1>);
+template <int KER, int NEWTON, int SPL> BlocksFn mix_blocks_t(int tpl) {
+  return tpl >= 2 ? k_blocks_mixed<KER, NEWTON, SPL, 2> : k_blocks_mixed<KER, NEWTON, SPL, 1>;
 }
-BlocksMixFn mix_blocks(int ker, int newton, int spl) {
+template <int KER, int NEWTON> BlocksFn mix_blocks_s(int spl, int tpl) {
+  return spl >= 4 ? mix_blocks_t<KER, NEWTON, 4>(tpl)
+                  : (spl >= 2 ? mix_blocks_t<KER, NEWTON, 2>(tpl) : mix_blocks_t<KER, NEWTON, 1>(tpl));
+}
+BlocksFn mix_blocks(int ker, int newton, int spl, int tpl) {
   switch (ker) {
-    case LAP_FXU: return newton ? mix_blocks<LAP_FXU, 1>(spl) : mix_blocks<LAP_FXU, 0>(spl);
-    case LAP_DXU: return newton ? mix_blocks<LAP_DXU, 1>(spl) : mix_blocks<LAP_DXU, 0>(spl);
-    case STK_FXU: return newton ? mix_blocks<STK_FXU, 1>(spl) : mix_blocks<STK_FXU, 0>(spl);
-    default: return newton ? mix_blocks<STK_DXU, 1>(spl) : mix_blocks<STK_DXU, 0>(spl);
+    case LAP_FXU: return newton ? mix_blocks_s<LAP_FXU, 1>(spl, tpl) : mix_blocks_s<LAP_FXU, 0>(spl, tpl);
+    case LAP_DXU: return newton ? mix_blocks_s<LAP_DXU, 1>(spl, tpl) : mix_blocks_s<LAP_DXU, 0>(spl, tpl);
+    case STK_FXU: return newton ? mix_blocks_s<STK_FXU, 1>(spl, tpl) : mix_blocks_s<STK_FXU, 0>(spl, tpl);
+    default: return newton ? mix_blocks_s<STK_DXU, 1>(spl, tpl) : mix_blocks_s<STK_DXU, 0>(spl, tpl);
   }
+}
+// Static LDS bytes of one workgroup of a blocks kernel.
+size_t blocks_lds(int ker, int precision, int spl) {
+  int K0, K1, ND;
+  kernel_dims(ker, &K0, &K1, &ND);
+  const size_t stage = 64 * (size_t)spl * (BWG / 64);
+  if (precision == SCTL_AMD_MIXED) return stage * (4 * sizeof(double) + (((ND + K0) + 3) & ~3) * sizeof(float));
+  return stage * (((3 + ND + K0) + 1) & ~1) * sizeof(double);
 }
 double fold(int ker, int newton) {
   switch (ker) {
@@ -322,13 +385,15 @@ using namespace sctl_amd;
 struct sctl_amd_plan_s {
   int kernel = 0, precision = SCTL_AMD_FP64, newton = 2, K0 = 1, K1 = 1, ND = 0;
   int device = 0;
-  int64_t nblocks = 0, nt_total = 0, ns_total = 0, interactions = 0;
+  int64_t nblocks = 0, nt_total = 0, ns_total = 0, interactions = 0, nonempty = 0;
+  std::vector<int64_t> trg_cnt, src_cnt;  // host copies (items are rebuilt per targets-per-lane)
   int nitems = 0;
-  int spl = 2;          // sources per lane per stage
+  PlanShape shape;      // resolved knobs
   int grid = 0;         // persistent grid (workgroups)
+  size_t dyn_lds = 0;   // dynamic LDS that enforces shape.occ
   bool has_geometry = false;
   Item* d_items = nullptr;
-  int64_t *d_trg_dsp = nullptr, *d_trg_cnt = nullptr, *d_src_dsp = nullptr;
+  int64_t *d_trg_dsp = nullptr, *d_src_dsp = nullptr;
   int32_t* d_src_cnt = nullptr;
   int* d_counter = nullptr;
   double *d_xt = nullptr, *d_xs = nullptr, *d_xn = nullptr;
@@ -337,7 +402,7 @@ struct sctl_amd_plan_s {
     int cur = -1;
     (void)hipGetDevice(&cur);
     if (cur != device) (void)hipSetDevice(device);
-    void* ptrs[] = {d_items, d_trg_dsp, d_trg_cnt, d_src_dsp, d_src_cnt, d_counter, d_xt, d_xs, d_xn};
+    void* ptrs[] = {d_items, d_trg_dsp, d_src_dsp, d_src_cnt, d_counter, d_xt, d_xs, d_xn};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
@@ -346,19 +411,63 @@ struct sctl_amd_plan_s {
 
 namespace sctl_amd {
 
-// Plan knobs: sources per lane per stage and the persistent grid size
-// (workgroups per CU from the occupancy calculator).
-int plan_configure(sctl_amd_plan_s* p, int spl) {
+namespace {
+
+// Work items for `tpl` targets per lane: chunks of <= 64*tpl targets, lane
+// groups for small chunks, longest (Ns x lanes) first.
+std::vector<Item> build_items(const sctl_amd_plan_s* p, int tpl) {
+  std::vector<Item> items;
+  std::vector<int64_t> cost;
+  const int64_t chunk = 64LL * tpl;
+  for (int64_t b = 0; b < p->nblocks; b++) {
+    const int64_t tc = p->trg_cnt[b], sc = p->src_cnt[b];
+    if (!tc || !sc) continue;
+    for (int64_t t0 = 0; t0 < tc; t0 += chunk) {
+      const int cnt = (int)std::min<int64_t>(chunk, tc - t0);
+      const int lg = cnt <= 16 * tpl ? 2 : (cnt <= 32 * tpl ? 1 : 0);
+      items.push_back(Item{(int32_t)b, (int32_t)t0, cnt, lg});
+      cost.push_back((int64_t)(64 >> lg) * tpl * sc);
+    }
+  }
+  std::vector<size_t> idx(items.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+  std::vector<Item> sorted(items.size());
+  for (size_t i = 0; i < idx.size(); i++) sorted[i] = items[idx[i]];
+  return sorted;
+}
+
+BlocksFn plan_fn(const sctl_amd_plan_s* p, const PlanShape& s) {
+  return p->precision == SCTL_AMD_MIXED ? mix_blocks(p->kernel, p->newton, s.spl, s.tpl)
+                                        : f64_blocks(p->kernel, p->newton, s.spl, s.tpl);
+}
+
+}  // namespace
+
+// Applies knobs to a plan: rebuilds the work items for the targets per lane
+// and sizes the persistent grid from the occupancy calculator (with the cap).
+int plan_configure(sctl_amd_plan_s* p, PlanShape s) {
   DeviceInfo dev;
   SCTL_TRY(device_info(&dev));
-  p->spl = spl;
-  int per_cu = 0;
-  if (p->precision == SCTL_AMD_MIXED) {
-    SCTL_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)mix_blocks(p->kernel, p->newton, spl), BWG, 0));
-  } else {
-    SCTL_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)f64_blocks(p->kernel, p->newton, spl), BWG, 0));
+  s.spl = s.spl >= 4 ? 4 : (s.spl >= 2 ? 2 : 1);
+  s.tpl = s.tpl >= 2 ? 2 : 1;
+  s.occ = std::max(0, s.occ);
+  if (!p->d_items || s.tpl != p->shape.tpl) {
+    const std::vector<Item> items = build_items(p, s.tpl);
+    SCTL_CHECK_ARG(items.size() < 0x7fffffffULL, "too many work items");
+    if (p->d_items) SCTL_HIP_TRY(hipFree(p->d_items));
+    p->d_items = nullptr;
+    SCTL_TRY(dev_upload(&p->d_items, items));
+    p->nitems = (int)items.size();
   }
+  const BlocksFn fn = plan_fn(p, s);
+  size_t dyn = 0;
+  SCTL_TRY(occupancy_lds((const void*)fn, s.occ, blocks_lds(p->kernel, p->precision, s.spl), &dyn));
+  int per_cu = 0;
+  SCTL_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, BWG, dyn));
   p->grid = std::max(1, per_cu) * dev.num_cus;
+  p->dyn_lds = dyn;
+  p->shape = s;
   return SCTL_AMD_SUCCESS;
 }
 
@@ -372,26 +481,22 @@ int plan_sizes(sctl_amd_plan p, int64_t* ns, int64_t* nt, int* K0, int* K1, int6
   return SCTL_AMD_SUCCESS;
 }
 int plan_kernel(sctl_amd_plan p) { return p ? p->kernel : -1; }
+int plan_precision(sctl_amd_plan p) { return p ? p->precision : -1; }
+int64_t plan_nonempty_blocks(sctl_amd_plan p) { return p ? p->nonempty : 0; }
+PlanShape plan_shape(sctl_amd_plan p) { return p ? p->shape : PlanShape(); }
 
 int plan_launch(sctl_amd_plan_s* p, const double* v_src, double* v_trg, hipStream_t s) {
   if (p->nitems == 0) return SCTL_AMD_SUCCESS;
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, p->d_counter);
   SCTL_HIP_TRY(hipGetLastError());
   const double scale = kernel_scale(p->kernel) * fold(p->kernel, p->newton);
-  const int grid = std::min<int64_t>(p->grid, ((int64_t)p->nitems + 3) / 4);
+  const int grid = (int)std::min<int64_t>(p->grid, ((int64_t)p->nitems + 3) / 4);
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   SCTL_HIP_TRY(hipStreamIsCapturing(s, &cs));
   if (cs == hipStreamCaptureStatusNone) profile_count(p->kernel, p->interactions);  // graphs count per launch
   ProfileScope prof(s);
-  if (p->precision == SCTL_AMD_MIXED) {
-    hipLaunchKernelGGL(mix_blocks(p->kernel, p->newton, p->spl), dim3(grid), dim3(BWG), 0, s, p->d_items, p->nitems,
-                       p->d_counter, p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, v_src,
-                       v_trg, scale);
-  } else {
-    hipLaunchKernelGGL(f64_blocks(p->kernel, p->newton, p->spl), dim3(grid), dim3(BWG), 0, s, p->d_items, p->nitems,
-                       p->d_counter, p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, v_src,
-                       v_trg, scale);
-  }
+  hipLaunchKernelGGL(plan_fn(p, p->shape), dim3(grid), dim3(BWG), p->dyn_lds, s, p->d_items, p->nitems, p->d_counter,
+                     p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, v_src, v_trg, scale);
   SCTL_HIP_TRY(hipGetLastError());
   return SCTL_AMD_SUCCESS;
 }
@@ -414,6 +519,7 @@ int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int dig
   SCTL_CHECK_ARG(digits >= -1, "digits must be -1 or >= 0");
   SCTL_CHECK_ARG(nblocks >= 0 && nt_total >= 0 && ns_total >= 0, "negative size");
   SCTL_CHECK_ARG(nblocks == 0 || (trg_dsp && trg_cnt && src_dsp && src_cnt), "NULL block array");
+  SCTL_CHECK_ARG(nblocks < 0x7fffffffLL, "too many blocks");
   if (precision == SCTL_AMD_AUTO) precision = (digits >= 0 && digits <= 7) ? SCTL_AMD_MIXED : SCTL_AMD_FP64;
 
   std::vector<int64_t> order(nblocks);
@@ -421,8 +527,10 @@ int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int dig
   for (int64_t b = 0; b < nblocks; b++) {
     SCTL_CHECK_ARG(trg_cnt[b] >= 0 && src_cnt[b] >= 0 && trg_dsp[b] >= 0 && src_dsp[b] >= 0,
                    "negative block offset or count (block " + std::to_string(b) + ")");
-    SCTL_CHECK_ARG(trg_dsp[b] + trg_cnt[b] <= nt_total, "block target range exceeds nt_total (block " + std::to_string(b) + ")");
-    SCTL_CHECK_ARG(src_dsp[b] + src_cnt[b] <= ns_total, "block source range exceeds ns_total (block " + std::to_string(b) + ")");
+    SCTL_CHECK_ARG(trg_dsp[b] + trg_cnt[b] <= nt_total,
+                   "block target range exceeds nt_total (block " + std::to_string(b) + ")");
+    SCTL_CHECK_ARG(src_dsp[b] + src_cnt[b] <= ns_total,
+                   "block source range exceeds ns_total (block " + std::to_string(b) + ")");
     SCTL_CHECK_ARG(src_cnt[b] <= 0x7fffffffLL && trg_cnt[b] <= 0x7fffffffLL, "block too large");
   }
   std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return trg_dsp[a] < trg_dsp[b]; });
@@ -431,28 +539,6 @@ int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int dig
     if (trg_cnt[a] && trg_cnt[b])
       SCTL_CHECK_ARG(trg_dsp[a] + trg_cnt[a] <= trg_dsp[b], "target ranges of blocks overlap");
   }
-
-  // Work items: chunks of <= 64 targets, lane groups for small chunks,
-  // longest first.
-  std::vector<Item> items;
-  std::vector<int64_t> cost;
-  int64_t inter = 0;
-  for (int64_t b = 0; b < nblocks; b++) {
-    if (!trg_cnt[b] || !src_cnt[b]) continue;
-    inter += trg_cnt[b] * src_cnt[b];
-    for (int64_t t0 = 0; t0 < trg_cnt[b]; t0 += 64) {
-      const int cnt = (int)std::min<int64_t>(64, trg_cnt[b] - t0);
-      const int lg = cnt <= 16 ? 2 : (cnt <= 32 ? 1 : 0);
-      items.push_back(Item{(int32_t)b, (int32_t)t0, cnt, lg});
-      cost.push_back((int64_t)(64 >> lg) * src_cnt[b]);
-    }
-  }
-  SCTL_CHECK_ARG(items.size() < 0x7fffffffULL, "too many work items");
-  std::vector<size_t> idx(items.size());
-  std::iota(idx.begin(), idx.end(), 0);
-  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-  std::vector<Item> sorted(items.size());
-  for (size_t i = 0; i < idx.size(); i++) sorted[i] = items[idx[i]];
 
   DeviceInfo dev;
   SCTL_TRY(device_info(&dev));
@@ -467,15 +553,17 @@ int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int dig
   p->nblocks = nblocks;
   p->nt_total = nt_total;
   p->ns_total = ns_total;
-  p->interactions = inter;
-  p->nitems = (int)sorted.size();
-  std::vector<int64_t> v_td(trg_dsp, trg_dsp + nblocks), v_tc(trg_cnt, trg_cnt + nblocks), v_sd(src_dsp, src_dsp + nblocks);
+  p->trg_cnt.assign(trg_cnt, trg_cnt + nblocks);
+  p->src_cnt.assign(src_cnt, src_cnt + nblocks);
+  for (int64_t b = 0; b < nblocks; b++) {
+    p->interactions += trg_cnt[b] * src_cnt[b];
+    p->nonempty += (trg_cnt[b] && src_cnt[b]) ? 1 : 0;
+  }
+  std::vector<int64_t> v_td(trg_dsp, trg_dsp + nblocks), v_sd(src_dsp, src_dsp + nblocks);
   std::vector<int32_t> v_sc(nblocks);
   for (int64_t b = 0; b < nblocks; b++) v_sc[b] = (int32_t)src_cnt[b];
   int rc = SCTL_AMD_SUCCESS;
-  if (!rc) rc = dev_upload(&p->d_items, sorted);
   if (!rc) rc = dev_upload(&p->d_trg_dsp, v_td);
-  if (!rc) rc = dev_upload(&p->d_trg_cnt, v_tc);
   if (!rc) rc = dev_upload(&p->d_src_dsp, v_sd);
   if (!rc) rc = dev_upload(&p->d_src_cnt, v_sc);
   if (!rc) {
@@ -485,7 +573,15 @@ int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int dig
       rc = SCTL_AMD_ERROR_OUT_OF_MEMORY;
     }
   }
-  if (!rc) rc = plan_configure(p, p->precision == SCTL_AMD_MIXED ? 2 : 1);
+  if (!rc) {
+    PlanShape s;
+    if (!lookup_plan_shape(kernel, precision, p->interactions, p->nonempty, &s)) {
+      s.spl = precision == SCTL_AMD_MIXED ? 2 : 1;
+      s.tpl = 1;
+      s.occ = 0;
+    }
+    rc = plan_configure(p, s);
+  }
   if (rc) {
     delete p;
     return rc;
@@ -542,6 +638,27 @@ int sctl_amd_plan_interactions(sctl_amd_plan p, int64_t* interactions) {
   SCTL_CHECK_ARG(p && interactions, "NULL argument");
   *interactions = p->interactions;
   return SCTL_AMD_SUCCESS;
+}
+
+int sctl_amd_plan_get_shape(sctl_amd_plan p, int* sources_per_lane, int* targets_per_lane, int* wg_per_cu_cap) {
+  SCTL_CHECK_ARG(p, "plan is NULL");
+  if (sources_per_lane) *sources_per_lane = p->shape.spl;
+  if (targets_per_lane) *targets_per_lane = p->shape.tpl;
+  if (wg_per_cu_cap) *wg_per_cu_cap = p->shape.occ;
+  return SCTL_AMD_SUCCESS;
+}
+
+int sctl_amd_plan_set_shape(sctl_amd_plan p, int sources_per_lane, int targets_per_lane, int wg_per_cu_cap) {
+  SCTL_CHECK_ARG(p, "plan is NULL");
+  SCTL_CHECK_ARG(sources_per_lane == 1 || sources_per_lane == 2 || sources_per_lane == 4,
+                 "sources_per_lane must be 1, 2 or 4");
+  SCTL_CHECK_ARG(targets_per_lane == 1 || targets_per_lane == 2, "targets_per_lane must be 1 or 2");
+  SCTL_CHECK_ARG(wg_per_cu_cap >= 0, "wg_per_cu_cap must be >= 0");
+  PlanShape s;
+  s.spl = sources_per_lane;
+  s.tpl = targets_per_lane;
+  s.occ = wg_per_cu_cap;
+  return plan_configure(p, s);
 }
 
 int sctl_amd_plan_destroy(sctl_amd_plan p) {

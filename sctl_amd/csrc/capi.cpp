@@ -201,11 +201,11 @@ int sctl_amd_eval(int kernel, int precision, int digits, const double* r_trg, in
     SCTL_TRY(stage_in(b_vt, v_trg, nt * m.K1, s, &vt_c));
     vt = b_vt.as<double>();
   }
-  const DenseShape shape = pick_dense_shape(kernel, precision, nt, ns);
+  const int newton = precision == SCTL_AMD_FP64 ? newton_steps_f64(digits) : ((flags & SCTL_AMD_MIXED_NEWTON) ? 1 : 0);
+  const DenseShape shape = pick_dense_shape(kernel, precision, newton, nt, ns);
   if (precision == SCTL_AMD_FP64) {
-    SCTL_TRY(launch_eval_f64(kernel, newton_steps_f64(digits), rt, nt, rs, nsrc, vs, ns, vt, shape, s));
+    SCTL_TRY(launch_eval_f64(kernel, newton, rt, nt, rs, nsrc, vs, ns, vt, shape, s));
   } else {
-    const int newton = (flags & SCTL_AMD_MIXED_NEWTON) ? 1 : 0;
     SCTL_TRY(launch_eval_mixed(kernel, newton, rt, nt, rs, nsrc, vs, ns, vt, shape, s));
   }
   if (host) {

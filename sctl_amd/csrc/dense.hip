@@ -206,6 +206,14 @@ EvalF64Fn f64_fn(int ker, int newton, int ti) {
   }
 }
 
+int rec_words_f64(int ker) {
+  switch (ker) {
+    case LAP_FXU: return rec_f64<LAP_FXU>();
+    case LAP_DXU: return rec_f64<LAP_DXU>();
+    case STK_FXU: return rec_f64<STK_FXU>();
+    default: return rec_f64<STK_DXU>();
+  }
+}
 template <int KER> KMatFn kmat_fn(int newton) {
   switch (newton) {
     case 0: return k_kernel_matrix<KER, 0>;
@@ -263,14 +271,17 @@ int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const 
   int64_t nsplit, tps;
   split_sources(tb, ntiles, dev.num_cus, &nsplit, &tps);
   const double scale = kernel_scale(ker) * fold(ker, newton);
+  const EvalF64Fn fn = f64_fn(ker, newton, ti);
+  size_t dyn = 0;
+  SCTL_TRY(occupancy_lds((const void*)fn, shape.occ, sizeof(double) * TILE * rec_words_f64(ker), &dyn));
   Scratch part;
   if (nsplit > 1) SCTL_TRY(part.alloc(sizeof(double) * nsplit * nt * K1, stream));
   dim3 grid((unsigned)tb, (unsigned)nsplit);
   profile_count(ker, nt * ns);
   {
     ProfileScope prof(stream);
-    hipLaunchKernelGGL(f64_fn(ker, newton, ti), grid, dim3(WG), 0, stream, r_trg, nt, r_src, n_src, v_src, ns,
-                       tps * TILE, nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale);
+    hipLaunchKernelGGL(fn, grid, dim3(WG), dyn, stream, r_trg, nt, r_src, n_src, v_src, ns, tps * TILE,
+                       nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale);
   }
   SCTL_HIP_TRY(hipGetLastError());
   if (nsplit > 1) {

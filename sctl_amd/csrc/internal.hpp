@@ -79,8 +79,14 @@ class Scratch {
 
 // Launch-shape knobs of the dense kernels (chosen by the tuner or defaults).
 struct DenseShape {
-  int tpl = 1;  // targets per lane
+  int tpl = 0;  // targets per lane: 1, 2 or 4 (0: default for the size)
+  int occ = 0;  // cap on resident workgroups per CU (0: no cap); enforced with dynamic LDS
 };
+
+// Dynamic LDS bytes that cap a kernel with `static_lds` bytes of LDS at `occ`
+// resident workgroups per CU (0 when occ <= 0).  Sets the kernel's dynamic-LDS
+// limit when the request exceeds the default.
+int occupancy_lds(const void* kernel, int occ, size_t static_lds, size_t* dyn_bytes);
 
 // Dense Eval launchers (device pointers, stream-ordered).  v_trg is
 // accumulated: v_trg[t*K1+k] += scale * sum.
@@ -97,8 +103,17 @@ int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, c
 // writes a permutation of [0, n).  Implemented in sort.hip.
 int morton_order(const double* pts, int64_t n, uint32_t* perm, hipStream_t stream);
 
-// Shape defaults and tuning-database lookup (tune.cpp).
-DenseShape pick_dense_shape(int ker, int precision, int64_t nt, int64_t ns);
+// Shape defaults and tuning-database lookup (tune.cpp).  `newton` is the
+// Newton count of the path (part of the tuning key).
+DenseShape pick_dense_shape(int ker, int precision, int newton, int64_t nt, int64_t ns);
+
+// Batched plan knobs (tune.cpp / batched.hip).
+struct PlanShape {
+  int spl = 0;  // sources per lane per LDS stage: 1, 2 or 4 (stage = 64*spl sources)
+  int tpl = 0;  // targets per lane: 1 or 2
+  int occ = 0;  // cap on resident workgroups per CU (0: no cap)
+};
+bool lookup_plan_shape(int ker, int precision, int64_t interactions, int64_t nitems, PlanShape* out);
 
 // Profiling (profile.cpp): count(...) is always on; a ProfileScope records
 // HIP events around one launch while timing is enabled.

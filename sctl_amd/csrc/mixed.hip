@@ -405,13 +405,16 @@ int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, cons
   nsplit = (ntiles + tps - 1) / tps;
 
   const double scale = kernel_scale(ker) * mixed_fold(ker, newton);
+  const EvalMixFn fn = mix_fn(ker, newton, ti);
+  size_t dyn = 0;
+  SCTL_TRY(occupancy_lds((const void*)fn, shape.occ, sizeof(float4) * TILE * Q, &dyn));
   Scratch part;
   if (nsplit > 1) SCTL_TRY(part.alloc(sizeof(double) * nsplit * nt * K1, stream));
   profile_count(ker, nt * ns);
   {
     ProfileScope prof(stream);
-    hipLaunchKernelGGL(mix_fn(ker, newton, ti), dim3((unsigned)tb, (unsigned)nsplit), dim3(WG), 0, stream, r_trg,
-                       nt, tperm.as<uint32_t>(), packed.as<float4>(), meta.as<double4>(), ntiles, tps,
+    hipLaunchKernelGGL(fn, dim3((unsigned)tb, (unsigned)nsplit), dim3(WG), dyn, stream, r_trg, nt,
+                       tperm.as<uint32_t>(), packed.as<float4>(), meta.as<double4>(), ntiles, tps,
                        nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale);
   }
   SCTL_HIP_TRY(hipGetLastError());
