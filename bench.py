@@ -205,6 +205,51 @@ def graph_pair(kernel, B, stream, nsteps, seed=0):
     return res
 
 
+def near_ops(kernel, B, stream, reps):
+    """BoundaryIntegralOp's near-field operator over one config's blocks, on
+    the device: the block matrices K_near (target-major, FP64) written
+    (store), updated (KNearSubtract) and applied (ComputeNearInterac's GEMV).
+    HBM-bound: bytes are 8 per matrix entry written (store), read + written
+    (subtract) and read (GEMV, plus its F and U_near).  Parity: the GEMV of
+    the stored matrices equals the plan's direct apply of w*F."""
+    _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
+    nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
+    P = S.Plan(kernel, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns)
+    geo = [S.DeviceArray.from_host(B[a]) for a in ("r_trg", "r_src", "n_src")]
+    P.set_geometry(geo[0], geo[1], geo[2] if ND else None, stream=stream)
+    k_len = P.near_layout()
+    w = W.uniform(0, 40, ns, 0.5, 1.5)
+    dw = S.DeviceArray.from_host(w)
+    dK = S.DeviceArray(k_len)
+    dF = S.DeviceArray.from_host(B["v_src"])
+    dU = S.DeviceArray.from_host(np.zeros(nt * K1))
+    peak = 8000.0
+    res = {"matrix_entries": int(k_len), "matrix_GB": round(k_len * 8 / 1e9, 2)}
+    P.near_matrix(dK, dw, None, S.NEAR_STORE, stream=stream)  # warm-up, and the matrices the GEMV uses
+    S.synchronize(stream)
+    for name, mode, bpe in (("store", S.NEAR_STORE, 8), ("subtract", S.NEAR_SUBTRACT, 16)):
+        ms = time_once(lambda: P.near_matrix(dK, dw, None, mode, stream=stream), stream, reps)
+        gbs = k_len * bpe / ms / 1e6
+        res[name] = {"ms": round(ms, 3), "GB_per_s": round(gbs, 1), "frac_hbm_peak": round(gbs / peak, 3)}
+    P.near_matrix(dK, dw, None, S.NEAR_STORE, stream=stream)
+    P.near_gemv(dK, dF, dU, stream=stream)
+    S.synchronize(stream)
+    via_matrix = dU.download()
+    ms = time_once(lambda: P.near_gemv(dK, dF, dU, stream=stream), stream, reps)
+    gbs = (k_len * 8 + ns * K0 * 8 + nt * K1 * 8) / ms / 1e6
+    res["gemv"] = {"ms": round(ms, 3), "GB_per_s": round(gbs, 1), "frac_hbm_peak": round(gbs / peak, 3)}
+    o = S.DeviceArray.from_host(np.zeros(nt * K1))
+    dwf = S.DeviceArray.from_host(np.repeat(w, K0) * B["v_src"])
+    P.apply(dwf, o, stream=stream)
+    S.synchronize(stream)
+    e_inf, e_rms = W.rel_errors(via_matrix, o.download())
+    res["gemv_of_stored_vs_direct_apply"] = {"max_rel": float("%.3g" % e_inf), "rms_rel": float("%.3g" % e_rms)}
+    for a in (dK, dF, dU, dw, dwf, o) + tuple(geo):
+        a.free()
+    P.destroy()
+    return res
+
+
 def other_configs(stream, quick=False):
     """The other configurations: measured, not the headline."""
     import oracle
@@ -236,9 +281,13 @@ def other_configs(stream, quick=False):
     log("config 4: batched Laplace DxU, 4096 torus blocks")
     B = W.torus_blocks(64, 64, seed=0, k0=1)
     out["c4_batched_laplace_dxu_4096"], _ = plan_pair(S.LAPLACE3D_DXU, B, stream, reps=5)
+    log("config 4 blocks: near-field matrices (KNearSubtract, GEMV), Laplace DxU")
+    out["c4_near_matrix_laplace_dxu_4096"] = near_ops(S.LAPLACE3D_DXU, B, stream, reps=3)
     log("config 5: batched Stokes DxU, 2048 torus blocks, 100 applies in one hipGraph")
     B = W.torus_blocks(64, 32, seed=0, k0=3)
     out["c5_stokes_dxu_graph_2048x100"] = graph_pair(S.STOKES3D_DXU, B, stream, nsteps=20 if quick else 100)
+    log("config 5 blocks: near-field matrices (KNearSubtract, GEMV), Stokes DxU")
+    out["c5_near_matrix_stokes_dxu_2048"] = near_ops(S.STOKES3D_DXU, B, stream, reps=2)
     return out
 
 

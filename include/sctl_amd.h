@@ -150,6 +150,76 @@ int sctl_amd_plan_destroy(sctl_amd_plan plan);
 int sctl_amd_plan_get_shape(sctl_amd_plan plan, int* sources_per_lane, int* targets_per_lane, int* wg_per_cu_cap);
 int sctl_amd_plan_set_shape(sctl_amd_plan plan, int sources_per_lane, int targets_per_lane, int wg_per_cu_cap);
 
+/* ------------------------------------------------------ near-field operator
+ * BoundaryIntegralOp's near interactions (boundary_integral.txx:1009-1058,
+ * :1165-1240, :1311-1381).  Per element e the reference keeps a target-major
+ * matrix K_near_e of near_elem_cnt[e]*KDIM1_ rows by elem_nds_cnt[e]*KDIM0
+ * columns (KDIM1_ = KDIM1, or KDIM1/3 when the output is contracted with the
+ * target normal, trg_normal_dot_prod_), all elements' matrices concatenated
+ * in element order (K_near_dsp, :1034-1058).  A plan whose block b is the
+ * near block (elem, t0, cnt) -- targets [near_elem_dsp[elem]+t0, +cnt) of
+ * Xtrg_near, sources [elem_nds_dsp_far[elem], +elem_nds_cnt_far[elem]) of
+ * X_far -- addresses block b's rows of K_near_elem as one row-major
+ * (cnt*KDIM1_) x (src_cnt*KDIM0) matrix. */
+
+/* Replaces the near-block partition of BoundaryIntegralOp::SetupNear
+ * (boundary_integral.txx:1009-1032): grain = max(1, N_near/(4*nthreads)),
+ * N_near = near_elem_dsp[nelem-1] + near_elem_cnt[nelem-1]; element e's
+ * near targets are cut into blocks (e, t0, min(grain, cnt-t0)) for
+ * t0 = 0, grain, ...; blocks are listed in element order.  Pass the
+ * reference's thread count (SCTL_GET_MAX_THREADS()) for identical blocks.
+ * *nblk receives the block count; the three block arrays (host, nblk
+ * entries) are filled when given -- call once with NULLs to size them.
+ * Host-only; touches no device. */
+int sctl_amd_near_blocks(int64_t nelem, const int64_t* near_elem_cnt, const int64_t* near_elem_dsp, int nthreads,
+                         int64_t* nblk, int64_t* blk_elem, int64_t* blk_t0, int64_t* blk_cnt);
+
+/* Sets where each block's matrix lives in K_near: blk_off[b] is the offset
+ * (in doubles) of block b's first entry, -1 for a block without a matrix
+ * (an element list with MatrixFree(), :1047-1048); NULL packs the blocks in
+ * block order, which is the reference's layout when the blocks are
+ * sctl_amd_near_blocks' blocks of matrix elements
+ * (K_near_dsp[e]*KDIM0*KDIM1_ + t0*KDIM1_*src_dof, :1187-1189).
+ * trg_normal_dot selects the target-normal contraction (Stokes kernels only).
+ * *k_len receives the length K_near must have.  Block matrices must not
+ * overlap. */
+int sctl_amd_plan_near_layout(sctl_amd_plan plan, const int64_t* blk_off, int trg_normal_dot, int64_t* k_len);
+
+#define SCTL_AMD_NEAR_SUBTRACT 0 /* K_near_ -= w * M^T  (KNearSubtract, :1165-1240) */
+#define SCTL_AMD_NEAR_STORE 1    /* K_near_  = w * M^T */
+
+/* Replaces the KNearSubtract loop of BoundaryIntegralOp::SetupNear
+ * (boundary_integral.txx:1165-1240) with the identity
+ * FarFieldDensityOperatorTranspose (:531-533): for every block with a matrix,
+ *   K_near_[t*KDIM1_+k1][s*KDIM0+k0] -= wts[s] * M[s*KDIM0+k0][t*KDIM1+k1]
+ * with M the block's KernelMatrix at full precision (:1211), or with the
+ * target-normal layout
+ *   K_near_[t][s*KDIM0+k0] -= sum_l M[s*KDIM0+k0][t*3+l] * wts[s] * n_trg[t][l]
+ * (:1193-1209).  wts (ns_total; NULL = all ones) is indexed like the plan's
+ * sources, n_trg (nt_total x 3) like its targets.  FP64. */
+int sctl_amd_plan_near_matrix(sctl_amd_plan plan, int mode, const double* wts, const double* n_trg, double* K,
+                              uint32_t flags, void* stream);
+
+/* Replaces the matrix route of BoundaryIntegralOp::ComputeNearInterac
+ * (boundary_integral.txx:1324-1340): for every block,
+ *   U_near[(trg_dsp[b])*KDIM1_ + r] = sum_j K_near_[r][j] * F[src_dsp[b]*KDIM0 + j]
+ * (overwrites; zero rows for blocks without a matrix or without sources).
+ * F (ns_total*KDIM0) holds the element-node densities, U_near
+ * (nt_total*KDIM1_) the near targets' values.  FP64. */
+int sctl_amd_plan_near_gemv(sctl_amd_plan plan, const double* K, const double* F, double* U_near, uint32_t flags,
+                            void* stream);
+
+/* Replaces ScatterForward + the per-target sum of ComputeNearInterac
+ * (boundary_integral.txx:1368-1379; comm.txx:1497-1508 on one process):
+ *   U[i*dof+k] += sum_{j < near_trg_cnt[i]} U_near[scatter_index[near_trg_dsp[i]+j]*dof + k]
+ * with the additions in the reference's order (bit-identical results).
+ * dof is 1 or 3; U_near has n_near*dof entries, scatter_index n_scatter.
+ * With HOST_PTRS out-of-range indices are an error; with device pointers
+ * they are skipped. */
+int sctl_amd_near_accumulate(int64_t ntrg, int dof, const int64_t* near_trg_cnt, const int64_t* near_trg_dsp,
+                             int64_t n_scatter, const int64_t* scatter_index, int64_t n_near, const double* U_near,
+                             double* U, uint32_t flags, void* stream);
+
 /* ------------------------------------------------------ repeated applies
  * A Krylov solve applies the same operator once per iteration with new
  * densities (lin-solve.txx:146-156 calling BoundaryIntegralOp::

@@ -47,6 +47,14 @@ def _c(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_long)) if a is not None and a.size else None
+
+
 class Oracle:
     """One build of the C restatement (liboracle.so, or a mutant build)."""
 
@@ -63,6 +71,15 @@ class Oracle:
         L.oracle_scale.argtypes = [ctypes.c_int]
         L.oracle_scale.restype = ctypes.c_double
         L.oracle_newton_iters_avx512.argtypes = [ctypes.c_int]
+        _lp = ctypes.POINTER(ctypes.c_long)
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_near_blocks.argtypes = [_L, _lp, _lp, ctypes.c_int, _lp, _lp, _lp]
+        L.oracle_near_blocks.restype = _L
+        L.oracle_near_layout.argtypes = [ctypes.c_int, ctypes.c_int, _L, _lp, _lp, _ip, _lp, _lp]
+        L.oracle_near_layout.restype = _L
+        L.oracle_near_subtract.argtypes = [ctypes.c_int, ctypes.c_int, _L] + [_lp] * 9 + [_ip] + [_dp] * 6
+        L.oracle_near_gemv.argtypes = [ctypes.c_int, ctypes.c_int, _L] + [_lp] * 8 + [_dp] * 3
+        L.oracle_near_accumulate.argtypes = [_L, ctypes.c_int, _lp, _lp, _L, _lp, _dp, _dp]
         self.lib = L
         self.path = path
 
@@ -105,6 +122,70 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle_kernel_matrix failed rc=%d" % rc)
         return M
+
+    # ------------------------------------------------------- near field
+    # Arguments are the reference's own arrays (boundary_integral.txx).
+    def near_blocks(self, near_elem_cnt, near_elem_dsp, nthreads):
+        """(near_blk_elem, near_blk_t0, near_blk_cnt), boundary_integral.txx:1009-1032."""
+        cnt, dsp = _i(near_elem_cnt), _i(near_elem_dsp)
+        n = self.lib.oracle_near_blocks(cnt.size, _ip(cnt), _ip(dsp), nthreads, None, None, None)
+        e, t0, c = (np.zeros(n, dtype=np.int64) for _ in range(3))
+        self.lib.oracle_near_blocks(cnt.size, _ip(cnt), _ip(dsp), nthreads, _ip(e), _ip(t0), _ip(c))
+        return e, t0, c
+
+    def near_layout(self, kernel, normal_dot, near_elem_cnt, elem_nds_cnt, matrix_free=None):
+        """(K_near_cnt, K_near_dsp, len(K_near)), boundary_integral.txx:1039-1057."""
+        cnt, nds = _i(near_elem_cnt), _i(elem_nds_cnt)
+        mf = None if matrix_free is None else np.ascontiguousarray(matrix_free, dtype=np.int32)
+        kc, kd = np.zeros(cnt.size, dtype=np.int64), np.zeros(cnt.size, dtype=np.int64)
+        n = self.lib.oracle_near_layout(kernel, int(normal_dot), cnt.size, _ip(cnt), _ip(nds),
+                                        None if mf is None else mf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                        _ip(kc), _ip(kd))
+        if n < 0:
+            raise ValueError("bad near layout")
+        return kc, kd, n
+
+    def near_subtract(self, kernel, normal_dot, blocks, near_elem_dsp, elem_nds_cnt, elem_nds_cnt_far,
+                      elem_nds_dsp_far, K_near_cnt, K_near_dsp, Xtrg_near, Xn_trg_near, X_far, Xn_far, wts_far,
+                      K_near, matrix_free=None):
+        """KNearSubtract (boundary_integral.txx:1165-1240) on a copy of K_near."""
+        be, bt, bc = (_i(a) for a in blocks)
+        arrs = [_i(a) for a in (near_elem_dsp, elem_nds_cnt, elem_nds_cnt_far, elem_nds_dsp_far, K_near_cnt,
+                                K_near_dsp)]
+        mf = None if matrix_free is None else np.ascontiguousarray(matrix_free, dtype=np.int32)
+        K = np.ascontiguousarray(K_near, dtype=np.float64).copy()
+        f = [_c(a) for a in (Xtrg_near, Xn_trg_near, X_far, Xn_far, wts_far)]
+        rc = self.lib.oracle_near_subtract(kernel, int(normal_dot), be.size, _ip(be), _ip(bt), _ip(bc),
+                                           *[_ip(a) for a in arrs],
+                                           None if mf is None else mf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                           *[_p(a) for a in f], _p(K))
+        if rc != 0:
+            raise RuntimeError("oracle_near_subtract failed rc=%d" % rc)
+        return K
+
+    def near_gemv(self, kernel, normal_dot, blocks, near_elem_dsp, elem_nds_cnt, elem_nds_dsp, K_near_cnt,
+                  K_near_dsp, K_near, F, n_near):
+        """U_near of ComputeNearInterac's matrix route (boundary_integral.txx:1324-1340)."""
+        _, K0, K1, _, _ = self.dims(kernel)
+        dof = K1 // 3 if normal_dot else K1
+        be, bt, bc = (_i(a) for a in blocks)
+        arrs = [_i(a) for a in (near_elem_dsp, elem_nds_cnt, elem_nds_dsp, K_near_cnt, K_near_dsp)]
+        U = np.zeros(n_near * dof)
+        rc = self.lib.oracle_near_gemv(kernel, int(normal_dot), be.size, _ip(be), _ip(bt), _ip(bc),
+                                       *[_ip(a) for a in arrs], _p(_c(K_near)), _p(_c(F)), _p(U))
+        if rc != 0:
+            raise RuntimeError("oracle_near_gemv failed rc=%d" % rc)
+        return U
+
+    def near_accumulate(self, U, dof, near_trg_cnt, near_trg_dsp, scatter_index, U_near):
+        """U += per-target sums after ScatterForward (boundary_integral.txx:1368-1379)."""
+        cnt, dsp, sc = _i(near_trg_cnt), _i(near_trg_dsp), _i(scatter_index)
+        out = np.ascontiguousarray(U, dtype=np.float64).copy()
+        rc = self.lib.oracle_near_accumulate(cnt.size, dof, _ip(cnt), _ip(dsp), sc.size, _ip(sc),
+                                             _p(_c(U_near)), _p(out))
+        if rc != 0:
+            raise RuntimeError("oracle_near_accumulate failed rc=%d" % rc)
+        return out
 
     def rsqrt(self, x, mode=EXACT, digits=-1):
         x = _c(x)
@@ -153,7 +234,8 @@ def build_mutant(index, out_dir):
     """Compiles the oracle with ORACLE_MUTANT=index into out_dir; returns the path."""
     out = os.path.join(out_dir, "liboracle_mutant%d.so" % index)
     subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-std=c11",
-                           "-DORACLE_MUTANT=%d" % index, "-o", out, os.path.join(HERE, "direct_sum.c")])
+                           "-DORACLE_MUTANT=%d" % index, "-o", out, os.path.join(HERE, "direct_sum.c"),
+                           os.path.join(HERE, "near_field.c")])
     return out
 
 

@@ -27,6 +27,7 @@ LAPLACE3D_FXU, LAPLACE3D_DXU, STOKES3D_FXU, STOKES3D_DXU = 0, 1, 2, 3
 KERNEL_NAMES = {0: "Laplace3D_FxU", 1: "Laplace3D_DxU", 2: "Stokes3D_FxU", 3: "Stokes3D_DxU"}
 FP64, MIXED, AUTO = 0, 1, 2
 HOST_PTRS, MIXED_NEWTON = 0x1, 0x2
+NEAR_SUBTRACT, NEAR_STORE = 0, 1
 
 STATUS = {0: "SUCCESS", 1: "INVALID_VALUE", 2: "UNKNOWN_KERNEL", 3: "UNSUPPORTED", 4: "NO_DEVICE",
           5: "OUT_OF_MEMORY", 6: "HIP", 7: "QUEUE"}
@@ -40,6 +41,8 @@ SYMBOLS = [
     "sctl_amd_profile_read", "sctl_amd_profile_reset", "sctl_amd_plan_graph_create", "sctl_amd_graph_launch",
     "sctl_amd_graph_interactions", "sctl_amd_graph_destroy", "sctl_amd_gen_normal", "sctl_amd_plan_get_shape",
     "sctl_amd_plan_set_shape", "sctl_amd_tune_db_path", "sctl_amd_tune_eval", "sctl_amd_plan_tune",
+    "sctl_amd_near_blocks", "sctl_amd_plan_near_layout", "sctl_amd_plan_near_matrix", "sctl_amd_plan_near_gemv",
+    "sctl_amd_near_accumulate",
 ]
 
 
@@ -96,6 +99,11 @@ def lib():
     L.sctl_amd_tune_db_path.argtypes = [ctypes.c_char_p]
     L.sctl_amd_tune_eval.argtypes = [i32, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp, ip, ip, _dp]
     L.sctl_amd_plan_tune.argtypes = [_vp, _vp, _vp, _vp, ip, ip, ip, _dp]
+    L.sctl_amd_near_blocks.argtypes = [i64, _i64p, _i64p, i32, _i64p, _i64p, _i64p, _i64p]
+    L.sctl_amd_plan_near_layout.argtypes = [_vp, _i64p, i32, _i64p]
+    L.sctl_amd_plan_near_matrix.argtypes = [_vp, i32, _vp, _vp, _vp, u32, _vp]
+    L.sctl_amd_plan_near_gemv.argtypes = [_vp, _vp, _vp, _vp, u32, _vp]
+    L.sctl_amd_near_accumulate.argtypes = [i64, i32, _vp, _vp, i64, _vp, i64, _vp, _vp, u32, _vp]
     L.sctl_amd_profile_enable.argtypes = [i32]
     L.sctl_amd_profile_read.argtypes = [_i64p, _i64p, _i64p, ctypes.POINTER(ctypes.c_double)]
     _lib = L
@@ -344,6 +352,39 @@ class Plan:
         """Captures nsteps applies with densities regenerated on the device (see Graph)."""
         return Graph(self, nsteps, seed, stream_id, v_src, v_trg)
 
+    # ------------------------------------------------- near-field operator
+    def near_layout(self, offsets=None, trg_normal_dot=False):
+        """Places block b's target-major matrix at offsets[b] of K_near (-1: none;
+        None: packed in block order).  Returns the length K_near must have."""
+        n = ctypes.c_int64()
+        off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.int64)
+        self._near_off = off
+        check(lib().sctl_amd_plan_near_layout(self.handle, None if off is None else off.ctypes.data_as(_i64p),
+                                              int(bool(trg_normal_dot)), ctypes.byref(n)))
+        self.near_len = n.value
+        return n.value
+
+    def near_matrix(self, K, wts=None, n_trg=None, mode=NEAR_SUBTRACT, stream=None):
+        """K -= w * M^T per block (mode NEAR_SUBTRACT) or K = w * M^T (NEAR_STORE).
+        Host K is updated in place; a DeviceArray K asynchronously on `stream`."""
+        flags = 0
+        if not isinstance(K, DeviceArray):
+            assert isinstance(K, np.ndarray) and K.dtype == np.float64 and K.flags.c_contiguous
+            wts, n_trg = _host(wts), _host(n_trg)
+            flags = HOST_PTRS
+        check(lib().sctl_amd_plan_near_matrix(self.handle, mode, _ptr(wts), _ptr(n_trg), _ptr(K), flags, stream))
+        return K
+
+    def near_gemv(self, K, F, U_near, stream=None):
+        """U_near[block rows] = K_block * F[block sources] (overwrites those rows)."""
+        flags = 0
+        if not isinstance(K, DeviceArray):
+            assert isinstance(U_near, np.ndarray) and U_near.dtype == np.float64 and U_near.flags.c_contiguous
+            K, F = _host(K), _host(F)
+            flags = HOST_PTRS
+        check(lib().sctl_amd_plan_near_gemv(self.handle, _ptr(K), _ptr(F), _ptr(U_near), flags, stream))
+        return U_near
+
     def destroy(self):
         if self.handle:
             lib().sctl_amd_plan_destroy(self.handle)
@@ -406,3 +447,126 @@ def gen_normal(seed, stream_id, out, stream=None):
     """Device generator: fills DeviceArray `out` with N(0,1) draws of (seed, stream_id)."""
     check(lib().sctl_amd_gen_normal(seed, stream_id, out.nbytes // 8, out.ptr, stream))
     return out
+
+
+# -------------------------------------------------------------- near field
+def near_blocks(near_elem_cnt, near_elem_dsp, nthreads):
+    """BoundaryIntegralOp's near-block partition (boundary_integral.txx:1009-1032):
+    returns (near_blk_elem, near_blk_t0, near_blk_cnt)."""
+    cnt = np.ascontiguousarray(near_elem_cnt, dtype=np.int64)
+    dsp = np.ascontiguousarray(near_elem_dsp, dtype=np.int64)
+    n = ctypes.c_int64()
+    P = lambda x: x.ctypes.data_as(_i64p)
+    check(lib().sctl_amd_near_blocks(cnt.size, P(cnt), P(dsp), int(nthreads), ctypes.byref(n), None, None, None))
+    e, t0, c = (np.zeros(n.value, dtype=np.int64) for _ in range(3))
+    check(lib().sctl_amd_near_blocks(cnt.size, P(cnt), P(dsp), int(nthreads), ctypes.byref(n), P(e), P(t0), P(c)))
+    return e, t0, c
+
+
+def near_accumulate(U, dof, near_trg_cnt, near_trg_dsp, scatter_index, U_near, n_near=None, stream=None):
+    """U[i] += sum of U_near over target i's near elements after ScatterForward
+    (boundary_integral.txx:1368-1379).  Host arrays: U is updated in place and
+    returned.  DeviceArrays: asynchronous on `stream` (give n_near)."""
+    if isinstance(U, DeviceArray):
+        ntrg = U.shape[0] // dof
+        n_sc = scatter_index.shape[0]
+        n_near = U_near.shape[0] // dof if n_near is None else n_near
+        check(lib().sctl_amd_near_accumulate(ntrg, dof, near_trg_cnt.ptr, near_trg_dsp.ptr, n_sc, scatter_index.ptr,
+                                             n_near, U_near.ptr, U.ptr, 0, stream))
+        return U
+    assert isinstance(U, np.ndarray) and U.dtype == np.float64 and U.flags.c_contiguous
+    cnt = np.ascontiguousarray(near_trg_cnt, dtype=np.int64)
+    dsp = np.ascontiguousarray(near_trg_dsp, dtype=np.int64)
+    sc = np.ascontiguousarray(scatter_index, dtype=np.int64)
+    un = _host(U_near)
+    check(lib().sctl_amd_near_accumulate(U.size // dof, dof, _ptr(cnt), _ptr(dsp), sc.size, _ptr(sc), un.size // dof,
+                                         _ptr(un), _ptr(U), HOST_PTRS, stream))
+    return U
+
+
+def near_offsets(K0, dof, blocks, near_elem_cnt, elem_nds_cnt, matrix_free=None):
+    """Offsets of each block's rows in the reference's K_near and its length.
+
+    K_near_cnt[e] = elem_nds_cnt[e]*near_elem_cnt[e] (0 for a matrix-free
+    element), K_near_dsp its exclusive scan (boundary_integral.txx:1039-1053);
+    block (e, t0, cnt) starts at K_near_dsp[e]*KDIM0*KDIM1_ + t0*KDIM1_*src_dof
+    (:1187-1189), -1 for a matrix-free element.  dof is KDIM1_."""
+    be, bt, _ = (np.asarray(a, dtype=np.int64) for a in blocks)
+    cnt = np.asarray(near_elem_cnt, dtype=np.int64)
+    nds = np.asarray(elem_nds_cnt, dtype=np.int64)
+    mf = np.zeros(cnt.size, dtype=bool) if matrix_free is None else np.asarray(matrix_free, dtype=bool)
+    if cnt.size == 0:
+        return np.zeros(be.size, dtype=np.int64), 0
+    k_cnt = np.where(mf, 0, nds * cnt)
+    k_dsp = np.concatenate(([0], np.cumsum(k_cnt)[:-1])).astype(np.int64)
+    off = k_dsp[be] * K0 * dof + bt * dof * (nds[be] * K0)
+    off = np.where(mf[be], -1, off).astype(np.int64)
+    return off, int((k_dsp[-1] + k_cnt[-1]) * K0 * dof)
+
+
+class NearField:
+    """The device near-field operator of one BoundaryIntegralOp, built from the
+    reference's own arrays (boundary_integral.txx):
+
+      near_elem_cnt/dsp     near targets of each element (Xtrg_near rows)
+      elem_nds_cnt/dsp_far  each element's far nodes (X_far rows); with the
+                            identity FarFieldDensityOperatorTranspose they are
+                            the element's nodes, so F is indexed the same way
+      matrix_free           per element: no K_near matrix (MatrixFree())
+      trg_normal_dot        contract the output with the target normal
+
+    Methods:
+      k_near_subtract(K_near)           KNearSubtract (:1165-1240)
+      near_interac(U, F, K_near, ...)   ComputeNearInterac's matrix route
+                                        (:1311-1381): GEMV, ScatterForward,
+                                        per-target sum
+    K_near uses the reference's layout (K_near_dsp[e]*KDIM0*KDIM1_ +
+    t0*KDIM1_*src_dof per block, :1187-1189)."""
+
+    def __init__(self, kernel, near_elem_cnt, near_elem_dsp, elem_nds_cnt_far, elem_nds_dsp_far, Xtrg_near,
+                 X_far, Xn_far=None, wts_far=None, Xn_trg_near=None, matrix_free=None, trg_normal_dot=False,
+                 nthreads=None, blocks=None):
+        _, K0, K1, ND, _, _, _ = kernel_info(kernel)
+        self.kernel, self.K0, self.K1 = kernel, K0, K1
+        self.dof = K1 // 3 if trg_normal_dot else K1
+        cnt = np.ascontiguousarray(near_elem_cnt, dtype=np.int64)
+        dsp = np.ascontiguousarray(near_elem_dsp, dtype=np.int64)
+        ncf = np.ascontiguousarray(elem_nds_cnt_far, dtype=np.int64)
+        ndf = np.ascontiguousarray(elem_nds_dsp_far, dtype=np.int64)
+        nelem = cnt.size
+        mf = np.zeros(nelem, dtype=bool) if matrix_free is None else np.asarray(matrix_free, dtype=bool)
+        if blocks is None:
+            blocks = near_blocks(cnt, dsp, nthreads or os.cpu_count() or 1)
+        be, bt, bc = (np.ascontiguousarray(a, dtype=np.int64) for a in blocks)
+        self.blocks = (be, bt, bc)
+        off, self.k_len = near_offsets(K0, self.dof, (be, bt, bc), cnt, ncf, mf)
+        nt_total = int(dsp[-1] + cnt[-1]) if nelem else 0
+        ns_total = int(np.max(ndf + ncf)) if nelem else 0
+        self.nt_total, self.ns_total = nt_total, ns_total
+        self.plan = Plan(kernel, dsp[be] + bt, bc, ndf[be], ncf[be], nt_total, ns_total)
+        self.plan.set_geometry(Xtrg_near, X_far, Xn_far if ND else None)
+        need = self.plan.near_layout(off, trg_normal_dot)
+        assert need <= self.k_len
+        self.wts_far = None if wts_far is None else _host(wts_far)
+        self.Xn_trg_near = None if Xn_trg_near is None else _host(Xn_trg_near)
+
+    def k_near_subtract(self, K_near, stream=None):
+        """K_near -= direct part of every block (in place; numpy or DeviceArray)."""
+        wts, ntr = self.wts_far, self.Xn_trg_near
+        if isinstance(K_near, DeviceArray):
+            wts = None if wts is None else DeviceArray.from_host(wts)
+            ntr = None if ntr is None else DeviceArray.from_host(ntr)
+            self._keep = (wts, ntr)
+        return self.plan.near_matrix(K_near, wts, ntr, NEAR_SUBTRACT, stream)
+
+    def near_interac(self, U, F, K_near, near_trg_cnt, near_trg_dsp, scatter_index):
+        """U += near interactions (host arrays): the GEMV route of
+        ComputeNearInterac followed by ScatterForward and the per-target sum.
+        U is resized and zeroed if its length is not Ntrg*KDIM1_ (:1316-1319)."""
+        ntrg = np.asarray(near_trg_cnt).size
+        if U is None or U.size != ntrg * self.dof:
+            U = np.zeros(ntrg * self.dof)
+        U_near = np.zeros(self.nt_total * self.dof)
+        self.plan.near_gemv(K_near, F, U_near)
+        return near_accumulate(U, self.dof, near_trg_cnt, near_trg_dsp, scatter_index, U_near)
+

@@ -36,6 +36,7 @@
 
 #include "internal.hpp"
 #include "kernels.hpp"
+#include "plan.hpp"
 
 namespace sctl_amd {
 
@@ -43,13 +44,6 @@ namespace {
 
 constexpr int BWG = 256;  // 4 waves per workgroup, each wave independent
 constexpr double PADD = 1e100;
-
-struct Item {
-  int32_t block;  // block index
-  int32_t t_off;  // first target of the chunk, relative to the block
-  int32_t t_cnt;  // targets in the chunk (<= 64*TPL)
-  int32_t lg;     // log2 of the number of lane groups
-};
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -369,45 +363,10 @@ double fold(int ker, int newton) {
   }
 }
 
-template <class T> int dev_upload(T** dst, const std::vector<T>& v) {
-  *dst = nullptr;
-  if (v.empty()) return SCTL_AMD_SUCCESS;
-  SCTL_HIP_TRY(hipMalloc(dst, sizeof(T) * v.size()));
-  SCTL_HIP_TRY(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
-  return SCTL_AMD_SUCCESS;
-}
-
 }  // namespace
 }  // namespace sctl_amd
 
 using namespace sctl_amd;
-
-struct sctl_amd_plan_s {
-  int kernel = 0, precision = SCTL_AMD_FP64, newton = 2, K0 = 1, K1 = 1, ND = 0;
-  int device = 0;
-  int64_t nblocks = 0, nt_total = 0, ns_total = 0, interactions = 0, nonempty = 0;
-  std::vector<int64_t> trg_cnt, src_cnt;  // host copies (items are rebuilt per targets-per-lane)
-  int nitems = 0;
-  PlanShape shape;      // resolved knobs
-  int grid = 0;         // persistent grid (workgroups)
-  size_t dyn_lds = 0;   // dynamic LDS that enforces shape.occ
-  bool has_geometry = false;
-  Item* d_items = nullptr;
-  int64_t *d_trg_dsp = nullptr, *d_src_dsp = nullptr;
-  int32_t* d_src_cnt = nullptr;
-  int* d_counter = nullptr;
-  double *d_xt = nullptr, *d_xs = nullptr, *d_xn = nullptr;
-
-  ~sctl_amd_plan_s() {
-    int cur = -1;
-    (void)hipGetDevice(&cur);
-    if (cur != device) (void)hipSetDevice(device);
-    void* ptrs[] = {d_items, d_trg_dsp, d_src_dsp, d_src_cnt, d_counter, d_xt, d_xs, d_xn};
-    for (void* p : ptrs)
-      if (p) (void)hipFree(p);
-    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
-  }
-};
 
 namespace sctl_amd {
 
@@ -553,19 +512,25 @@ int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int dig
   p->nblocks = nblocks;
   p->nt_total = nt_total;
   p->ns_total = ns_total;
+  p->trg_dsp.assign(trg_dsp, trg_dsp + nblocks);
   p->trg_cnt.assign(trg_cnt, trg_cnt + nblocks);
+  p->src_dsp.assign(src_dsp, src_dsp + nblocks);
   p->src_cnt.assign(src_cnt, src_cnt + nblocks);
   for (int64_t b = 0; b < nblocks; b++) {
     p->interactions += trg_cnt[b] * src_cnt[b];
     p->nonempty += (trg_cnt[b] && src_cnt[b]) ? 1 : 0;
   }
   std::vector<int64_t> v_td(trg_dsp, trg_dsp + nblocks), v_sd(src_dsp, src_dsp + nblocks);
-  std::vector<int32_t> v_sc(nblocks);
-  for (int64_t b = 0; b < nblocks; b++) v_sc[b] = (int32_t)src_cnt[b];
+  std::vector<int32_t> v_sc(nblocks), v_tc(nblocks);
+  for (int64_t b = 0; b < nblocks; b++) {
+    v_sc[b] = (int32_t)src_cnt[b];
+    v_tc[b] = (int32_t)trg_cnt[b];
+  }
   int rc = SCTL_AMD_SUCCESS;
   if (!rc) rc = dev_upload(&p->d_trg_dsp, v_td);
   if (!rc) rc = dev_upload(&p->d_src_dsp, v_sd);
   if (!rc) rc = dev_upload(&p->d_src_cnt, v_sc);
+  if (!rc) rc = dev_upload(&p->d_trg_cnt, v_tc);
   if (!rc) {
     hipError_t e = hipMalloc(&p->d_counter, sizeof(int));
     if (e != hipSuccess) {

@@ -1,0 +1,458 @@
+// Near-field operators of BoundaryIntegralOp on the device.
+//
+// BoundaryIntegralOp keeps, per element, a target-major matrix K_near of
+// near-interaction corrections (boundary_integral.txx:1034-1058):
+// K_near_e has near_elem_cnt[e]*KDIM1_ rows and elem_nds_cnt[e]*KDIM0
+// columns, KDIM1_ = KDIM1 (or KDIM1/3 when the output is contracted with the
+// target normal).  Three steps of the reference run here:
+//
+//  * near-block partition (:1009-1032)            sctl_amd_near_blocks (host)
+//  * KNearSubtract (:1165-1240): per block
+//      K_near_[t*KDIM1_+k1][s*KDIM0+k0] -= w_s * M[s*KDIM0+k0][t*KDIM1+k1]
+//    with M the scaled KernelMatrix of the block (optionally contracted with
+//    the target normal, :1193-1209)              k_near_matrix
+//  * ComputeNearInterac, matrix route (:1324-1340): per block
+//      U_near[block rows] = K_near_ * F[element's nodes]      k_near_gemv
+//    then ScatterForward and the per-target sum (:1368-1379,
+//    comm.txx:1497-1508)                                      k_near_accumulate
+//
+// The blocks are the plan's blocks (target range = the block's near targets,
+// source range = the element's far nodes, which are the element's nodes when
+// FarFieldDensityOperatorTranspose is the identity, boundary_integral.txx:531).
+//
+// All three kernels are bound by HBM traffic (8 bytes of K_near per matrix
+// entry, read and written by the subtract, read by the GEMV); the matrix
+// entries are FP64 whatever the plan's precision.  Work is cut into wave
+// tiles listed once per layout (tile -> block, no search at run time):
+//   near matrix  16 targets x 64 sources: one source per lane, the target
+//                loop wave-uniform, so every row segment a wave writes is
+//                64*KDIM0 consecutive doubles;
+//   GEMV         16 rows: 16 lanes per row, 4 rows at a time, the row's
+//                reduction by cross-lane shuffles inside the 16-lane group.
+// Nothing here allocates or issues a memset with device pointers, so the
+// calls can be captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace sctl_amd {
+
+namespace {
+
+constexpr int NWG = 256;        // 4 independent waves per workgroup
+constexpr int MAT_RT = 16;      // targets per near-matrix wave tile
+constexpr int GEMV_ROWS = 16;   // rows per GEMV wave tile
+constexpr int GEMV_LPR = 16;    // lanes per row in the GEMV
+constexpr int NEAR_NEWTON = 2;  // KernelMatrix at digits -1 (boundary_integral.txx:1197,1211)
+
+// KNearSubtract / store of one layout's blocks.  NDOT: contract the KDIM1
+// outputs with the target normal (KDIM1 == 3 -> one row per target).
+template <int KER, bool NDOT, bool SUB>
+__global__ void __launch_bounds__(NWG)
+k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ tile_dsp, int64_t ntiles,
+              const int64_t* __restrict__ trg_dsp, const int32_t* __restrict__ trg_cnt,
+              const int64_t* __restrict__ src_dsp, const int32_t* __restrict__ src_cnt,
+              const int64_t* __restrict__ mat_off, const double* __restrict__ xt, const double* __restrict__ n_trg,
+              const double* __restrict__ xs, const double* __restrict__ xn, const double* __restrict__ wts,
+              double* __restrict__ K, double scale) {
+  using Kt = Ker<KER>;
+  constexpr int K0 = Kt::K0, K1 = Kt::K1, ND = Kt::ND;
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (NWG / 64);
+  for (int64_t tile = (int64_t)blockIdx.x * (NWG / 64) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
+    const int b = __builtin_amdgcn_readfirstlane(tile_blk[tile]);
+    const int64_t local = tile - tile_dsp[b];
+    const int sc = src_cnt[b], tc = trg_cnt[b];
+    const int ts = (sc + 63) >> 6;
+    const int ti = (int)(local / ts);
+    const int si = (int)(local - (int64_t)ti * ts);
+    const int s = si * 64 + lane;
+    const bool active = s < sc;
+    const int64_t gs = src_dsp[b] + (active ? s : 0);
+    const double sx = xs[gs * 3 + 0], sy = xs[gs * 3 + 1], sz = xs[gs * 3 + 2];
+    double n[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < ND; k++) n[k] = xn[gs * 3 + k];
+    const double w = wts ? wts[gs] : 1.0;
+    const int64_t ld = (int64_t)sc * K0;
+    double* Kb = K + mat_off[b] + (int64_t)s * K0;
+    const int64_t tg0 = trg_dsp[b];
+    const int t_end = min(ti * MAT_RT + MAT_RT, tc);
+    for (int t = ti * MAT_RT; t < t_end; t++) {
+      const double* x = xt + (tg0 + t) * 3;
+      double u[3][3];
+      kmatrix<KER, NEAR_NEWTON>(u, x[0] - sx, x[1] - sy, x[2] - sz, n);
+      if (!active) continue;
+      if constexpr (NDOT) {
+        const double* nt = n_trg + (tg0 + t) * 3;
+        double* row = Kb + (int64_t)t * ld;
+#pragma unroll
+        for (int k0 = 0; k0 < K0; k0++) {
+          double v = 0.0;
+#pragma unroll
+          for (int l = 0; l < 3; l++) v += (u[k0][l] * scale) * w * nt[l];
+          if constexpr (SUB) row[k0] -= v;
+          else row[k0] = v;
+        }
+      } else {
+#pragma unroll
+        for (int k1 = 0; k1 < K1; k1++) {
+          double* row = Kb + (int64_t)(t * K1 + k1) * ld;
+#pragma unroll
+          for (int k0 = 0; k0 < K0; k0++) {
+            const double v = (u[k0][k1] * scale) * w;
+            if constexpr (SUB) row[k0] -= v;
+            else row[k0] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// U_near[block rows] = K_near_ * F_element.  Blocks without a matrix
+// (offset < 0) or without sources get zero rows, as the reference's GEMV loop
+// writes them (boundary_integral.txx:1329-1332).
+__global__ void __launch_bounds__(NWG)
+k_near_gemv(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ tile_dsp, int64_t ntiles,
+            const int64_t* __restrict__ trg_dsp, const int32_t* __restrict__ trg_cnt,
+            const int64_t* __restrict__ src_dsp, const int32_t* __restrict__ src_cnt,
+            const int64_t* __restrict__ mat_off, int K0, int dof, const double* __restrict__ K,
+            const double* __restrict__ F, double* __restrict__ U) {
+  const int lane = threadIdx.x & 63;
+  const int grp = lane / GEMV_LPR, li = lane % GEMV_LPR;
+  const int64_t nwaves = (int64_t)gridDim.x * (NWG / 64);
+  for (int64_t tile = (int64_t)blockIdx.x * (NWG / 64) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
+    const int b = __builtin_amdgcn_readfirstlane(tile_blk[tile]);
+    const int64_t local = tile - tile_dsp[b];
+    const int64_t rows = (int64_t)trg_cnt[b] * dof;
+    const int64_t ld = (int64_t)src_cnt[b] * K0;
+    const int64_t off = mat_off[b];
+    const double* Fb = F + src_dsp[b] * K0;
+    double* Ub = U + trg_dsp[b] * dof;
+#pragma unroll
+    for (int rr = 0; rr < GEMV_ROWS; rr += 64 / GEMV_LPR) {
+      const int64_t row = local * GEMV_ROWS + rr + grp;
+      double acc = 0.0;
+      if (row < rows && off >= 0) {
+        const double* Kr = K + off + row * ld;
+#pragma unroll 4
+        for (int64_t j = li; j < ld; j += GEMV_LPR) acc = fmad(Kr[j], Fb[j], acc);
+      }
+#pragma unroll
+      for (int m = GEMV_LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, 64);
+      if (li == 0 && row < rows) Ub[row] = acc;
+    }
+  }
+}
+
+// ScatterForward + per-target accumulation: for every target i and every
+// component k, U[i*DOF+k] += U_near[scatter[dsp_i + j]*DOF + k] for j in
+// order, the reference's order of additions.  Indices outside the arrays are
+// skipped (the host checks them when it has them).
+template <int DOF>
+__global__ void __launch_bounds__(NWG)
+k_near_accumulate(int64_t ntrg, const int64_t* __restrict__ cnt, const int64_t* __restrict__ dsp, int64_t n_scatter,
+                  const int64_t* __restrict__ scatter, int64_t n_near, const double* __restrict__ U_near,
+                  double* __restrict__ U) {
+  const int64_t i = (int64_t)blockIdx.x * NWG + threadIdx.x;
+  if (i >= ntrg) return;
+  const int64_t c = cnt[i], d = dsp[i];
+  double acc[DOF];
+#pragma unroll
+  for (int k = 0; k < DOF; k++) acc[k] = U[i * DOF + k];
+  for (int64_t j = 0; j < c; j++) {
+    const int64_t q = d + j;
+    if (q < 0 || q >= n_scatter) continue;
+    const int64_t src = scatter[q];
+    if (src < 0 || src >= n_near) continue;
+#pragma unroll
+    for (int k = 0; k < DOF; k++) acc[k] += U_near[src * DOF + k];
+  }
+#pragma unroll
+  for (int k = 0; k < DOF; k++) U[i * DOF + k] = acc[k];
+}
+
+using NearMatFn = void (*)(const int32_t*, const int64_t*, int64_t, const int64_t*, const int32_t*, const int64_t*,
+                           const int32_t*, const int64_t*, const double*, const double*, const double*,
+                           const double*, const double*, double*, double);
+
+template <int KER, bool NDOT> NearMatFn near_mat_fn(bool sub) {
+  return sub ? k_near_matrix<KER, NDOT, true> : k_near_matrix<KER, NDOT, false>;
+}
+NearMatFn near_mat_fn(int ker, bool ndot, bool sub) {
+  switch (ker) {
+    case LAP_FXU: return near_mat_fn<LAP_FXU, false>(sub);
+    case LAP_DXU: return near_mat_fn<LAP_DXU, false>(sub);
+    case STK_FXU: return ndot ? near_mat_fn<STK_FXU, true>(sub) : near_mat_fn<STK_FXU, false>(sub);
+    default: return ndot ? near_mat_fn<STK_DXU, true>(sub) : near_mat_fn<STK_DXU, false>(sub);
+  }
+}
+
+// Flattens per-block tile counts into a device tile list.
+int build_tiles(const std::vector<int64_t>& per_block, TileList* out) {
+  std::vector<int64_t> dsp(per_block.size());
+  int64_t total = 0;
+  for (size_t b = 0; b < per_block.size(); b++) {
+    dsp[b] = total;
+    total += per_block[b];
+  }
+  SCTL_CHECK_ARG(total < (int64_t)1 << 34, "near-field layout has too many tiles");
+  std::vector<int32_t> blk((size_t)total);
+  for (size_t b = 0; b < per_block.size(); b++)
+    for (int64_t i = 0; i < per_block[b]; i++) blk[(size_t)(dsp[b] + i)] = (int32_t)b;
+  out->ntiles = total;
+  SCTL_TRY(dev_upload(&out->d_tile_blk, blk));
+  SCTL_TRY(dev_upload(&out->d_tile_dsp, dsp));
+  return SCTL_AMD_SUCCESS;
+}
+
+int tile_grid(int64_t ntiles) {
+  DeviceInfo dev;
+  if (device_info(&dev) != SCTL_AMD_SUCCESS) return 1;
+  const int64_t want = (ntiles + NWG / 64 - 1) / (NWG / 64);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)dev.num_cus * 64));
+}
+
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+int near_matrix_launch(sctl_amd_plan_s* p, bool sub, const double* wts, const double* n_trg, double* K,
+                       hipStream_t s) {
+  if (p->mat_tiles.ntiles == 0) return SCTL_AMD_SUCCESS;
+  if (!capturing(s)) {
+    int64_t inter = 0;
+    for (int64_t b = 0; b < p->nblocks; b++)
+      if (p->near_off[b] >= 0) inter += p->trg_cnt[b] * p->src_cnt[b];
+    profile_count(p->kernel, inter);
+  }
+  ProfileScope prof(s);
+  hipLaunchKernelGGL(near_mat_fn(p->kernel, p->near_ndot, sub), dim3(tile_grid(p->mat_tiles.ntiles)), dim3(NWG), 0,
+                     s, p->mat_tiles.d_tile_blk, p->mat_tiles.d_tile_dsp, p->mat_tiles.ntiles, p->d_trg_dsp,
+                     p->d_trg_cnt, p->d_src_dsp, p->d_src_cnt, p->d_near_off, p->d_xt, n_trg, p->d_xs, p->d_xn, wts,
+                     K, kernel_scale(p->kernel));
+  SCTL_HIP_TRY(hipGetLastError());
+  return SCTL_AMD_SUCCESS;
+}
+
+int near_gemv_launch(sctl_amd_plan_s* p, const double* K, const double* F, double* U, hipStream_t s) {
+  if (p->gemv_tiles.ntiles == 0) return SCTL_AMD_SUCCESS;
+  ProfileScope prof(s);
+  hipLaunchKernelGGL(k_near_gemv, dim3(tile_grid(p->gemv_tiles.ntiles)), dim3(NWG), 0, s, p->gemv_tiles.d_tile_blk,
+                     p->gemv_tiles.d_tile_dsp, p->gemv_tiles.ntiles, p->d_trg_dsp, p->d_trg_cnt, p->d_src_dsp,
+                     p->d_src_cnt, p->d_near_off, p->K0, p->near_dof, K, F, U);
+  SCTL_HIP_TRY(hipGetLastError());
+  return SCTL_AMD_SUCCESS;
+}
+
+// Host staging of the HOST_PTRS mode: copies in, runs, copies `out` back.
+struct Staged {
+  Scratch buf;
+  const void* host = nullptr;
+  int64_t bytes = 0;
+  int in(const void* h, int64_t b, hipStream_t s) {
+    host = h;
+    bytes = b;
+    if (!h || b <= 0) return SCTL_AMD_SUCCESS;
+    SCTL_TRY(buf.alloc((size_t)b, s));
+    SCTL_HIP_TRY(hipMemcpyAsync(buf.as<void>(), h, (size_t)b, hipMemcpyHostToDevice, s));
+    return SCTL_AMD_SUCCESS;
+  }
+  int out(void* h, hipStream_t s) {
+    if (!h || bytes <= 0) return SCTL_AMD_SUCCESS;
+    SCTL_HIP_TRY(hipMemcpyAsync(h, buf.as<void>(), (size_t)bytes, hipMemcpyDeviceToHost, s));
+    return SCTL_AMD_SUCCESS;
+  }
+  template <class T> T* dev() const { return host && bytes > 0 ? buf.as<T>() : nullptr; }
+};
+
+}  // namespace
+}  // namespace sctl_amd
+
+using namespace sctl_amd;
+
+extern "C" {
+
+int sctl_amd_near_blocks(int64_t nelem, const int64_t* near_elem_cnt, const int64_t* near_elem_dsp, int nthreads,
+                         int64_t* nblk, int64_t* blk_elem, int64_t* blk_t0, int64_t* blk_cnt) {
+  SCTL_CHECK_ARG(nblk, "nblk is NULL");
+  SCTL_CHECK_ARG(nelem >= 0, "negative element count");
+  SCTL_CHECK_ARG(nelem == 0 || (near_elem_cnt && near_elem_dsp), "NULL near-list array");
+  SCTL_CHECK_ARG(nthreads >= 1, "nthreads must be >= 1");
+  for (int64_t e = 0; e < nelem; e++)
+    SCTL_CHECK_ARG(near_elem_cnt[e] >= 0, "negative near-target count (element " + std::to_string(e) + ")");
+  const int64_t n_near = nelem ? near_elem_dsp[nelem - 1] + near_elem_cnt[nelem - 1] : 0;
+  const int64_t grain = std::max<int64_t>(1, n_near / (4 * (int64_t)nthreads));
+  int64_t count = 0;
+  for (int64_t e = 0; e < nelem; e++) count += (near_elem_cnt[e] + grain - 1) / grain;
+  const bool fill = blk_elem || blk_t0 || blk_cnt;
+  SCTL_CHECK_ARG(!fill || (blk_elem && blk_t0 && blk_cnt), "give all three block arrays or none");
+  if (fill) {
+    int64_t b = 0;
+    for (int64_t e = 0; e < nelem; e++) {
+      for (int64_t t0 = 0; t0 < near_elem_cnt[e]; t0 += grain, b++) {
+        blk_elem[b] = e;
+        blk_t0[b] = t0;
+        blk_cnt[b] = std::min(grain, near_elem_cnt[e] - t0);
+      }
+    }
+  }
+  *nblk = count;
+  return SCTL_AMD_SUCCESS;
+}
+
+int sctl_amd_plan_near_layout(sctl_amd_plan p, const int64_t* blk_off, int trg_normal_dot, int64_t* k_len) {
+  SCTL_CHECK_ARG(p, "plan is NULL");
+  SCTL_CHECK_ARG(!trg_normal_dot || p->K1 % 3 == 0,
+                 "the target-normal contraction needs a kernel with 3 outputs per target (Stokes)");
+  const int dof = trg_normal_dot ? p->K1 / 3 : p->K1;
+  const int64_t nb = p->nblocks;
+  std::vector<int64_t> off(nb), size(nb);
+  int64_t len = 0;
+  for (int64_t b = 0; b < nb; b++) {
+    size[b] = p->trg_cnt[b] * dof * p->src_cnt[b] * p->K0;
+    if (blk_off) {
+      off[b] = blk_off[b] < 0 ? -1 : blk_off[b];
+      if (off[b] >= 0) {
+        SCTL_CHECK_ARG(off[b] <= ((int64_t)1 << 62) - size[b], "block matrix offset too large");
+        len = std::max(len, off[b] + size[b]);
+      }
+    } else {
+      off[b] = len;
+      len += size[b];
+    }
+  }
+  if (blk_off) {  // block matrices must not overlap (their writes would race)
+    std::vector<int64_t> order;
+    for (int64_t b = 0; b < nb; b++)
+      if (off[b] >= 0 && size[b] > 0) order.push_back(b);
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return off[a] < off[b]; });
+    for (size_t i = 1; i < order.size(); i++)
+      SCTL_CHECK_ARG(off[order[i - 1]] + size[order[i - 1]] <= off[order[i]],
+                     "block matrices overlap (blocks " + std::to_string(order[i - 1]) + " and " +
+                         std::to_string(order[i]) + ")");
+  }
+  std::vector<int64_t> mat(nb), gemv(nb);
+  for (int64_t b = 0; b < nb; b++) {
+    const int64_t tc = p->trg_cnt[b], sc = p->src_cnt[b];
+    mat[b] = (off[b] >= 0 && tc > 0 && sc > 0) ? ((tc + MAT_RT - 1) / MAT_RT) * ((sc + 63) / 64) : 0;
+    gemv[b] = tc > 0 ? (tc * dof + GEMV_ROWS - 1) / GEMV_ROWS : 0;
+  }
+  p->free_near();
+  int rc = dev_upload(&p->d_near_off, off);
+  if (!rc) rc = build_tiles(mat, &p->mat_tiles);
+  if (!rc) rc = build_tiles(gemv, &p->gemv_tiles);
+  if (rc) {
+    p->free_near();
+    return rc;
+  }
+  p->near_off = off;
+  p->near_dof = dof;
+  p->near_ndot = trg_normal_dot != 0;
+  p->near_len = len;
+  p->has_near = true;
+  if (k_len) *k_len = len;
+  return SCTL_AMD_SUCCESS;
+}
+
+int sctl_amd_plan_near_matrix(sctl_amd_plan p, int mode, const double* wts, const double* n_trg, double* K,
+                              uint32_t flags, void* stream) {
+  SCTL_CHECK_ARG(p, "plan is NULL");
+  SCTL_CHECK_ARG(p->has_geometry, "plan has no geometry (call sctl_amd_plan_set_geometry first)");
+  SCTL_CHECK_ARG(p->has_near, "plan has no near-field layout (call sctl_amd_plan_near_layout first)");
+  SCTL_CHECK_ARG(mode == SCTL_AMD_NEAR_SUBTRACT || mode == SCTL_AMD_NEAR_STORE, "unknown near-matrix mode");
+  SCTL_CHECK_ARG((flags & ~SCTL_AMD_HOST_PTRS) == 0, "unknown flag bits");
+  SCTL_CHECK_ARG(p->near_len == 0 || K, "K is NULL");
+  SCTL_CHECK_ARG(!p->near_ndot || p->nt_total == 0 || n_trg, "n_trg is NULL for a target-normal layout");
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread;
+  const bool sub = mode == SCTL_AMD_NEAR_SUBTRACT;
+  const double* nt = p->near_ndot ? n_trg : nullptr;
+  if (!(flags & SCTL_AMD_HOST_PTRS)) return near_matrix_launch(p, sub, wts, nt, K, s);
+  Staged sw, sn, sk;
+  SCTL_TRY(sw.in(wts, (int64_t)sizeof(double) * p->ns_total, s));
+  SCTL_TRY(sn.in(nt, (int64_t)sizeof(double) * p->nt_total * 3, s));
+  // K is copied in for both modes: the store mode leaves the entries outside
+  // the block matrices as the caller had them.
+  SCTL_TRY(sk.in(K, (int64_t)sizeof(double) * p->near_len, s));
+  SCTL_TRY(near_matrix_launch(p, sub, sw.dev<double>(), sn.dev<double>(), sk.dev<double>(), s));
+  SCTL_TRY(sk.out(K, s));
+  SCTL_HIP_TRY(hipStreamSynchronize(s));
+  return SCTL_AMD_SUCCESS;
+}
+
+int sctl_amd_plan_near_gemv(sctl_amd_plan p, const double* K, const double* F, double* U_near, uint32_t flags,
+                            void* stream) {
+  SCTL_CHECK_ARG(p, "plan is NULL");
+  SCTL_CHECK_ARG(p->has_near, "plan has no near-field layout (call sctl_amd_plan_near_layout first)");
+  SCTL_CHECK_ARG((flags & ~SCTL_AMD_HOST_PTRS) == 0, "unknown flag bits");
+  SCTL_CHECK_ARG(p->near_len == 0 || K, "K is NULL");
+  SCTL_CHECK_ARG(p->ns_total == 0 || F, "F is NULL");
+  SCTL_CHECK_ARG(p->nt_total == 0 || U_near, "U_near is NULL");
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread;
+  if (!(flags & SCTL_AMD_HOST_PTRS)) return near_gemv_launch(p, K, F, U_near, s);
+  Staged sk, sf, su;
+  SCTL_TRY(sk.in(K, (int64_t)sizeof(double) * p->near_len, s));
+  SCTL_TRY(sf.in(F, (int64_t)sizeof(double) * p->ns_total * p->K0, s));
+  SCTL_TRY(su.in(U_near, (int64_t)sizeof(double) * p->nt_total * p->near_dof, s));
+  SCTL_TRY(near_gemv_launch(p, sk.dev<double>(), sf.dev<double>(), su.dev<double>(), s));
+  SCTL_TRY(su.out(U_near, s));
+  SCTL_HIP_TRY(hipStreamSynchronize(s));
+  return SCTL_AMD_SUCCESS;
+}
+
+int sctl_amd_near_accumulate(int64_t ntrg, int dof, const int64_t* near_trg_cnt, const int64_t* near_trg_dsp,
+                             int64_t n_scatter, const int64_t* scatter_index, int64_t n_near, const double* U_near,
+                             double* U, uint32_t flags, void* stream) {
+  SCTL_CHECK_ARG(dof == 1 || dof == 3, "dof must be 1 or 3");
+  SCTL_CHECK_ARG(ntrg >= 0 && n_scatter >= 0 && n_near >= 0, "negative size");
+  SCTL_CHECK_ARG(ntrg == 0 || (near_trg_cnt && near_trg_dsp && U), "NULL target array");
+  SCTL_CHECK_ARG(n_scatter == 0 || scatter_index, "scatter_index is NULL");
+  SCTL_CHECK_ARG(n_near == 0 || U_near, "U_near is NULL");
+  SCTL_CHECK_ARG((flags & ~SCTL_AMD_HOST_PTRS) == 0, "unknown flag bits");
+  if (ntrg == 0) return SCTL_AMD_SUCCESS;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread;
+  const int64_t grid = (ntrg + NWG - 1) / NWG;
+  SCTL_CHECK_ARG(grid <= 0x7fffffffLL, "too many targets for one launch");
+  auto launch = [&](const int64_t* cnt, const int64_t* dsp, const int64_t* sc, const double* un, double* u) -> int {
+    ProfileScope prof(s);
+    if (dof == 1)
+      hipLaunchKernelGGL(k_near_accumulate<1>, dim3((unsigned)grid), dim3(NWG), 0, s, ntrg, cnt, dsp, n_scatter, sc,
+                         n_near, un, u);
+    else
+      hipLaunchKernelGGL(k_near_accumulate<3>, dim3((unsigned)grid), dim3(NWG), 0, s, ntrg, cnt, dsp, n_scatter, sc,
+                         n_near, un, u);
+    SCTL_HIP_TRY(hipGetLastError());
+    return SCTL_AMD_SUCCESS;
+  };
+  if (!(flags & SCTL_AMD_HOST_PTRS)) return launch(near_trg_cnt, near_trg_dsp, scatter_index, U_near, U);
+  for (int64_t i = 0; i < ntrg; i++) {
+    SCTL_CHECK_ARG(near_trg_cnt[i] >= 0, "negative near count (target " + std::to_string(i) + ")");
+    if (near_trg_cnt[i] == 0) continue;
+    SCTL_CHECK_ARG(near_trg_dsp[i] >= 0 && near_trg_dsp[i] + near_trg_cnt[i] <= n_scatter,
+                   "near range outside scatter_index (target " + std::to_string(i) + ")");
+  }
+  for (int64_t q = 0; q < n_scatter; q++)
+    SCTL_CHECK_ARG(scatter_index[q] >= 0 && scatter_index[q] < n_near,
+                   "scatter_index out of range (entry " + std::to_string(q) + ")");
+  Staged sc, sd, ss, sun, su;
+  SCTL_TRY(sc.in(near_trg_cnt, (int64_t)sizeof(int64_t) * ntrg, s));
+  SCTL_TRY(sd.in(near_trg_dsp, (int64_t)sizeof(int64_t) * ntrg, s));
+  SCTL_TRY(ss.in(scatter_index, (int64_t)sizeof(int64_t) * n_scatter, s));
+  SCTL_TRY(sun.in(U_near, (int64_t)sizeof(double) * n_near * dof, s));
+  SCTL_TRY(su.in(U, (int64_t)sizeof(double) * ntrg * dof, s));
+  SCTL_TRY(launch(sc.dev<int64_t>(), sd.dev<int64_t>(), ss.dev<int64_t>(), sun.dev<double>(), su.dev<double>()));
+  SCTL_TRY(su.out(U, s));
+  SCTL_HIP_TRY(hipStreamSynchronize(s));
+  return SCTL_AMD_SUCCESS;
+}
+
+}  // extern "C"

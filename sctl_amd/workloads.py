@@ -156,6 +156,66 @@ def torus_blocks(n_u=64, n_v=64, seed=0, k0=1, ns_range=(64, 4096), nt_range=(16
                 src_dsp=src_dsp, src_cnt=src_cnt)
 
 
+def near_field_case(n_u=8, n_v=6, n_trg=400, seed=0, k0=1, p_range=(4, 8), offset=0.05, stencil=1, R=1.0, r=0.3):
+    """A BoundaryIntegralOp near field on a torus, as the reference's arrays.
+
+    Elements are the n_u x n_v (u, v) patches, each with a p x p Gauss-Legendre
+    tensor grid (p uniform in p_range) as its nodes; the far nodes are the
+    nodes (the identity far-field density operator), with quadrature weights
+    w_i w_j du dv |x_u x x_v| and analytic normals.  Targets are points of the
+    torus moved along the normal by U[-offset, offset]; each target is near the
+    (2*stencil+1)^2 patches around its own.  Returned keys follow
+    boundary_integral.txx: X_far, Xn_far, wts_far, elem_nds_cnt, elem_nds_dsp
+    (= *_far), Xtrg, Xn_trg, near_elem_cnt, near_elem_dsp, Xtrg_near,
+    Xn_trg_near (element-major near pairs), near_scatter_index (target-major
+    pair q <- element-major pair scatter[q]), near_trg_cnt, near_trg_dsp, and F
+    (k0 N(0,1) densities per node)."""
+    nelem = n_u * n_v
+    p = p_range[0] + (u64(seed, 30, nelem) % np.uint64(p_range[1] - p_range[0] + 1)).astype(np.int64)
+    du, dv = 2 * np.pi / n_u, 2 * np.pi / n_v
+    X, Nn, W = [], [], []
+    for e in range(nelem):
+        iu, iv = e // n_v, e % n_v
+        x, w = np.polynomial.legendre.leggauss(int(p[e]))
+        x, w = (x + 1) / 2, w / 2
+        uu = ((iu + x)[:, None] * du + 0 * x[None, :]).ravel()
+        vv = (0 * x[:, None] + (iv + x)[None, :] * dv).ravel()
+        Xe, Ne = torus_point(uu, vv, R, r)
+        X.append(Xe)
+        Nn.append(Ne)
+        W.append((w[:, None] * w[None, :]).ravel() * du * dv * r * (R + r * np.cos(vv)))
+    nds_cnt = p * p
+    nds_dsp = np.concatenate([[0], np.cumsum(nds_cnt)[:-1]]).astype(np.int64)
+    ut = uniform(seed, 31, n_trg) * 2 * np.pi
+    vt = uniform(seed, 32, n_trg) * 2 * np.pi
+    ht = uniform(seed, 33, n_trg, -offset, offset)
+    Xt, Nt = torus_point(ut, vt, R, r)
+    Xt = Xt + ht[:, None] * Nt
+    iu_t = np.minimum((ut / du).astype(np.int64), n_u - 1)
+    iv_t = np.minimum((vt / dv).astype(np.int64), n_v - 1)
+    near = []
+    for i in range(n_trg):
+        es = {((iu_t[i] + a) % n_u) * n_v + (iv_t[i] + b) % n_v
+              for a in range(-stencil, stencil + 1) for b in range(-stencil, stencil + 1)}
+        near.append(sorted(es))
+    pairs_e = [(e, i) for i in range(n_trg) for e in near[i]]
+    order_em = sorted(range(len(pairs_e)), key=lambda q: (pairs_e[q][0], pairs_e[q][1]))
+    em_index = np.empty(len(pairs_e), dtype=np.int64)
+    em_index[np.array(order_em, dtype=np.int64)] = np.arange(len(pairs_e))
+    near_trg = np.array([pairs_e[q][1] for q in order_em], dtype=np.int64)
+    near_elem_cnt = np.bincount(np.array([pairs_e[q][0] for q in order_em], dtype=np.int64), minlength=nelem)
+    near_elem_dsp = np.concatenate([[0], np.cumsum(near_elem_cnt)[:-1]]).astype(np.int64)
+    near_trg_cnt = np.array([len(near[i]) for i in range(n_trg)], dtype=np.int64)
+    near_trg_dsp = np.concatenate([[0], np.cumsum(near_trg_cnt)[:-1]]).astype(np.int64)
+    Xf = np.concatenate(X)
+    return dict(X_far=Xf.ravel(), Xn_far=np.concatenate(Nn).ravel(), wts_far=np.concatenate(W),
+                elem_nds_cnt=nds_cnt.astype(np.int64), elem_nds_dsp=nds_dsp, elem_nds_cnt_far=nds_cnt.astype(np.int64),
+                elem_nds_dsp_far=nds_dsp, Xtrg=Xt.ravel(), Xn_trg=Nt.ravel(),
+                near_elem_cnt=near_elem_cnt.astype(np.int64), near_elem_dsp=near_elem_dsp,
+                Xtrg_near=Xt[near_trg].ravel(), Xn_trg_near=Nt[near_trg].ravel(), near_scatter_index=em_index,
+                near_trg_cnt=near_trg_cnt, near_trg_dsp=near_trg_dsp, F=normal(seed, 34, k0 * int(nds_cnt.sum())))
+
+
 def interactions(blocks):
     return int(np.sum(blocks["trg_cnt"].astype(np.int64) * blocks["src_cnt"].astype(np.int64)))
 

@@ -23,6 +23,13 @@ Pins, with what each rests on:
                     -f inside, 0 outside; Stokes FxU(f) = (2/3) f inside
   linearity         Eval(a f + b g) == a Eval(f) + b Eval(g), superposition
                     of two sources (test-generic-kernel.cpp:89-103)
+Near field (oracle/near_field.c):
+  near_partition_hand    the block partition worked by hand
+  near_layout_hand       K_near_cnt / K_near_dsp worked by hand
+  near_subtract_identity (K_after - K_before) F = -Eval(w F) per element
+  near_gemv_numpy        the GEMV against numpy's matrix product
+  near_accumulate_hand   ScatterForward + per-target sum, by hand and
+                         against numpy.add.reduceat
 """
 import math
 import os
@@ -263,5 +270,152 @@ def linearity(orc):
     return not bad, ", ".join(bad)
 
 
+# ------------------------------------------------------------- near field
+# oracle/near_field.c, pinned by hand-worked cases read off the reference's
+# loops, by the identity that ties KNearSubtract to the (already pinned) Eval,
+# and by numpy's matrix product.
+
+
+def near_case(k0, n_u=4, n_v=3, n_trg=60, seed=5):
+    from sctl_amd import workloads as W
+    return W.near_field_case(n_u=n_u, n_v=n_v, n_trg=n_trg, seed=seed, k0=k0)
+
+
+def near_partition_hand(orc):
+    """boundary_integral.txx:1009-1032 worked by hand: N_near = 8,
+    one thread -> grain 2; four threads -> grain max(1, 0) = 1; N_near = 1000,
+    two threads -> grain 125."""
+    bad = []
+    got = orc.near_blocks([5, 0, 3], [0, 5, 5], 1)
+    want = ([0, 0, 0, 2, 2], [0, 2, 4, 0, 2], [2, 2, 1, 2, 1])
+    if not all(np.array_equal(a, b) for a, b in zip(got, want)):
+        bad.append("grain 2: %r" % (got,))
+    got = orc.near_blocks([5, 0, 3], [0, 5, 5], 4)
+    if not (np.array_equal(got[2], np.ones(8)) and np.array_equal(got[1], [0, 1, 2, 3, 4, 0, 1, 2])):
+        bad.append("grain 1: %r" % (got,))
+    got = orc.near_blocks([300, 700], [0, 300], 2)
+    want = ([0, 0, 0] + [1] * 6, [0, 125, 250, 0, 125, 250, 375, 500, 625], [125, 125, 50] + [125] * 5 + [75])
+    if not all(np.array_equal(a, b) for a, b in zip(got, want)):
+        bad.append("grain 125: %r" % (got,))
+    if orc.near_blocks([], [], 3)[0].size != 0:
+        bad.append("empty")
+    return not bad, ", ".join(bad)
+
+
+def near_layout_hand(orc):
+    """K_near_cnt = elem_nds_cnt * near_elem_cnt, 0 for matrix-free elements,
+    K_near_dsp its exclusive scan, |K_near| = total * KDIM0 * KDIM1_
+    (boundary_integral.txx:1039-1057)."""
+    kc, kd, n = orc.near_layout(3, True, [4, 2, 5], [10, 7, 3], matrix_free=[0, 1, 0])
+    ok = list(kc) == [40, 0, 15] and list(kd) == [0, 40, 40] and n == 55 * 3 * 1
+    kc, kd, n2 = orc.near_layout(2, False, [4, 2, 5], [10, 7, 3])
+    ok = ok and list(kc) == [40, 14, 15] and list(kd) == [0, 40, 54] and n2 == 69 * 9
+    return ok, "" if ok else "layout %r %r" % (kc, kd)
+
+
+def _near_setup(orc, kernel, ndot, seed=5, matrix_free=None):
+    _, K0, K1, ND, _ = orc.dims(kernel)
+    c = near_case(K0, seed=seed)
+    blocks = orc.near_blocks(c["near_elem_cnt"], c["near_elem_dsp"], 3)
+    kc, kd, n = orc.near_layout(kernel, ndot, c["near_elem_cnt"], c["elem_nds_cnt"], matrix_free)
+    return c, blocks, kc, kd, n
+
+
+def near_subtract_identity(orc):
+    """KNearSubtract (boundary_integral.txx:1165-1240) on a random K_near: for
+    every element e, (K_after - K_before)_e F_e = -Eval(near targets of e,
+    nodes of e, w * F_e), and with the target-normal contraction
+    -sum_l n_t[l] Eval(...)[t*3+l].  Rests on Eval, which is pinned to the
+    reference bit for bit."""
+    bad = []
+    rng = np.random.default_rng(11)
+    for kernel, ndot in ((0, False), (1, False), (2, False), (3, False), (2, True), (3, True)):
+        _, K0, K1, ND, _ = orc.dims(kernel)
+        dof = K1 // 3 if ndot else K1
+        c, blocks, kc, kd, n = _near_setup(orc, kernel, ndot)
+        K0m = rng.standard_normal(n)
+        K = orc.near_subtract(kernel, ndot, blocks, c["near_elem_dsp"], c["elem_nds_cnt"], c["elem_nds_cnt_far"],
+                              c["elem_nds_dsp_far"], kc, kd, c["Xtrg_near"], c["Xn_trg_near"], c["X_far"],
+                              c["Xn_far"], c["wts_far"], K0m)
+        D = K - K0m
+        worst = 0.0
+        for e in range(c["near_elem_cnt"].size):
+            nt, ns = c["near_elem_cnt"][e], c["elem_nds_cnt"][e]
+            if not nt:
+                continue
+            t0, s0 = c["near_elem_dsp"][e], c["elem_nds_dsp"][e]
+            De = D[kd[e] * K0 * dof:(kd[e] + kc[e]) * K0 * dof].reshape(nt * dof, ns * K0)
+            Fe = c["F"][s0 * K0:(s0 + ns) * K0]
+            w = np.repeat(c["wts_far"][s0:s0 + ns], K0)
+            xt = c["Xtrg_near"][t0 * 3:(t0 + nt) * 3]
+            ev = orc.eval(kernel, xt, c["X_far"][s0 * 3:(s0 + ns) * 3],
+                          c["Xn_far"][s0 * 3:(s0 + ns) * 3] if ND else None, w * Fe)
+            if ndot:
+                ev = (ev.reshape(nt, 3) * c["Xn_trg_near"][t0 * 3:(t0 + nt) * 3].reshape(nt, 3)).sum(1)
+            worst = max(worst, _nrm(De @ Fe, -ev))
+        if not worst < 1e-12:
+            bad.append("k%d ndot=%d err %.2e" % (kernel, ndot, worst))
+    return not bad, ", ".join(bad)
+
+
+def near_gemv_numpy(orc):
+    """ComputeNearInterac's GEMV (boundary_integral.txx:1324-1340) against
+    numpy's matrix product, block by block; a matrix-free element's rows are
+    zero (:1329-1332)."""
+    bad = []
+    rng = np.random.default_rng(12)
+    for kernel, ndot in ((2, False), (3, True), (1, False)):
+        _, K0, K1, ND, _ = orc.dims(kernel)
+        dof = K1 // 3 if ndot else K1
+        mf = np.zeros(12, dtype=np.int32)
+        mf[5] = 1
+        c, blocks, kc, kd, n = _near_setup(orc, kernel, ndot, matrix_free=mf)
+        K = rng.standard_normal(n)
+        n_near = int(c["near_elem_dsp"][-1] + c["near_elem_cnt"][-1])
+        U = orc.near_gemv(kernel, ndot, blocks, c["near_elem_dsp"], c["elem_nds_cnt"], c["elem_nds_dsp"], kc, kd, K,
+                          c["F"], n_near)
+        worst = 0.0
+        for e, t0, cnt in zip(*blocks):
+            r0 = (c["near_elem_dsp"][e] + t0) * dof
+            got = U[r0:r0 + cnt * dof]
+            if mf[e]:
+                if np.any(got != 0):
+                    bad.append("k%d matrix-free rows not zero" % kernel)
+                continue
+            sd = c["elem_nds_cnt"][e] * K0
+            Kb = K[kd[e] * K0 * dof + t0 * dof * sd:][:cnt * dof * sd].reshape(cnt * dof, sd)
+            s0 = c["elem_nds_dsp"][e] * K0
+            worst = max(worst, _nrm(got, Kb @ c["F"][s0:s0 + sd]))
+        if not worst < 1e-14:
+            bad.append("k%d err %.2e" % (kernel, worst))
+    return not bad, ", ".join(bad)
+
+
+def near_accumulate_hand(orc):
+    """ScatterForward then the per-target sum (boundary_integral.txx:1368-1379,
+    comm.txx:1497-1508) worked by hand, and on the torus case against
+    numpy.add.reduceat over the gathered pairs."""
+    bad = []
+    U = orc.near_accumulate(np.array([1.0, 2.0, 3.0]), 1, [2, 0, 1], [0, 2, 2], [2, 0, 1], np.array([10.0, 20.0, 30.0]))
+    if not np.array_equal(U, [41.0, 2.0, 23.0]):
+        bad.append("hand dof 1: %r" % (U,))
+    U = orc.near_accumulate(np.zeros(3), 3, [1], [0], [1, 0], np.arange(6.0))
+    if not np.array_equal(U, [3.0, 4.0, 5.0]):
+        bad.append("hand dof 3: %r" % (U,))
+    c = near_case(3)
+    rng = np.random.default_rng(13)
+    n_pairs = c["near_scatter_index"].size
+    Un = rng.standard_normal(n_pairs * 3)
+    U0 = rng.standard_normal(c["near_trg_cnt"].size * 3)
+    U = orc.near_accumulate(U0, 3, c["near_trg_cnt"], c["near_trg_dsp"], c["near_scatter_index"], Un)
+    g = Un.reshape(-1, 3)[c["near_scatter_index"]]
+    ref = U0.reshape(-1, 3) + np.add.reduceat(g, c["near_trg_dsp"], axis=0)
+    if _nrm(U, ref.ravel()) > 1e-15:
+        bad.append("torus case")
+    return not bad, ", ".join(bad)
+
+
+NEAR_CHECKS = [near_partition_hand, near_layout_hand, near_subtract_identity, near_gemv_numpy, near_accumulate_hand]
+
 CHECKS = [golden_exact, golden_avx512, golden_rsqrt, rsqrt_bounds, closed_forms, long_double, eval_vs_matrix,
-          surface_identities, linearity]
+          surface_identities, linearity] + NEAR_CHECKS
