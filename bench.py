@@ -250,6 +250,21 @@ def near_ops(kernel, B, stream, reps):
     return res
 
 
+def hbm_copy_reference(stream, gib=4):
+    """Achievable HBM bandwidth for context: one device-to-device copy of
+    `gib` GiB (read + write bytes / time)."""
+    n = gib << 30
+    a, b = S.DeviceArray(n // 8), S.DeviceArray(n // 8)
+    L = S.lib()
+    S.check(L.sctl_amd_memcpy(b.ptr, a.ptr, n, 2, stream))
+    S.synchronize(stream)
+    ms = time_once(lambda: S.check(L.sctl_amd_memcpy(b.ptr, a.ptr, n, 2, stream)), stream, 3)
+    a.free()
+    b.free()
+    return {"bytes": 2 * n, "ms": round(ms, 3), "GB_per_s": round(2 * n / ms / 1e6, 1),
+            "frac_hbm_peak": round(2 * n / ms / 1e6 / 8000.0, 3)}
+
+
 def other_configs(stream, quick=False):
     """The other configurations: measured, not the headline."""
     import oracle
@@ -281,6 +296,8 @@ def other_configs(stream, quick=False):
     log("config 4: batched Laplace DxU, 4096 torus blocks")
     B = W.torus_blocks(64, 64, seed=0, k0=1)
     out["c4_batched_laplace_dxu_4096"], _ = plan_pair(S.LAPLACE3D_DXU, B, stream, reps=5)
+    log("HBM reference: device-to-device copy")
+    out["hbm_copy_4GiB"] = hbm_copy_reference(stream)
     log("config 4 blocks: near-field matrices (KNearSubtract, GEMV), Laplace DxU")
     out["c4_near_matrix_laplace_dxu_4096"] = near_ops(S.LAPLACE3D_DXU, B, stream, reps=3)
     log("config 5: batched Stokes DxU, 2048 torus blocks, 100 applies in one hipGraph")
@@ -307,6 +324,9 @@ def main():
                     help="with --headline-only: which path to run")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--shape", default=None, metavar="TPL,CAP,MC",
+                    help="pin the mixed headline launch shape (targets per lane, workgroups-per-CU cap, "
+                         "matrix cores 0/1) instead of tuning it")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -332,11 +352,18 @@ def main():
         S.synchronize(stream)
         log("calibration copy: %d bytes device to device" % nb)
     tuned = {}
+    if args.shape:
+        tpl, cap, mc = (int(x) for x in args.shape.split(","))
+        S.set_dense_shape(K, N, N, tpl, cap, mc, precision=S.MIXED)
+        tuned["mixed"] = {"targets_per_lane": tpl, "wg_per_cu_cap": cap, "matrix_cores": mc, "pinned": True}
     if not args.headline_only and not args.no_tune:
         log("autotuning the launch shapes (not timed)")
         for name, prec, dig in (("mixed", S.MIXED, -1), ("fp64", S.FP64, -1), ("fp64_digits14", S.FP64, 14)):
-            tpl, cap, ms = S.tune_eval(K, dev[0], dev[1], None, dev[3], out, precision=prec, digits=dig, stream=stream)
-            tuned[name] = {"targets_per_lane": tpl, "wg_per_cu_cap": cap, "ms": round(ms, 3)}
+            if name in tuned:
+                continue
+            tpl, cap, ms, mc = S.tune_eval(K, dev[0], dev[1], None, dev[3], out, precision=prec, digits=dig,
+                                           stream=stream)
+            tuned[name] = {"targets_per_lane": tpl, "wg_per_cu_cap": cap, "matrix_cores": mc, "ms": round(ms, 3)}
         out.upload(np.zeros(N))
     log("mixed precision: %d warmup + %d timed steps" % (args.warmup, args.steps))
     hp = S.FP64 if (args.headline_only and args.headline_precision == "fp64") else S.MIXED

@@ -250,11 +250,22 @@ int sctl_amd_gen_normal(uint64_t seed, uint64_t stream_id, int64_t n, double* ou
  * sctl_amd_eval / sctl_amd_plan_create with a matching key use the result. */
 int sctl_amd_tune_db_path(const char* path);
 
-/* Dense Eval: targets per lane {1,2,4} x workgroups-per-CU cap {none,3,2}.
- * Device pointers only; v_scratch (nt*K1) receives garbage. */
+/* Dense Eval: targets per lane {1,2,4} x workgroups-per-CU cap {none,3,2},
+ * and for the mixed Laplace kernels the far path on the VALU or on the matrix
+ * cores.  Device pointers only; v_scratch (nt*K1) receives garbage. */
 int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_trg, int64_t nt, const double* r_src,
                        const double* n_src, const double* v_src, int64_t ns, double* v_scratch, uint32_t flags,
-                       void* stream, int* best_tpl, int* best_occ, double* best_ms);
+                       void* stream, int* best_tpl, int* best_occ, double* best_ms, int* best_matrix_cores);
+
+/* The launch shape sctl_amd_eval uses for this (kernel, precision, digits,
+ * flags, nt, ns) size class: get resolves it (tuning database, else
+ * defaults); set records one in the database (targets_per_lane 1, 2 or 4;
+ * wg_per_cu_cap >= 0; matrix_cores 0/1, honoured by the mixed Laplace
+ * kernels only).  Host-only. */
+int sctl_amd_dense_shape_get(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
+                             int* targets_per_lane, int* wg_per_cu_cap, int* matrix_cores);
+int sctl_amd_dense_shape_set(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
+                             int targets_per_lane, int wg_per_cu_cap, int matrix_cores);
 
 /* Plan: sources per lane {1,2,4} x targets per lane {1,2} x cap {none,4,2};
  * leaves the plan configured with the fastest.  v_scratch (nt_total*K1,

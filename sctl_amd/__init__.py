@@ -42,7 +42,7 @@ SYMBOLS = [
     "sctl_amd_graph_interactions", "sctl_amd_graph_destroy", "sctl_amd_gen_normal", "sctl_amd_plan_get_shape",
     "sctl_amd_plan_set_shape", "sctl_amd_tune_db_path", "sctl_amd_tune_eval", "sctl_amd_plan_tune",
     "sctl_amd_near_blocks", "sctl_amd_plan_near_layout", "sctl_amd_plan_near_matrix", "sctl_amd_plan_near_gemv",
-    "sctl_amd_near_accumulate",
+    "sctl_amd_near_accumulate", "sctl_amd_dense_shape_get", "sctl_amd_dense_shape_set",
 ]
 
 
@@ -97,7 +97,9 @@ def lib():
     L.sctl_amd_plan_get_shape.argtypes = [_vp, ip, ip, ip]
     L.sctl_amd_plan_set_shape.argtypes = [_vp, i32, i32, i32]
     L.sctl_amd_tune_db_path.argtypes = [ctypes.c_char_p]
-    L.sctl_amd_tune_eval.argtypes = [i32, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp, ip, ip, _dp]
+    L.sctl_amd_tune_eval.argtypes = [i32, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp, ip, ip, _dp, ip]
+    L.sctl_amd_dense_shape_get.argtypes = [i32, i32, i32, u32, i64, i64, ip, ip, ip]
+    L.sctl_amd_dense_shape_set.argtypes = [i32, i32, i32, u32, i64, i64, i32, i32, i32]
     L.sctl_amd_plan_tune.argtypes = [_vp, _vp, _vp, _vp, ip, ip, ip, _dp]
     L.sctl_amd_near_blocks.argtypes = [i64, _i64p, _i64p, i32, _i64p, _i64p, _i64p, _i64p]
     L.sctl_amd_plan_near_layout.argtypes = [_vp, _i64p, i32, _i64p]
@@ -434,13 +436,30 @@ def tune_db_path(path):
 
 
 def tune_eval(kernel, r_trg, r_src, n_src, v_src, v_scratch, precision=FP64, digits=-1, flags=0, stream=None):
-    """Autotunes the dense Eval shape for these device arrays; returns (tpl, cap, ms)."""
+    """Autotunes the dense Eval shape for these device arrays; returns
+    (targets_per_lane, wg_per_cu_cap, ms, matrix_cores)."""
     nt, ns = r_trg.nbytes // 24, r_src.nbytes // 24
-    a, b = ctypes.c_int(), ctypes.c_int()
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     ms = ctypes.c_double()
     check(lib().sctl_amd_tune_eval(kernel, precision, digits, r_trg.ptr, nt, r_src.ptr, _ptr(n_src), v_src.ptr, ns,
-                                   v_scratch.ptr, flags, stream, ctypes.byref(a), ctypes.byref(b), ctypes.byref(ms)))
-    return a.value, b.value, ms.value
+                                   v_scratch.ptr, flags, stream, ctypes.byref(a), ctypes.byref(b), ctypes.byref(ms),
+                                   ctypes.byref(c)))
+    return a.value, b.value, ms.value, c.value
+
+
+def dense_shape(kernel, nt, ns, precision=FP64, digits=-1, flags=0):
+    """(targets_per_lane, wg_per_cu_cap, matrix_cores) that Eval uses for this size class."""
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib().sctl_amd_dense_shape_get(kernel, precision, digits, flags, int(nt), int(ns), ctypes.byref(a),
+                                         ctypes.byref(b), ctypes.byref(c)))
+    return a.value, b.value, c.value
+
+
+def set_dense_shape(kernel, nt, ns, targets_per_lane, wg_per_cu_cap=0, matrix_cores=0, precision=FP64, digits=-1,
+                    flags=0):
+    """Records the Eval launch shape for this size class in the tuning database."""
+    check(lib().sctl_amd_dense_shape_set(kernel, precision, digits, flags, int(nt), int(ns), targets_per_lane,
+                                         wg_per_cu_cap, matrix_cores))
 
 
 def gen_normal(seed, stream_id, out, stream=None):

@@ -25,6 +25,8 @@ SOURCES = ["dense.hip", "mixed.hip", "sort.hip", "batched.hip", "near.hip", "gra
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core results in VGPRs: the VALU reads them without accvgpr copies
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I" + os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function",
 ]
 

@@ -79,9 +79,12 @@ class Scratch {
 
 // Launch-shape knobs of the dense kernels (chosen by the tuner or defaults).
 struct DenseShape {
-  int tpl = 0;  // targets per lane: 1, 2 or 4 (0: default for the size)
-  int occ = 0;  // cap on resident workgroups per CU (0: no cap); enforced with dynamic LDS
+  int tpl = 0;   // targets per lane: 1, 2 or 4 (0: default for the size)
+  int occ = 0;   // cap on resident workgroups per CU (0: no cap); enforced with dynamic LDS
+  int mfma = 0;  // mixed Laplace kernels: far tiles on the matrix cores
 };
+// Kernels whose mixed-precision far path has a matrix-core form.
+inline bool mfma_supported(int ker) { return ker == 0 || ker == 1; }
 
 // Dynamic LDS bytes that cap a kernel with `static_lds` bytes of LDS at `occ`
 // resident workgroups per CU (0 when occ <= 0).  Sets the kernel's dynamic-LDS

@@ -23,10 +23,13 @@
 // All three kernels are bound by HBM traffic (8 bytes of K_near per matrix
 // entry, read and written by the subtract, read by the GEMV); the matrix
 // entries are FP64 whatever the plan's precision.  Work is cut into wave
-// tiles listed once per layout (tile -> block, no search at run time):
-//   near matrix  16 targets x 64 sources: one source per lane, the target
-//                loop wave-uniform, so every row segment a wave writes is
-//                64*KDIM0 consecutive doubles;
+// tiles listed once per layout (tile -> block, no search at run time), each
+// a contiguous range of K_near so that no cache line is shared between
+// waves (or XCDs) except at tile ends, blocks longest first:
+//   near matrix  a stripe of 16 targets x all the block's sources: one
+//                source per lane, 64 at a time, the target loop
+//                wave-uniform (targets broadcast from lanes 0..15), so every
+//                row segment written is 64*KDIM0 consecutive doubles;
 //   GEMV         16 rows: 16 lanes per row, 4 rows at a time, the row's
 //                reduction by cross-lane shuffles inside the 16-lane group.
 // Nothing here allocates or issues a memset with device pointers, so the
@@ -47,13 +50,24 @@ namespace sctl_amd {
 namespace {
 
 constexpr int NWG = 256;        // 4 independent waves per workgroup
-constexpr int MAT_RT = 16;      // targets per near-matrix wave tile
+constexpr int MAT_ST = 16;      // targets per near-matrix stripe (one wave)
 constexpr int GEMV_ROWS = 16;   // rows per GEMV wave tile
 constexpr int GEMV_LPR = 16;    // lanes per row in the GEMV
 constexpr int NEAR_NEWTON = 2;  // KernelMatrix at digits -1 (boundary_integral.txx:1197,1211)
 
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long i = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(i & 0xffffffffLL), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(i >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // KNearSubtract / store of one layout's blocks.  NDOT: contract the KDIM1
 // outputs with the target normal (KDIM1 == 3 -> one row per target).
+// A wave owns one stripe (MAT_ST targets x all sources of a block), a
+// contiguous range of K_near: lanes 0..MAT_ST-1 hold the stripe's targets,
+// the wave walks the sources 64 at a time (one per lane) and, per chunk,
+// writes every target row's 64*KDIM0 entries.
 template <int KER, bool NDOT, bool SUB>
 __global__ void __launch_bounds__(NWG)
 k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ tile_dsp, int64_t ntiles,
@@ -64,52 +78,59 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
               double* __restrict__ K, double scale) {
   using Kt = Ker<KER>;
   constexpr int K0 = Kt::K0, K1 = Kt::K1, ND = Kt::ND;
+  constexpr int DOF = NDOT ? 1 : K1;
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (NWG / 64);
   for (int64_t tile = (int64_t)blockIdx.x * (NWG / 64) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
     const int b = __builtin_amdgcn_readfirstlane(tile_blk[tile]);
-    const int64_t local = tile - tile_dsp[b];
-    const int sc = src_cnt[b], tc = trg_cnt[b];
-    const int ts = (sc + 63) >> 6;
-    const int ti = (int)(local / ts);
-    const int si = (int)(local - (int64_t)ti * ts);
-    const int s = si * 64 + lane;
-    const bool active = s < sc;
-    const int64_t gs = src_dsp[b] + (active ? s : 0);
-    const double sx = xs[gs * 3 + 0], sy = xs[gs * 3 + 1], sz = xs[gs * 3 + 2];
-    double n[3] = {0, 0, 0};
+    const int t0 = (int)(tile - tile_dsp[b]) * MAT_ST;
+    const int sc = src_cnt[b];
+    const int nts = min(MAT_ST, trg_cnt[b] - t0);
+    const int64_t tg = trg_dsp[b] + t0 + (lane < nts ? lane : 0);
+    const double mx = xt[tg * 3 + 0], my = xt[tg * 3 + 1], mz = xt[tg * 3 + 2];
+    double mn[3] = {0, 0, 0};
+    if constexpr (NDOT) {
 #pragma unroll
-    for (int k = 0; k < ND; k++) n[k] = xn[gs * 3 + k];
-    const double w = wts ? wts[gs] : 1.0;
+      for (int l = 0; l < 3; l++) mn[l] = n_trg[tg * 3 + l];
+    }
     const int64_t ld = (int64_t)sc * K0;
-    double* Kb = K + mat_off[b] + (int64_t)s * K0;
-    const int64_t tg0 = trg_dsp[b];
-    const int t_end = min(ti * MAT_RT + MAT_RT, tc);
-    for (int t = ti * MAT_RT; t < t_end; t++) {
-      const double* x = xt + (tg0 + t) * 3;
-      double u[3][3];
-      kmatrix<KER, NEAR_NEWTON>(u, x[0] - sx, x[1] - sy, x[2] - sz, n);
-      if (!active) continue;
-      if constexpr (NDOT) {
-        const double* nt = n_trg + (tg0 + t) * 3;
-        double* row = Kb + (int64_t)t * ld;
+    double* Ks = K + mat_off[b] + (int64_t)t0 * DOF * ld;
+    for (int s0 = 0; s0 < sc; s0 += 64) {
+      const int s = s0 + lane;
+      const bool active = s < sc;
+      const int64_t gs = src_dsp[b] + (active ? s : 0);
+      const double sx = xs[gs * 3 + 0], sy = xs[gs * 3 + 1], sz = xs[gs * 3 + 2];
+      double n[3] = {0, 0, 0};
 #pragma unroll
-        for (int k0 = 0; k0 < K0; k0++) {
-          double v = 0.0;
-#pragma unroll
-          for (int l = 0; l < 3; l++) v += (u[k0][l] * scale) * w * nt[l];
-          if constexpr (SUB) row[k0] -= v;
-          else row[k0] = v;
-        }
-      } else {
-#pragma unroll
-        for (int k1 = 0; k1 < K1; k1++) {
-          double* row = Kb + (int64_t)(t * K1 + k1) * ld;
+      for (int k = 0; k < ND; k++) n[k] = xn[gs * 3 + k];
+      const double ws = wts ? wts[gs] : 1.0;
+      double* Kc = Ks + (int64_t)s * K0;
+      for (int t = 0; t < nts; t++) {
+        double u[3][3];
+        kmatrix<KER, NEAR_NEWTON>(u, readlane_d(mx, t) - sx, readlane_d(my, t) - sy, readlane_d(mz, t) - sz, n);
+        if (!active) continue;
+        if constexpr (NDOT) {
+          const double n0 = readlane_d(mn[0], t), n1 = readlane_d(mn[1], t), n2 = readlane_d(mn[2], t);
+          double* row = Kc + (int64_t)t * ld;
 #pragma unroll
           for (int k0 = 0; k0 < K0; k0++) {
-            const double v = (u[k0][k1] * scale) * w;
+            double v = 0.0;
+            v += (u[k0][0] * scale) * ws * n0;
+            v += (u[k0][1] * scale) * ws * n1;
+            v += (u[k0][2] * scale) * ws * n2;
             if constexpr (SUB) row[k0] -= v;
             else row[k0] = v;
+          }
+        } else {
+#pragma unroll
+          for (int k1 = 0; k1 < K1; k1++) {
+            double* row = Kc + (int64_t)(t * K1 + k1) * ld;
+#pragma unroll
+            for (int k0 = 0; k0 < K0; k0++) {
+              const double v = (u[k0][k1] * scale) * ws;
+              if constexpr (SUB) row[k0] -= v;
+              else row[k0] = v;
+            }
           }
         }
       }
@@ -119,7 +140,9 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
 
 // U_near[block rows] = K_near_ * F_element.  Blocks without a matrix
 // (offset < 0) or without sources get zero rows, as the reference's GEMV loop
-// writes them (boundary_integral.txx:1329-1332).
+// writes them (boundary_integral.txx:1329-1332).  A wave owns GEMV_ROWS
+// consecutive rows (a contiguous range of K_near); 16 lanes per row, 8
+// independent loads per lane in flight, two partial sums.
 __global__ void __launch_bounds__(NWG)
 k_near_gemv(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ tile_dsp, int64_t ntiles,
             const int64_t* __restrict__ trg_dsp, const int32_t* __restrict__ trg_cnt,
@@ -140,12 +163,25 @@ k_near_gemv(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ ti
 #pragma unroll
     for (int rr = 0; rr < GEMV_ROWS; rr += 64 / GEMV_LPR) {
       const int64_t row = local * GEMV_ROWS + rr + grp;
-      double acc = 0.0;
+      double acc0 = 0.0, acc1 = 0.0;
       if (row < rows && off >= 0) {
         const double* Kr = K + off + row * ld;
-#pragma unroll 4
-        for (int64_t j = li; j < ld; j += GEMV_LPR) acc = fmad(Kr[j], Fb[j], acc);
+        int64_t j = li;
+        for (; j + 7 * GEMV_LPR < ld; j += 8 * GEMV_LPR) {
+          double k[8], f[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) k[u] = Kr[j + u * GEMV_LPR];
+#pragma unroll
+          for (int u = 0; u < 8; u++) f[u] = Fb[j + u * GEMV_LPR];
+#pragma unroll
+          for (int u = 0; u < 8; u += 2) {
+            acc0 = fmad(k[u], f[u], acc0);
+            acc1 = fmad(k[u + 1], f[u + 1], acc1);
+          }
+        }
+        for (; j < ld; j += GEMV_LPR) acc0 = fmad(Kr[j], Fb[j], acc0);
       }
+      double acc = acc0 + acc1;
 #pragma unroll
       for (int m = GEMV_LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, 64);
       if (li == 0 && row < rows) Ub[row] = acc;
@@ -196,17 +232,23 @@ NearMatFn near_mat_fn(int ker, bool ndot, bool sub) {
   }
 }
 
-// Flattens per-block tile counts into a device tile list.
-int build_tiles(const std::vector<int64_t>& per_block, TileList* out) {
-  std::vector<int64_t> dsp(per_block.size());
+// Flattens per-block tile counts into a device tile list, blocks in
+// decreasing order of cost per tile (longest-processing-time first; a
+// block's tiles stay consecutive).
+int build_tiles(const std::vector<int64_t>& per_block, const std::vector<int64_t>& cost, TileList* out) {
+  const size_t nb = per_block.size();
+  std::vector<size_t> order(nb);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+  std::vector<int64_t> dsp(nb);
   int64_t total = 0;
-  for (size_t b = 0; b < per_block.size(); b++) {
+  for (size_t b : order) {
     dsp[b] = total;
     total += per_block[b];
   }
   SCTL_CHECK_ARG(total < (int64_t)1 << 34, "near-field layout has too many tiles");
   std::vector<int32_t> blk((size_t)total);
-  for (size_t b = 0; b < per_block.size(); b++)
+  for (size_t b = 0; b < nb; b++)
     for (int64_t i = 0; i < per_block[b]; i++) blk[(size_t)(dsp[b] + i)] = (int32_t)b;
   out->ntiles = total;
   SCTL_TRY(dev_upload(&out->d_tile_blk, blk));
@@ -341,16 +383,17 @@ int sctl_amd_plan_near_layout(sctl_amd_plan p, const int64_t* blk_off, int trg_n
                      "block matrices overlap (blocks " + std::to_string(order[i - 1]) + " and " +
                          std::to_string(order[i]) + ")");
   }
-  std::vector<int64_t> mat(nb), gemv(nb);
+  std::vector<int64_t> mat(nb), gemv(nb), cost(nb);
   for (int64_t b = 0; b < nb; b++) {
     const int64_t tc = p->trg_cnt[b], sc = p->src_cnt[b];
-    mat[b] = (off[b] >= 0 && tc > 0 && sc > 0) ? ((tc + MAT_RT - 1) / MAT_RT) * ((sc + 63) / 64) : 0;
+    mat[b] = (off[b] >= 0 && tc > 0 && sc > 0) ? (tc + MAT_ST - 1) / MAT_ST : 0;
     gemv[b] = tc > 0 ? (tc * dof + GEMV_ROWS - 1) / GEMV_ROWS : 0;
+    cost[b] = off[b] >= 0 ? sc : 0;
   }
   p->free_near();
   int rc = dev_upload(&p->d_near_off, off);
-  if (!rc) rc = build_tiles(mat, &p->mat_tiles);
-  if (!rc) rc = build_tiles(gemv, &p->gemv_tiles);
+  if (!rc) rc = build_tiles(mat, cost, &p->mat_tiles);
+  if (!rc) rc = build_tiles(gemv, cost, &p->gemv_tiles);
   if (rc) {
     p->free_near();
     return rc;

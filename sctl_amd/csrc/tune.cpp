@@ -161,6 +161,7 @@ DenseShape pick_dense_shape(int ker, int precision, int newton, int64_t nt, int6
   if (lookup(dense_key(ker, precision, newton, nt, ns), &v) && v.size() >= 2) {
     s.tpl = (int)v[0];
     s.occ = (int)v[1];
+    s.mfma = v.size() >= 4 ? (int)v[3] : 0;
     return s;
   }
   // Enough target blocks to fill the device with two targets per lane,
@@ -194,9 +195,53 @@ int sctl_amd_tune_db_path(const char* path) {
   return SCTL_AMD_SUCCESS;
 }
 
+namespace {
+
+// Resolves precision AUTO and the Newton count exactly as sctl_amd_eval does.
+int resolve_dense(int kernel, int* precision, int digits, uint32_t flags, int* newton) {
+  int K0, K1, ND;
+  if (kernel_dims(kernel, &K0, &K1, &ND) != SCTL_AMD_SUCCESS) {
+    set_error("unknown kernel id " + std::to_string(kernel));
+    return SCTL_AMD_ERROR_UNKNOWN_KERNEL;
+  }
+  SCTL_CHECK_ARG(digits >= -1, "digits must be -1 or >= 0");
+  SCTL_CHECK_ARG(*precision >= SCTL_AMD_FP64 && *precision <= SCTL_AMD_AUTO, "unknown precision");
+  if (*precision == SCTL_AMD_AUTO) *precision = (digits >= 0 && digits <= 7) ? SCTL_AMD_MIXED : SCTL_AMD_FP64;
+  *newton = *precision == SCTL_AMD_FP64 ? newton_steps_f64(digits) : ((flags & SCTL_AMD_MIXED_NEWTON) ? 1 : 0);
+  return SCTL_AMD_SUCCESS;
+}
+
+}  // namespace
+
+int sctl_amd_dense_shape_get(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
+                             int* targets_per_lane, int* wg_per_cu_cap, int* matrix_cores) {
+  int newton = 0;
+  SCTL_TRY(resolve_dense(kernel, &precision, digits, flags, &newton));
+  SCTL_CHECK_ARG(nt >= 0 && ns >= 0, "negative size");
+  const DenseShape s = pick_dense_shape(kernel, precision, newton, nt, ns);
+  if (targets_per_lane) *targets_per_lane = s.tpl;
+  if (wg_per_cu_cap) *wg_per_cu_cap = s.occ;
+  if (matrix_cores) *matrix_cores = s.mfma;
+  return SCTL_AMD_SUCCESS;
+}
+
+int sctl_amd_dense_shape_set(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
+                             int targets_per_lane, int wg_per_cu_cap, int matrix_cores) {
+  int newton = 0;
+  SCTL_TRY(resolve_dense(kernel, &precision, digits, flags, &newton));
+  SCTL_CHECK_ARG(nt >= 0 && ns >= 0, "negative size");
+  SCTL_CHECK_ARG(targets_per_lane == 1 || targets_per_lane == 2 || targets_per_lane == 4,
+                 "targets_per_lane must be 1, 2 or 4");
+  SCTL_CHECK_ARG(wg_per_cu_cap >= 0, "wg_per_cu_cap must be >= 0");
+  SCTL_CHECK_ARG(matrix_cores == 0 || matrix_cores == 1, "matrix_cores must be 0 or 1");
+  store(dense_key(kernel, precision, newton, nt, ns),
+        {(double)targets_per_lane, (double)wg_per_cu_cap, 0.0, (double)matrix_cores});
+  return SCTL_AMD_SUCCESS;
+}
+
 int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_trg, int64_t nt, const double* r_src,
                        const double* n_src, const double* v_src, int64_t ns, double* v_scratch, uint32_t flags,
-                       void* stream, int* best_tpl, int* best_occ, double* best_ms) {
+                       void* stream, int* best_tpl, int* best_occ, double* best_ms, int* best_matrix_cores) {
   int K0, K1, ND;
   if (kernel_dims(kernel, &K0, &K1, &ND) != SCTL_AMD_SUCCESS) {
     set_error("unknown kernel id " + std::to_string(kernel));
@@ -214,28 +259,34 @@ int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_tr
   const int newton = precision == SCTL_AMD_FP64 ? newton_steps_f64(digits) : ((flags & SCTL_AMD_MIXED_NEWTON) ? 1 : 0);
   double best = 1e300;
   DenseShape bs;
-  for (int tpl : {1, 2, 4}) {
-    for (int occ : {0, 3, 2}) {
-      DenseShape sh;
-      sh.tpl = tpl;
-      sh.occ = occ;
-      double ms = 0;
-      auto run = [&]() {
-        return precision == SCTL_AMD_FP64
-                   ? launch_eval_f64(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, s)
-                   : launch_eval_mixed(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, s);
-      };
-      SCTL_TRY(time_ms(run, s, 2, &ms));
-      if (ms < best) {
-        best = ms;
-        bs = sh;
+  const bool with_mfma = precision == SCTL_AMD_MIXED && mfma_supported(kernel);
+  for (int mfma = 0; mfma <= (with_mfma ? 1 : 0); mfma++) {
+    for (int tpl : {1, 2, 4}) {
+      if (mfma && tpl == 4) continue;  // 16 row tiles per wave do not fit the register file
+      for (int occ : {0, 3, 2}) {
+        DenseShape sh;
+        sh.tpl = tpl;
+        sh.occ = occ;
+        sh.mfma = mfma;
+        double ms = 0;
+        auto run = [&]() {
+          return precision == SCTL_AMD_FP64
+                     ? launch_eval_f64(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, s)
+                     : launch_eval_mixed(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, s);
+        };
+        SCTL_TRY(time_ms(run, s, 2, &ms));
+        if (ms < best) {
+          best = ms;
+          bs = sh;
+        }
       }
     }
   }
-  store(dense_key(kernel, precision, newton, nt, ns), {(double)bs.tpl, (double)bs.occ, best});
+  store(dense_key(kernel, precision, newton, nt, ns), {(double)bs.tpl, (double)bs.occ, best, (double)bs.mfma});
   if (best_tpl) *best_tpl = bs.tpl;
   if (best_occ) *best_occ = bs.occ;
   if (best_ms) *best_ms = best;
+  if (best_matrix_cores) *best_matrix_cores = bs.mfma;
   return SCTL_AMD_SUCCESS;
 }
 
