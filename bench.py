@@ -251,18 +251,25 @@ def near_ops(kernel, B, stream, reps):
 
 
 def hbm_copy_reference(stream, gib=4):
-    """Achievable HBM bandwidth for context: one device-to-device copy of
-    `gib` GiB (read + write bytes / time)."""
+    """Achievable HBM bandwidth for context: the runtime's device-to-device
+    copy of `gib` GiB (read + write bytes / time) and its fill of the same
+    buffer (write bytes / time)."""
+    from sctl_amd.hiptime import memset_async
     n = gib << 30
     a, b = S.DeviceArray(n // 8), S.DeviceArray(n // 8)
     L = S.lib()
     S.check(L.sctl_amd_memcpy(b.ptr, a.ptr, n, 2, stream))
     S.synchronize(stream)
     ms = time_once(lambda: S.check(L.sctl_amd_memcpy(b.ptr, a.ptr, n, 2, stream)), stream, 3)
+    memset_async(b.ptr, 0, n, stream)
+    S.synchronize(stream)
+    ms_fill = time_once(lambda: memset_async(b.ptr, 0, n, stream), stream, 3)
     a.free()
     b.free()
-    return {"bytes": 2 * n, "ms": round(ms, 3), "GB_per_s": round(2 * n / ms / 1e6, 1),
-            "frac_hbm_peak": round(2 * n / ms / 1e6 / 8000.0, 3)}
+    return {"copy": {"bytes": 2 * n, "ms": round(ms, 3), "GB_per_s": round(2 * n / ms / 1e6, 1),
+                     "frac_hbm_peak": round(2 * n / ms / 1e6 / 8000.0, 3)},
+            "fill": {"bytes": n, "ms": round(ms_fill, 3), "GB_per_s": round(n / ms_fill / 1e6, 1),
+                     "frac_hbm_peak": round(n / ms_fill / 1e6 / 8000.0, 3)}}
 
 
 def other_configs(stream, quick=False):
@@ -296,7 +303,7 @@ def other_configs(stream, quick=False):
     log("config 4: batched Laplace DxU, 4096 torus blocks")
     B = W.torus_blocks(64, 64, seed=0, k0=1)
     out["c4_batched_laplace_dxu_4096"], _ = plan_pair(S.LAPLACE3D_DXU, B, stream, reps=5)
-    log("HBM reference: device-to-device copy")
+    log("HBM reference: device-to-device copy and fill")
     out["hbm_copy_4GiB"] = hbm_copy_reference(stream)
     log("config 4 blocks: near-field matrices (KNearSubtract, GEMV), Laplace DxU")
     out["c4_near_matrix_laplace_dxu_4096"] = near_ops(S.LAPLACE3D_DXU, B, stream, reps=3)

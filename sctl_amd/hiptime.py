@@ -17,6 +17,7 @@ def hip():
         L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
         L.hipEventDestroy.argtypes = [vp]
         L.hipDeviceSynchronize.argtypes = []
+        L.hipMemsetAsync.argtypes = [vp, ctypes.c_int, ctypes.c_size_t, vp]
         _hip = L
     return _hip
 
@@ -58,3 +59,11 @@ def device_synchronize():
     rc = hip().hipDeviceSynchronize()
     if rc != 0:
         raise RuntimeError("hipDeviceSynchronize failed (%d)" % rc)
+
+
+def memset_async(ptr, value, nbytes, stream):
+    """hipMemsetAsync on `stream` (benchmark reference only; never inside a captured graph)."""
+    rc = hip().hipMemsetAsync(ptr, value, nbytes, stream)
+    if rc != 0:
+        raise RuntimeError("hipMemsetAsync failed (%d)" % rc)
+
