@@ -220,6 +220,33 @@ int sctl_amd_near_accumulate(int64_t ntrg, int dof, const int64_t* near_trg_cnt,
                              int64_t n_scatter, const int64_t* scatter_index, int64_t n_near, const double* U_near,
                              double* U, uint32_t flags, void* stream);
 
+/* ------------------------------------------------------------ near lists
+ * Replaces BuildNearList (boundary_integral.txx:50-520) on one process:
+ * target t is near element e when some node s of e (nodes
+ * [elem_nds_dsp[e], +elem_nds_cnt[e]) of Xsrc) has |x_t - x_s|^2 <
+ * src_radius[s]^2.  The near pairs are listed by element, then target index;
+ * Xtrg_near / Xn_trg_near hold the pairs' target coordinates / normals,
+ * near_elem_cnt / near_elem_dsp count them per element, near_scatter_index
+ * lists them in target order (pairs of one target in element order; the
+ * reference leaves that order unspecified, comm.txx:1393 sorts by key only),
+ * near_trg_cnt / near_trg_dsp count them per target (dsp 0 for a target with
+ * no pair, as :485-514).  The build synchronises `stream` (it sizes the
+ * output); the list keeps its results on the device until destroyed. */
+typedef struct sctl_amd_near_list_s* sctl_amd_near_list;
+
+int sctl_amd_near_list_build(sctl_amd_near_list* list, const double* Xtrg, int64_t ntrg, const double* Xn_trg,
+                             const double* Xsrc, int64_t nsrc, const double* src_radius, int64_t nelem,
+                             const int64_t* elem_nds_cnt, const int64_t* elem_nds_dsp, uint32_t flags,
+                             void* stream);
+/* Number of near pairs (length of near_scatter_index; Xtrg_near has 3x). */
+int sctl_amd_near_list_size(sctl_amd_near_list list, int64_t* n_near);
+/* Copies the results out (host with SCTL_AMD_HOST_PTRS, else device); any
+ * output may be NULL.  Xn_trg_near needs a list built with Xn_trg. */
+int sctl_amd_near_list_get(sctl_amd_near_list list, double* Xtrg_near, double* Xn_trg_near, int64_t* near_elem_cnt,
+                           int64_t* near_elem_dsp, int64_t* near_scatter_index, int64_t* near_trg_cnt,
+                           int64_t* near_trg_dsp, uint32_t flags, void* stream);
+int sctl_amd_near_list_destroy(sctl_amd_near_list list);
+
 /* ------------------------------------------------------ repeated applies
  * A Krylov solve applies the same operator once per iteration with new
  * densities (lin-solve.txx:146-156 calling BoundaryIntegralOp::

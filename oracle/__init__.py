@@ -55,6 +55,28 @@ def _ip(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_long)) if a is not None and a.size else None
 
 
+def _near_list(fn, Xtrg, Xsrc, src_radius, elem_nds_cnt, elem_nds_dsp, Xn_trg=None):
+    """Calls a BuildNearList-shaped C function twice (size, then fill)."""
+    Xt, Xs, rad = _c(Xtrg), _c(Xsrc), _c(src_radius)
+    Xn = None if Xn_trg is None else _c(Xn_trg)
+    ec, ed = _i(elem_nds_cnt), _i(elem_nds_dsp)
+    nt, ns, ne = Xt.size // 3, Xs.size // 3, ec.size
+    n = fn(_p(Xt), nt, _p(Xn), _p(Xs), ns, _p(rad), ne, _ip(ec), _ip(ed), 0, None, None, None, None, None, None,
+           None)
+    if n < 0:
+        raise RuntimeError("near list failed rc=%d" % n)
+    out = dict(Xtrg_near=np.zeros(3 * n), Xn_trg_near=np.zeros(3 * n) if Xn is not None else None,
+               near_elem_cnt=np.zeros(ne, dtype=np.int64), near_elem_dsp=np.zeros(ne, dtype=np.int64),
+               near_scatter_index=np.zeros(n, dtype=np.int64), near_trg_cnt=np.zeros(nt, dtype=np.int64),
+               near_trg_dsp=np.zeros(nt, dtype=np.int64))
+    m = fn(_p(Xt), nt, _p(Xn), _p(Xs), ns, _p(rad), ne, _ip(ec), _ip(ed), n, _p(out["Xtrg_near"]),
+           _p(out["Xn_trg_near"]), _ip(out["near_elem_cnt"]), _ip(out["near_elem_dsp"]),
+           _ip(out["near_scatter_index"]), _ip(out["near_trg_cnt"]), _ip(out["near_trg_dsp"]))
+    if m != n:
+        raise RuntimeError("near list size changed between calls (%d, %d)" % (n, m))
+    return out
+
+
 class Oracle:
     """One build of the C restatement (liboracle.so, or a mutant build)."""
 
@@ -80,6 +102,8 @@ class Oracle:
         L.oracle_near_subtract.argtypes = [ctypes.c_int, ctypes.c_int, _L] + [_lp] * 9 + [_ip] + [_dp] * 6
         L.oracle_near_gemv.argtypes = [ctypes.c_int, ctypes.c_int, _L] + [_lp] * 8 + [_dp] * 3
         L.oracle_near_accumulate.argtypes = [_L, ctypes.c_int, _lp, _lp, _L, _lp, _dp, _dp]
+        L.oracle_near_list.argtypes = [_dp, _L, _dp, _dp, _L, _dp, _L, _lp, _lp, _L, _dp, _dp, _lp, _lp, _lp, _lp, _lp]
+        L.oracle_near_list.restype = _L
         self.lib = L
         self.path = path
 
@@ -186,6 +210,12 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle_near_accumulate failed rc=%d" % rc)
         return out
+
+    def near_list(self, Xtrg, Xsrc, src_radius, elem_nds_cnt, elem_nds_dsp, Xn_trg=None):
+        """BuildNearList (boundary_integral.txx:50-520), one process, brute force:
+        dict of Xtrg_near, Xn_trg_near, near_elem_cnt, near_elem_dsp,
+        near_scatter_index, near_trg_cnt, near_trg_dsp."""
+        return _near_list(self.lib.oracle_near_list, Xtrg, Xsrc, src_radius, elem_nds_cnt, elem_nds_dsp, Xn_trg)
 
     def rsqrt(self, x, mode=EXACT, digits=-1):
         x = _c(x)
@@ -306,3 +336,31 @@ class Reference:
         if self.lib.sctlref_rsqrt(digits, _p(x), x.size, _p(y)) != 0:
             raise ValueError("digits %r not instantiated" % digits)
         return y
+
+
+class ReferenceNearList:
+    """The reference library's own BuildNearList (compiled reference,
+    oracle/ref_near_shim.cpp)."""
+
+    def __init__(self):
+        path = os.path.join(REF_DIR, "libsctlref_near.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = ctypes.CDLL(path)
+        _lp = ctypes.POINTER(ctypes.c_long)
+        L.sctlref_build_near_list.argtypes = [_dp, _L, _dp, _dp, _L, _dp, _L, _lp, _lp, _L, _lp, _dp, _dp, _lp, _lp,
+                                              _lp, _lp, _lp]
+        self.lib = L
+
+    @staticmethod
+    def available():
+        return os.path.exists(os.path.join(REF_DIR, "libsctlref_near.so"))
+
+    def near_list(self, Xtrg, Xsrc, src_radius, elem_nds_cnt, elem_nds_dsp, Xn_trg=None):
+        def fn(xt, nt, xn, xs, ns, rad, ne, ec, ed, max_near, *outs):
+            n = ctypes.c_long()
+            rc = self.lib.sctlref_build_near_list(xt, nt, xn, xs, ns, rad, ne, ec, ed, max_near, ctypes.byref(n),
+                                                  *outs)
+            return n.value if rc in (0, 1) else -1
+        return _near_list(fn, Xtrg, Xsrc, src_radius, elem_nds_cnt, elem_nds_dsp, Xn_trg)
+

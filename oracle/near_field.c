@@ -193,3 +193,76 @@ int oracle_near_accumulate(long ntrg, int dof, const long* near_trg_cnt, const l
   free(data);
   return 0;
 }
+
+/* BuildNearList (boundary_integral.txx:50-520) on one process, by brute
+ * force over the output it defines: target t is near element e when some node
+ * s of e has dist2(s, t) < rad_s^2 (:390-401, dist2 summed over the axes as
+ * node_dist2, :72-79); pairs ordered by (element, target index) (:418-428);
+ * Xtrg_near / Xn_trg_near gathered per pair (:430-447); near_elem_cnt and
+ * its exclusive scan (:449-476); the pairs in target order (SortScatterIndex,
+ * :483 -- ties in element order here: the reference's std::sort on the key
+ * alone leaves them unspecified); near_trg_cnt, and near_trg_dsp = first
+ * target-major position for targets with pairs, 0 otherwise (:485-514).
+ * Returns the number of pairs; fills the outputs when max_near >= it. */
+long oracle_near_list(const double* Xtrg, long ntrg, const double* Xn_trg, const double* Xsrc, long nsrc,
+                      const double* rad, long nelem, const long* ecnt, const long* edsp, long max_near,
+                      double* Xtrg_near, double* Xn_trg_near, long* near_elem_cnt, long* near_elem_dsp,
+                      long* scatter, long* near_trg_cnt, long* near_trg_dsp) {
+  (void)nsrc;
+  long n = 0;
+  long* pair_trg = NULL;
+  for (int pass = 0; pass < 2; pass++) {
+    n = 0;
+    for (long e = 0; e < nelem; e++) {
+      const long cnt0 = n;
+      for (long t = 0; t < ntrg; t++) {
+        int near = 0;
+        for (long s = edsp[e]; s < edsp[e] + ecnt[e] && !near; s++) {
+          double dist2 = 0;
+          for (int k = 0; k < 3; k++) {
+            const double dX = Xsrc[s * 3 + k] - Xtrg[t * 3 + k];
+            dist2 += dX * dX;
+          }
+          near = ORACLE_MUTANT == 12 ? (dist2 <= rad[s] * rad[s]) : (dist2 < rad[s] * rad[s]);
+        }
+        if (near) {
+          if (pass == 1) {
+            for (int k = 0; k < 3; k++) {
+              Xtrg_near[n * 3 + k] = Xtrg[t * 3 + k];
+              if (Xn_trg && Xn_trg_near) Xn_trg_near[n * 3 + k] = Xn_trg[t * 3 + k];
+            }
+            pair_trg[n] = t;
+          }
+          n++;
+        }
+      }
+      if (pass == 1) {
+        near_elem_cnt[e] = n - cnt0;
+        near_elem_dsp[e] = cnt0;
+      }
+    }
+    if (pass == 0) {
+      if (max_near < n) return n;
+      pair_trg = (long*)malloc(sizeof(long) * (size_t)(n + 1));
+      if (!pair_trg) return -3;
+    }
+  }
+  /* target order, ties in pair (= element) order: a stable counting sort */
+  for (long t = 0; t < ntrg; t++) near_trg_cnt[t] = 0;
+  for (long i = 0; i < n; i++) near_trg_cnt[pair_trg[i]]++;
+  long* next = (long*)malloc(sizeof(long) * (size_t)(ntrg + 1));
+  if (!next) {
+    free(pair_trg);
+    return -3;
+  }
+  long run = 0;
+  for (long t = 0; t < ntrg; t++) {
+    next[t] = run;
+    near_trg_dsp[t] = (ORACLE_MUTANT == 13 || near_trg_cnt[t]) ? run : 0;
+    run += near_trg_cnt[t];
+  }
+  for (long i = 0; i < n; i++) scatter[next[pair_trg[i]]++] = i;
+  free(next);
+  free(pair_trg);
+  return n;
+}

@@ -42,7 +42,8 @@ SYMBOLS = [
     "sctl_amd_graph_interactions", "sctl_amd_graph_destroy", "sctl_amd_gen_normal", "sctl_amd_plan_get_shape",
     "sctl_amd_plan_set_shape", "sctl_amd_tune_db_path", "sctl_amd_tune_eval", "sctl_amd_plan_tune",
     "sctl_amd_near_blocks", "sctl_amd_plan_near_layout", "sctl_amd_plan_near_matrix", "sctl_amd_plan_near_gemv",
-    "sctl_amd_near_accumulate", "sctl_amd_dense_shape_get", "sctl_amd_dense_shape_set",
+    "sctl_amd_near_accumulate", "sctl_amd_dense_shape_get", "sctl_amd_dense_shape_set", "sctl_amd_near_list_build",
+    "sctl_amd_near_list_size", "sctl_amd_near_list_get", "sctl_amd_near_list_destroy",
 ]
 
 
@@ -106,6 +107,10 @@ def lib():
     L.sctl_amd_plan_near_matrix.argtypes = [_vp, i32, _vp, _vp, _vp, u32, _vp]
     L.sctl_amd_plan_near_gemv.argtypes = [_vp, _vp, _vp, _vp, u32, _vp]
     L.sctl_amd_near_accumulate.argtypes = [i64, i32, _vp, _vp, i64, _vp, i64, _vp, _vp, u32, _vp]
+    L.sctl_amd_near_list_build.argtypes = [ctypes.POINTER(_vp), _vp, i64, _vp, _vp, i64, _vp, i64, _vp, _vp, u32, _vp]
+    L.sctl_amd_near_list_size.argtypes = [_vp, _i64p]
+    L.sctl_amd_near_list_get.argtypes = [_vp] + [_vp] * 7 + [u32, _vp]
+    L.sctl_amd_near_list_destroy.argtypes = [_vp]
     L.sctl_amd_profile_enable.argtypes = [i32]
     L.sctl_amd_profile_read.argtypes = [_i64p, _i64p, _i64p, ctypes.POINTER(ctypes.c_double)]
     _lib = L
@@ -588,4 +593,41 @@ class NearField:
         U_near = np.zeros(self.nt_total * self.dof)
         self.plan.near_gemv(K_near, F, U_near)
         return near_accumulate(U, self.dof, near_trg_cnt, near_trg_dsp, scatter_index, U_near)
+
+
+def near_list(Xtrg, Xsrc, src_radius, elem_nds_cnt, elem_nds_dsp, Xn_trg=None, stream=None):
+    """BuildNearList (boundary_integral.txx:50-520) on the device, one process.
+
+    Host arrays in, host arrays out: dict of Xtrg_near, Xn_trg_near (None
+    without Xn_trg), near_elem_cnt, near_elem_dsp, near_scatter_index,
+    near_trg_cnt, near_trg_dsp.  With DeviceArray inputs (all of them) the
+    build runs on device memory and the results are still returned on the host."""
+    dev = isinstance(Xtrg, DeviceArray)
+    if dev:
+        xt, xs, rad, ec, ed, xn = Xtrg, Xsrc, src_radius, elem_nds_cnt, elem_nds_dsp, Xn_trg
+        nt, ns, ne = xt.shape[0] // 3, xs.shape[0] // 3, ec.shape[0]
+        flags = 0
+    else:
+        xt, xs, rad, xn = _host(Xtrg), _host(Xsrc), _host(src_radius), _host(Xn_trg)
+        ec = np.ascontiguousarray(elem_nds_cnt, dtype=np.int64)
+        ed = np.ascontiguousarray(elem_nds_dsp, dtype=np.int64)
+        nt, ns, ne = xt.size // 3, xs.size // 3, ec.size
+        flags = HOST_PTRS
+    h = _vp()
+    check(lib().sctl_amd_near_list_build(ctypes.byref(h), _ptr(xt), nt, _ptr(xn), _ptr(xs), ns, _ptr(rad), ne,
+                                         _ptr(ec), _ptr(ed), flags, stream))
+    try:
+        n = ctypes.c_int64()
+        check(lib().sctl_amd_near_list_size(h, ctypes.byref(n)))
+        n = n.value
+        out = dict(Xtrg_near=np.zeros(3 * n), Xn_trg_near=np.zeros(3 * n) if Xn_trg is not None else None,
+                   near_elem_cnt=np.zeros(ne, dtype=np.int64), near_elem_dsp=np.zeros(ne, dtype=np.int64),
+                   near_scatter_index=np.zeros(n, dtype=np.int64), near_trg_cnt=np.zeros(nt, dtype=np.int64),
+                   near_trg_dsp=np.zeros(nt, dtype=np.int64))
+        keys = ("Xtrg_near", "Xn_trg_near", "near_elem_cnt", "near_elem_dsp", "near_scatter_index", "near_trg_cnt",
+                "near_trg_dsp")
+        check(lib().sctl_amd_near_list_get(h, *[_ptr(out[k]) for k in keys], HOST_PTRS, stream))
+    finally:
+        lib().sctl_amd_near_list_destroy(h)
+    return out
 

@@ -105,6 +105,9 @@ int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, c
 // Morton ordering of points (30-bit keys over the points' bounding box);
 // writes a permutation of [0, n).  Implemented in sort.hip.
 int morton_order(const double* pts, int64_t n, uint32_t* perm, hipStream_t stream);
+// Axis-aligned bounding box of two point sets (either may be empty):
+// box6 (device) = {min x, min y, min z, max x, max y, max z}.  sort.hip.
+int bounding_box(const double* a, int64_t na, const double* b, int64_t nb, double* box6, hipStream_t stream);
 
 // Shape defaults and tuning-database lookup (tune.cpp).  `newton` is the
 // Newton count of the path (part of the tuning key).

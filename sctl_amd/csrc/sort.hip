@@ -107,6 +107,27 @@ __global__ void __launch_bounds__(BB_WG) k_morton_keys(const double* __restrict_
 
 }  // namespace
 
+int bounding_box(const double* a, int64_t na, const double* b, int64_t nb, double* box6, hipStream_t stream) {
+  const int64_t per_blk = 8192;
+  const int blk_a = (int)((na + per_blk - 1) / per_blk), blk_b = (int)((nb + per_blk - 1) / per_blk);
+  SCTL_CHECK_ARG(blk_a + blk_b > 0, "bounding box of no points");
+  Scratch part;
+  SCTL_TRY(part.alloc(sizeof(double) * 6 * (blk_a + blk_b), stream));
+  if (blk_a) {
+    hipLaunchKernelGGL(k_bbox_partial, dim3(blk_a), dim3(BB_WG), 0, stream, a, na, per_blk, part.as<double>());
+    SCTL_HIP_TRY(hipGetLastError());
+  }
+  if (blk_b) {
+    hipLaunchKernelGGL(k_bbox_partial, dim3(blk_b), dim3(BB_WG), 0, stream, b, nb, per_blk,
+                       part.as<double>() + 6 * blk_a);
+    SCTL_HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_WG), 0, stream, part.as<double>(), blk_a + blk_b);
+  SCTL_HIP_TRY(hipGetLastError());
+  SCTL_HIP_TRY(hipMemcpyAsync(box6, part.as<double>(), sizeof(double) * 6, hipMemcpyDeviceToDevice, stream));
+  return SCTL_AMD_SUCCESS;
+}
+
 int morton_order(const double* pts, int64_t n, uint32_t* perm, hipStream_t stream) {
   if (n <= 0) return SCTL_AMD_SUCCESS;
   const int64_t per_blk = 8192;

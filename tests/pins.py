@@ -30,6 +30,9 @@ Near field (oracle/near_field.c):
   near_gemv_numpy        the GEMV against numpy's matrix product
   near_accumulate_hand   ScatterForward + per-target sum, by hand and
                          against numpy.add.reduceat
+  near_list_golden       BuildNearList against the reference's own output
+                         (golden cases of tests/golden/make_golden_near_list.py)
+  near_list_hand         the strict radius test and the bookkeeping by hand
 """
 import math
 import os
@@ -415,7 +418,70 @@ def near_accumulate_hand(orc):
     return not bad, ", ".join(bad)
 
 
-NEAR_CHECKS = [near_partition_hand, near_layout_hand, near_subtract_identity, near_gemv_numpy, near_accumulate_hand]
+NEAR_LIST_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sctl_ref_near_list.npz")
+NEAR_LIST_CASES = ("torus", "radii", "edge", "cloud")
+
+
+def near_list_golden_case(name):
+    """(inputs, expected outputs) of one golden BuildNearList case."""
+    G = np.load(NEAR_LIST_GOLDEN, allow_pickle=False)
+    g = lambda k: G[name + "_" + k] if (name + "_" + k) in G.files else None
+    inp = dict(Xtrg=g("in_Xtrg"), Xsrc=g("in_Xsrc"), src_radius=g("in_rad"), elem_nds_cnt=g("in_cnt"),
+               elem_nds_dsp=g("in_dsp"), Xn_trg=g("in_Xn_trg"))
+    keys = ("Xtrg_near", "Xn_trg_near", "near_elem_cnt", "near_elem_dsp", "near_scatter_index", "near_trg_cnt",
+            "near_trg_dsp")
+    return inp, {k: g(k) for k in keys}
+
+
+def near_list_mismatches(got, want):
+    """Differences between two BuildNearList outputs; the order of one
+    target's pairs in near_scatter_index is left open (the reference sorts by
+    target alone, comm.txx:1393-1416)."""
+    bad = []
+    for k in ("Xtrg_near", "Xn_trg_near", "near_elem_cnt", "near_elem_dsp", "near_trg_cnt", "near_trg_dsp"):
+        a, b = got.get(k), want.get(k)
+        if (a is None) != (b is None) or (a is not None and not np.array_equal(a, b)):
+            bad.append(k)
+    sa, sb = got["near_scatter_index"], want["near_scatter_index"]
+    if sa.shape != sb.shape:
+        bad.append("near_scatter_index size")
+    elif "near_trg_cnt" not in bad:
+        for t in np.nonzero(want["near_trg_cnt"])[0]:
+            d, c = want["near_trg_dsp"][t], want["near_trg_cnt"][t]
+            if sorted(sa[d:d + c]) != sorted(sb[d:d + c]):
+                bad.append("near_scatter_index (target %d)" % t)
+                break
+    return bad
+
+
+def near_list_golden(orc):
+    """BuildNearList (boundary_integral.txx:50-520) against the reference's
+    own output on the golden cases (tests/golden/make_golden_near_list.py)."""
+    bad = []
+    for name in NEAR_LIST_CASES:
+        inp, want = near_list_golden_case(name)
+        got = orc.near_list(inp["Xtrg"], inp["Xsrc"], inp["src_radius"], inp["elem_nds_cnt"], inp["elem_nds_dsp"],
+                            Xn_trg=inp["Xn_trg"])
+        bad += ["%s: %s" % (name, b) for b in near_list_mismatches(got, want)]
+    return not bad, ", ".join(bad)
+
+
+def near_list_hand(orc):
+    """The strict test |x_t - x_s|^2 < r^2 (:395): a target at distance
+    exactly r is not near; and the pair / target bookkeeping worked by hand."""
+    # nodes: element 0 = {origin, r = 1}, element 1 = {(3,0,0), r = 2.6}; targets t0 = (1,0,0) (distance
+    # exactly 1 from the origin: not near element 0), t1 = (0.5,0,0), t2 = (9,9,9).  Pairs by element:
+    # (e0,t1), (e1,t0), (e1,t1); by target: t0 -> pair 1, t1 -> pairs 0, 2.
+    r = orc.near_list([1.0, 0, 0, 0.5, 0, 0, 9, 9, 9], [0.0, 0, 0, 3, 0, 0], [1.0, 2.6], [1, 1], [0, 1])
+    ok = (list(r["near_elem_cnt"]) == [1, 2] and list(r["near_elem_dsp"]) == [0, 1]
+          and np.array_equal(r["Xtrg_near"], [0.5, 0, 0, 1.0, 0, 0, 0.5, 0, 0])
+          and list(r["near_trg_cnt"]) == [1, 2, 0] and list(r["near_trg_dsp"]) == [0, 1, 0]
+          and list(r["near_scatter_index"]) == [1, 0, 2])
+    return ok, "" if ok else repr(r)
+
+
+NEAR_CHECKS = [near_partition_hand, near_layout_hand, near_subtract_identity, near_gemv_numpy, near_accumulate_hand,
+               near_list_golden, near_list_hand]
 
 CHECKS = [golden_exact, golden_avx512, golden_rsqrt, rsqrt_bounds, closed_forms, long_double, eval_vs_matrix,
           surface_identities, linearity] + NEAR_CHECKS
