@@ -250,6 +250,58 @@ def near_ops(kernel, B, stream, reps):
     return res
 
 
+def _near_list_ms(xt, xs, radius, cnt_h, dsp_h, stream, reps):
+    """(median ms, pairs) of device near-list builds on device-resident inputs
+    (the build sizes its output, so it synchronises: host clock)."""
+    ne = cnt_h.size
+    d = [S.DeviceArray.from_host(a) for a in (xt, xs)]
+    rad = S.DeviceArray.from_host(np.full(xs.size // 3, radius))
+    cnt = S.DeviceArray.from_host(cnt_h.astype(np.int64), dtype=np.int64)
+    dsp = S.DeviceArray.from_host(dsp_h.astype(np.int64), dtype=np.int64)
+    L = S.lib()
+    nt, ns = xt.size // 3, xs.size // 3
+    times, n_near = [], 0
+    for it in range(reps + 1):
+        h = ctypes.c_void_p()
+        S.synchronize(stream)
+        t0 = time.perf_counter()
+        S.check(L.sctl_amd_near_list_build(ctypes.byref(h), d[0].ptr, nt, None, d[1].ptr, ns, rad.ptr, ne, cnt.ptr,
+                                           dsp.ptr, 0, stream))
+        t1 = time.perf_counter()
+        n = ctypes.c_int64()
+        S.check(L.sctl_amd_near_list_size(h, ctypes.byref(n)))
+        n_near = n.value
+        L.sctl_amd_near_list_destroy(h)
+        if it:
+            times.append((t1 - t0) * 1e3)
+    return float(np.median(times)), n_near
+
+
+def near_list_build(B, radius, stream, reps=3, cpu_elements=256, cpu=True):
+    """BuildNearList on the device for one config's surface: the elements are
+    the blocks' source patches, the targets the blocks' targets, every node's
+    radius `radius`.  Beside it, the reference's own BuildNearList (compiled
+    from its sources, on the host cores) on a bounded sample -- the first
+    `cpu_elements` elements against all targets -- and the device on the
+    same sample."""
+    ms, n_near = _near_list_ms(B["r_trg"], B["r_src"], radius, B["src_cnt"], B["src_dsp"], stream, reps)
+    nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
+    res = {"elements": int(B["src_cnt"].size), "nodes": ns, "targets": nt, "radius": radius,
+           "near_pairs": n_near, "ms": round(ms, 3), "M_pairs_per_s": round(n_near / ms / 1e3, 1)}
+    import oracle
+    if cpu and oracle.ReferenceNearList.available():
+        cnt, dsp = B["src_cnt"][:cpu_elements], B["src_dsp"][:cpu_elements]
+        nss = int(dsp[-1] + cnt[-1])
+        xs = B["r_src"][:3 * nss]
+        sec, pairs = oracle.ReferenceNearList().build_seconds(B["r_trg"], xs, np.full(nss, radius), cnt, dsp)
+        gms, gpairs = _near_list_ms(B["r_trg"], xs, radius, cnt, dsp, stream, reps)
+        res["cpu_reference_sample"] = {
+            "sample": "first %d elements (%d nodes) x all %d targets" % (cpu_elements, nss, nt),
+            "kind": "reference", "cores": os.cpu_count(), "s": round(sec, 3), "pairs": pairs,
+            "device_ms_same_sample": round(gms, 3), "device_pairs_same_sample": gpairs}
+    return res
+
+
 def hbm_copy_reference(stream, gib=4):
     """Achievable HBM bandwidth for context: the runtime's device-to-device
     copy of `gib` GiB (read + write bytes / time) and its fill of the same
@@ -307,6 +359,8 @@ def other_configs(stream, quick=False):
     out["hbm_copy_4GiB"] = hbm_copy_reference(stream)
     log("config 4 blocks: near-field matrices (KNearSubtract, GEMV), Laplace DxU")
     out["c4_near_matrix_laplace_dxu_4096"] = near_ops(S.LAPLACE3D_DXU, B, stream, reps=3)
+    log("config 4 surface: near lists (BuildNearList)")
+    out["c4_near_list_4096"] = near_list_build(B, 0.05, stream)
     log("config 5: batched Stokes DxU, 2048 torus blocks, 100 applies in one hipGraph")
     B = W.torus_blocks(64, 32, seed=0, k0=3)
     out["c5_stokes_dxu_graph_2048x100"] = graph_pair(S.STOKES3D_DXU, B, stream, nsteps=20 if quick else 100)

@@ -356,6 +356,17 @@ class ReferenceNearList:
     def available():
         return os.path.exists(os.path.join(REF_DIR, "libsctlref_near.so"))
 
+    def build_seconds(self, Xtrg, Xsrc, src_radius, elem_nds_cnt, elem_nds_dsp):
+        """(seconds, pairs) of one BuildNearList call (the sizing call builds the whole list)."""
+        import time
+        Xt, Xs, rad = _c(Xtrg), _c(Xsrc), _c(src_radius)
+        ec, ed = _i(elem_nds_cnt), _i(elem_nds_dsp)
+        n = ctypes.c_long()
+        t = time.perf_counter()
+        self.lib.sctlref_build_near_list(_p(Xt), Xt.size // 3, None, _p(Xs), Xs.size // 3, _p(rad), ec.size, _ip(ec),
+                                         _ip(ed), 0, ctypes.byref(n), None, None, None, None, None, None, None)
+        return time.perf_counter() - t, n.value
+
     def near_list(self, Xtrg, Xsrc, src_radius, elem_nds_cnt, elem_nds_dsp, Xn_trg=None):
         def fn(xt, nt, xn, xs, ns, rad, ne, ec, ed, max_near, *outs):
             n = ctypes.c_long()

@@ -15,14 +15,19 @@
 // The reference finds candidates through a Morton tree of the nodes.  Here:
 //   1. targets get 63-bit Morton keys over the reference's bounding cube
 //      (all points, grown by 5 % per side, :104-154) and are radix sorted;
-//   2. one wave per element takes the bounding box of its nodes grown by the
-//      largest radius, picks the finest Morton depth at which that box spans
-//      at most 64 cells, finds each cell's targets by binary search in the
-//      sorted keys, drops candidates outside the grown box and tests the
-//      rest exactly against the element's nodes (first hit wins);
+//   2. one wave per element (its nodes in the lanes) takes the bounding box
+//      of its nodes grown by the largest radius, picks the finest Morton
+//      depth at which that box spans at most 64 cells, finds each cell's
+//      targets by binary search in the sorted keys, queues the candidates
+//      inside the grown box and tests them 64 at a time exactly against the
+//      element's nodes (one ballot per candidate and 64 nodes, each node
+//      chunk loaded once per batch, first hit wins);
 //   3. a count pass sizes the output, a fill pass writes (element, target)
 //      keys, which are radix sorted; a second sort by (target, pair) gives
 //      the scatter index.
+// Each element is one workgroup of 4 waves that take turns over its
+// 64-candidate groups (counts are kept per wave, so the fill is
+// deterministic); elements are launched largest first.
 // Every target within a node's radius lies in the grown box, hence in one of
 // the cells (the cell range is computed with the same quantisation as the
 // target keys), so the list is exact.
@@ -72,20 +77,16 @@ __device__ __forceinline__ uint64_t interleave(uint64_t x, uint64_t y, uint64_t 
   return spread21(x) | (spread21(y) << 1) | (spread21(z) << 2);
 }
 
-// The reference's bounding cube (:104-154): origin x0 and edge length.
-struct Cube {
-  double x0[3], len;
-};
-__device__ __forceinline__ Cube make_cube(const double* __restrict__ box) {
-  Cube c;
+// The reference's bounding cube (:104-154), computed once so that the key
+// and the search kernels quantise with the same bits: cube = {origin x, y,
+// z, 2^21 / edge length}.
+__global__ void k_nl_cube(const double* box, double* cube) {  // in place: cube may alias box
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double bl = 0;
-#pragma unroll
   for (int k = 0; k < 3; k++) bl = fmax(bl, box[3 + k] - box[k]);
   const double L = bl > 0 ? bl : 1.0;
-#pragma unroll
-  for (int k = 0; k < 3; k++) c.x0[k] = box[k] - 0.05 * L;
-  c.len = 1.1 * L;
-  return c;
+  for (int k = 0; k < 3; k++) cube[k] = box[k] - 0.05 * L;
+  cube[3] = 2097152.0 / (1.1 * L);
 }
 __device__ __forceinline__ uint32_t quant(double x, double x0, double scale) {
   double u = (x - x0) * scale;
@@ -98,10 +99,9 @@ k_nl_keys(const double* __restrict__ xt, int64_t nt, const double* __restrict__ 
           uint32_t* __restrict__ idx) {
   const int64_t i = (int64_t)blockIdx.x * NL_WG + threadIdx.x;
   if (i >= nt) return;
-  const Cube c = make_cube(box);
-  const double scale = 2097152.0 / c.len;
-  keys[i] = interleave(quant(xt[i * 3 + 0], c.x0[0], scale), quant(xt[i * 3 + 1], c.x0[1], scale),
-                       quant(xt[i * 3 + 2], c.x0[2], scale));
+  const double scale = box[3];
+  keys[i] = interleave(quant(xt[i * 3 + 0], box[0], scale), quant(xt[i * 3 + 1], box[1], scale),
+                       quant(xt[i * 3 + 2], box[2], scale));
   idx[i] = (uint32_t)i;
 }
 
@@ -126,32 +126,82 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+
+// |d|^2 rounded as node_dist2 sums it (boundary_integral.txx:72-79), without
+// fused multiply-adds, so the radius test matches the reference's bit for bit.
+__device__ __forceinline__ double dist2(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  return (dx * dx + dy * dy) + dz * dz;
+}
+
+constexpr int NODE_REGS = 4;  // node chunks of 64 kept in registers (larger elements load the rest per batch)
+constexpr int QCAP = 128;     // per-wave queue of candidates inside the grown box
+constexpr int NWAVE = NL_WG / 64;  // waves sharing one element's candidates
+
+struct Cand {
+  double x, y, z;
+  uint64_t t;
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One wave per element.  FILL = false: counts[e] = number of near targets;
 // FILL = true: writes (e << 32 | t) at out[offsets[e] ...].
+// Lanes hold the element's nodes (64 per chunk, the first NODE_REGS chunks in
+// registers).  Candidate targets are fetched 64 at a time; those inside the
+// grown box are appended to a per-wave LDS queue, and every 64 queued
+// candidates are tested as a batch: node chunk by node chunk (each chunk
+// loaded once per batch), each undecided candidate broadcast from LDS and
+// tested against the chunk's 64 nodes with one ballot, decided at its first
+// hit.
 template <bool FILL>
 __global__ void __launch_bounds__(NL_WG)
-k_nl_elements(int64_t nelem, const int64_t* __restrict__ ecnt, const int64_t* __restrict__ edsp, int64_t nsrc,
-              const double* __restrict__ xs, const double* __restrict__ rad, int64_t nt,
-              const double* __restrict__ xt, const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ sidx,
-              const double* __restrict__ box, int64_t* __restrict__ counts, const int64_t* __restrict__ offsets,
-              uint64_t* __restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  const int64_t nwaves = (int64_t)gridDim.x * (NL_WG / 64);
-  const Cube cube = make_cube(box);
-  const double scale = 2097152.0 / cube.len;
-  for (int64_t e = (int64_t)blockIdx.x * (NL_WG / 64) + (threadIdx.x >> 6); e < nelem; e += nwaves) {
+k_nl_elements(const uint32_t* __restrict__ eorder, const int64_t* __restrict__ ecnt,
+              const int64_t* __restrict__ edsp, int64_t nsrc, const double* __restrict__ xs,
+              const double* __restrict__ rad, int64_t nt, const double* __restrict__ xt,
+              const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ sidx, const double* __restrict__ box,
+              int64_t* __restrict__ counts, const int64_t* __restrict__ offsets, uint64_t* __restrict__ out) {
+  __shared__ Cand queue_all[NWAVE][QCAP];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  Cand* queue = queue_all[w];
+  const double x0[3] = {box[0], box[1], box[2]}, scale = box[3];
+  {
+    const int64_t e = eorder[blockIdx.x];
     const int64_t s0 = edsp[e], sc = ecnt[e];
     int64_t count = 0;
     if (s0 >= 0 && sc > 0 && s0 <= nsrc && sc <= nsrc - s0 && nt > 0) {
+      const int nch = (int)((sc + 63) / 64);
+      // nodes of the first chunks in registers; r2 = -1 marks a lane without a node
+      double nx[NODE_REGS], ny[NODE_REGS], nz[NODE_REGS], nr2[NODE_REGS];
       double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, rmax = 0;
-      for (int64_t s = lane; s < sc; s += 64) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const double v = xs[(s0 + s) * 3 + k];
-          lo[k] = fmin(lo[k], v);
-          hi[k] = fmax(hi[k], v);
-        }
-        rmax = fmax(rmax, fabs(rad[s0 + s]));
+      for (int c = 0; c < NODE_REGS; c++) {
+        nx[c] = ny[c] = nz[c] = 0.0;
+        nr2[c] = -1.0;
+      }
+      for (int64_t s = lane; s < sc; s += 64) {
+        const double x = xs[(s0 + s) * 3 + 0], y = xs[(s0 + s) * 3 + 1], z = xs[(s0 + s) * 3 + 2];
+        const double r = rad[s0 + s];
+        lo[0] = fmin(lo[0], x);
+        lo[1] = fmin(lo[1], y);
+        lo[2] = fmin(lo[2], z);
+        hi[0] = fmax(hi[0], x);
+        hi[1] = fmax(hi[1], y);
+        hi[2] = fmax(hi[2], z);
+        rmax = fmax(rmax, fabs(r));
+        const int c = (int)(s >> 6);
+#pragma unroll
+        for (int k = 0; k < NODE_REGS; k++)
+          if (c == k) {
+            nx[k] = x;
+            ny[k] = y;
+            nz[k] = z;
+            nr2[k] = r * r;
+          }
       }
 #pragma unroll
       for (int k = 0; k < 3; k++) {
@@ -159,6 +209,54 @@ k_nl_elements(int64_t nelem, const int64_t* __restrict__ ecnt, const int64_t* __
         hi[k] = wave_max(hi[k]);
       }
       rmax = wave_max(rmax);
+
+      // Tests queue entries [0, n) (n <= 64); returns the mask of near ones
+      // and, with FILL, writes them.
+      auto batch = [&](int n) {
+        uint64_t rem = n >= 64 ? ~0ULL : ((1ULL << n) - 1);
+        uint64_t nearm = 0;
+        for (int k = 0; k < nch && rem; k++) {
+          double bx = 0, by = 0, bz = 0, br2 = -1.0;
+          if (k < NODE_REGS) {
+#pragma unroll
+            for (int kk = 0; kk < NODE_REGS; kk++)
+              if (kk == k) {
+                bx = nx[kk];
+                by = ny[kk];
+                bz = nz[kk];
+                br2 = nr2[kk];
+              }
+          } else {
+            const int64_t s = (int64_t)k * 64 + lane;
+            if (s < sc) {
+              bx = xs[(s0 + s) * 3 + 0];
+              by = xs[(s0 + s) * 3 + 1];
+              bz = xs[(s0 + s) * 3 + 2];
+              const double r = rad[s0 + s];
+              br2 = r * r;
+            }
+          }
+          uint64_t todo = rem;
+          while (todo) {
+            const int j = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const double cx = queue[j].x, cy = queue[j].y, cz = queue[j].z;
+            if (__builtin_amdgcn_ballot_w64(dist2(bx - cx, by - cy, bz - cz) < br2) != 0) {
+              nearm |= 1ULL << j;
+              rem &= ~(1ULL << j);
+            }
+          }
+        }
+        if constexpr (FILL) {
+          if ((nearm >> lane) & 1ULL) {
+            const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(nearm >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)nearm, 0));
+            out[offsets[e * NWAVE + w] + count + below] = ((uint64_t)e << 32) | queue[lane].t;
+          }
+        }
+        count += __builtin_popcountll(nearm);
+      };
+
       if (rmax > 0) {
         // Grown box, with a margin above the rounding of the two additions.
         const double grow = rmax * (1.0 + 1e-9);
@@ -168,8 +266,8 @@ k_nl_elements(int64_t nelem, const int64_t* __restrict__ ecnt, const int64_t* __
         for (int k = 0; k < 3; k++) {
           glo[k] = lo[k] - grow - 1e-14 * fabs(lo[k]);
           ghi[k] = hi[k] + grow + 1e-14 * fabs(hi[k]);
-          qlo[k] = quant(glo[k], cube.x0[k], scale);
-          qhi[k] = quant(ghi[k], cube.x0[k], scale);
+          qlo[k] = quant(glo[k], x0[k], scale);
+          qhi[k] = quant(ghi[k], x0[k], scale);
         }
         int shift = 0;
         for (int d = MAX_DEPTH; d >= 0; d--) {
@@ -180,6 +278,8 @@ k_nl_elements(int64_t nelem, const int64_t* __restrict__ ecnt, const int64_t* __
           if (cells <= MAX_CELLS) break;
         }
         const uint64_t span = 1ULL << (3 * shift);
+        int qn = 0;        // queued candidates (< 64 between batches)
+        int64_t group = 0; // 64-candidate groups fetched so far; wave w takes every NWAVE-th
         for (uint32_t bz = qlo[2] >> shift; bz <= (qhi[2] >> shift); bz++)
           for (uint32_t by = qlo[1] >> shift; by <= (qhi[1] >> shift); by++)
             for (uint32_t bx = qlo[0] >> shift; bx <= (qhi[0] >> shift); bx++) {
@@ -187,39 +287,81 @@ k_nl_elements(int64_t nelem, const int64_t* __restrict__ ecnt, const int64_t* __
               const int64_t i0 = lower_bound_u64(skeys, nt, klo);
               const int64_t i1 = lower_bound_u64(skeys, nt, klo + span);
               for (int64_t c0 = i0; c0 < i1; c0 += 64) {
+                if ((group++ % NWAVE) != w) continue;
                 const int64_t c = c0 + lane;
-                bool near = false;
-                uint32_t t = 0;
+                bool inside = false;
+                double mx = 0, my = 0, mz = 0;
+                uint64_t mt = 0;
                 if (c < i1) {
-                  t = sidx[c];
-                  const double px = xt[(int64_t)t * 3 + 0], py = xt[(int64_t)t * 3 + 1], pz = xt[(int64_t)t * 3 + 2];
-                  if (px >= glo[0] && px <= ghi[0] && py >= glo[1] && py <= ghi[1] && pz >= glo[2] && pz <= ghi[2]) {
-                    // dist2 rounded as node_dist2 sums it (:72-79), no fused multiply-add
-                    for (int64_t s = 0; s < sc && !near; s++) {
-                      const double dx = xs[(s0 + s) * 3 + 0] - px, dy = xs[(s0 + s) * 3 + 1] - py;
-                      const double dz = xs[(s0 + s) * 3 + 2] - pz, r = rad[s0 + s];
-                      const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-                      near = d2 < __dmul_rn(r, r);
-                    }
-                  }
+                  mt = sidx[c];
+                  mx = xt[mt * 3 + 0];
+                  my = xt[mt * 3 + 1];
+                  mz = xt[mt * 3 + 2];
+                  inside = mx >= glo[0] && mx <= ghi[0] && my >= glo[1] && my <= ghi[1] && mz >= glo[2] &&
+                           mz <= ghi[2];
                 }
-                const uint64_t ballot = __builtin_amdgcn_ballot_w64(near);
-                if constexpr (FILL) {
-                  if (near) {
-                    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0));
-                    out[offsets[e] + count + below] = ((uint64_t)e << 32) | t;
-                  }
+                const uint64_t in = __builtin_amdgcn_ballot_w64(inside);
+                if (inside) {
+                  const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(in >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)in, 0));
+                  Cand& q = queue[qn + below];  // qn < 64, so qn + below < QCAP
+                  q.x = mx;
+                  q.y = my;
+                  q.z = mz;
+                  q.t = mt;
                 }
-                count += __builtin_popcountll(ballot);
+                qn += __builtin_popcountll(in);
+                wave_sync();
+                if (qn >= 64) {
+                  batch(64);
+                  wave_sync();
+                  const int from = 64 + (lane < qn - 64 ? lane : 0);
+                  const double rx = queue[from].x, ry = queue[from].y, rz = queue[from].z;
+                  const uint64_t rt = queue[from].t;
+                  wave_sync();
+                  if (lane < qn - 64) {
+                    queue[lane].x = rx;
+                    queue[lane].y = ry;
+                    queue[lane].z = rz;
+                    queue[lane].t = rt;
+                  }
+                  qn -= 64;
+                  wave_sync();
+                }
               }
             }
+        if (qn > 0) batch(qn);
+        wave_sync();
       }
     }
     if constexpr (!FILL) {
-      if (lane == 0) counts[e] = count;
+      if (lane == 0) counts[e * NWAVE + w] = count;
     }
   }
+}
+
+// near_elem_cnt[e] = sum of its waves' counts, near_elem_dsp[e] = the
+// exclusive scan at its first wave.
+__global__ void __launch_bounds__(NL_WG)
+k_nl_elem_totals(int64_t nelem, const int64_t* __restrict__ cnt_w, const int64_t* __restrict__ dsp_w,
+                 int64_t* __restrict__ ecnt_out, int64_t* __restrict__ edsp_out) {
+  const int64_t e = (int64_t)blockIdx.x * NL_WG + threadIdx.x;
+  if (e >= nelem) return;
+  int64_t c = 0;
+  for (int k = 0; k < NWAVE; k++) c += cnt_w[e * NWAVE + k];
+  ecnt_out[e] = c;
+  edsp_out[e] = dsp_w[e * NWAVE];
+}
+
+// Elements ordered by node count, largest first (the order only schedules).
+__global__ void __launch_bounds__(NL_WG)
+k_nl_order_keys(int64_t nelem, const int64_t* __restrict__ ecnt, uint32_t* __restrict__ keys,
+                uint32_t* __restrict__ idx) {
+  const int64_t e = (int64_t)blockIdx.x * NL_WG + threadIdx.x;
+  if (e >= nelem) return;
+  const int64_t c = ecnt[e];
+  keys[e] = 0xffffffffu - (uint32_t)(c < 0 ? 0 : (c > 0xffffffffLL ? 0xffffffffLL : c));
+  idx[e] = (uint32_t)e;
 }
 
 __global__ void __launch_bounds__(NL_WG) k_nl_zero(int64_t* __restrict__ a, int64_t n) {
@@ -295,6 +437,8 @@ int build_list(sctl_amd_near_list_s* L, const double* xt, int64_t nt, const doub
   Scratch box, keys_in, keys, idx_in, idx, tmp;
   SCTL_TRY(box.alloc(sizeof(double) * 6, s));
   SCTL_TRY(bounding_box(xt, nt, xs, ns, box.as<double>(), s));
+  hipLaunchKernelGGL(k_nl_cube, dim3(1), dim3(64), 0, s, box.as<double>(), box.as<double>());
+  SCTL_HIP_TRY(hipGetLastError());
   SCTL_TRY(keys_in.alloc(sizeof(uint64_t) * nt, s));
   SCTL_TRY(keys.alloc(sizeof(uint64_t) * nt, s));
   SCTL_TRY(idx_in.alloc(sizeof(uint32_t) * nt, s));
@@ -312,22 +456,44 @@ int build_list(sctl_amd_near_list_s* L, const double* xt, int64_t nt, const doub
   keys_in.release();
   idx_in.release();
 
-  DeviceInfo dev;
-  SCTL_TRY(device_info(&dev));
-  const int64_t waves = std::min<int64_t>(nelem, (int64_t)dev.num_cus * 32);
-  const dim3 egrid((unsigned)((waves + NL_WG / 64 - 1) / (NL_WG / 64)));
-  hipLaunchKernelGGL(k_nl_elements<false>, egrid, dim3(NL_WG), 0, s, nelem, ecnt, edsp, ns, xs, rad, nt, xt,
-                     keys.as<uint64_t>(), idx.as<uint32_t>(), box.as<double>(), L->d_ecnt, nullptr, nullptr);
+  // elements largest first (scheduling only)
+  Scratch okeys_in, okeys, oidx_in, oidx;
+  SCTL_TRY(okeys_in.alloc(sizeof(uint32_t) * nelem, s));
+  SCTL_TRY(okeys.alloc(sizeof(uint32_t) * nelem, s));
+  SCTL_TRY(oidx_in.alloc(sizeof(uint32_t) * nelem, s));
+  SCTL_TRY(oidx.alloc(sizeof(uint32_t) * nelem, s));
+  hipLaunchKernelGGL(k_nl_order_keys, dim3(grid_of(nelem)), dim3(NL_WG), 0, s, nelem, ecnt, okeys_in.as<uint32_t>(),
+                     oidx_in.as<uint32_t>());
   SCTL_HIP_TRY(hipGetLastError());
-  SCTL_HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, L->d_ecnt, L->d_edsp, (int64_t)0, (size_t)nelem,
-                                       rocprim::plus<int64_t>(), s));
+  SCTL_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, okeys_in.as<uint32_t>(), okeys.as<uint32_t>(),
+                                         oidx_in.as<uint32_t>(), oidx.as<uint32_t>(), (size_t)nelem, 0, 32, s));
   SCTL_TRY(tmp.alloc(bytes, s));
-  SCTL_HIP_TRY(rocprim::exclusive_scan(tmp.as<void>(), bytes, L->d_ecnt, L->d_edsp, (int64_t)0, (size_t)nelem,
-                                       rocprim::plus<int64_t>(), s));
+  SCTL_HIP_TRY(rocprim::radix_sort_pairs(tmp.as<void>(), bytes, okeys_in.as<uint32_t>(), okeys.as<uint32_t>(),
+                                         oidx_in.as<uint32_t>(), oidx.as<uint32_t>(), (size_t)nelem, 0, 32, s));
   tmp.release();
+
+  // count pass: per (element, wave)
+  const int64_t nw = nelem * NWAVE;
+  Scratch cnt_w, dsp_w;
+  SCTL_TRY(cnt_w.alloc(sizeof(int64_t) * nw, s));
+  SCTL_TRY(dsp_w.alloc(sizeof(int64_t) * nw, s));
+  const dim3 egrid((unsigned)nelem);
+  hipLaunchKernelGGL(k_nl_elements<false>, egrid, dim3(NL_WG), 0, s, oidx.as<uint32_t>(), ecnt, edsp, ns, xs, rad,
+                     nt, xt, keys.as<uint64_t>(), idx.as<uint32_t>(), box.as<double>(), cnt_w.as<int64_t>(), nullptr,
+                     nullptr);
+  SCTL_HIP_TRY(hipGetLastError());
+  SCTL_HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, cnt_w.as<int64_t>(), dsp_w.as<int64_t>(), (int64_t)0,
+                                       (size_t)nw, rocprim::plus<int64_t>(), s));
+  SCTL_TRY(tmp.alloc(bytes, s));
+  SCTL_HIP_TRY(rocprim::exclusive_scan(tmp.as<void>(), bytes, cnt_w.as<int64_t>(), dsp_w.as<int64_t>(), (int64_t)0,
+                                       (size_t)nw, rocprim::plus<int64_t>(), s));
+  tmp.release();
+  hipLaunchKernelGGL(k_nl_elem_totals, dim3(grid_of(nelem)), dim3(NL_WG), 0, s, nelem, cnt_w.as<int64_t>(),
+                     dsp_w.as<int64_t>(), L->d_ecnt, L->d_edsp);
+  SCTL_HIP_TRY(hipGetLastError());
   int64_t last[2] = {0, 0};
-  SCTL_HIP_TRY(hipMemcpyAsync(&last[0], L->d_ecnt + nelem - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  SCTL_HIP_TRY(hipMemcpyAsync(&last[1], L->d_edsp + nelem - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  SCTL_HIP_TRY(hipMemcpyAsync(&last[0], cnt_w.as<int64_t>() + nw - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  SCTL_HIP_TRY(hipMemcpyAsync(&last[1], dsp_w.as<int64_t>() + nw - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   SCTL_HIP_TRY(hipStreamSynchronize(s));
   const int64_t n = last[0] + last[1];
   L->n_near = n;
@@ -337,8 +503,8 @@ int build_list(sctl_amd_near_list_s* L, const double* xt, int64_t nt, const doub
   Scratch pairs_in, pairs, key2_in, key2;
   SCTL_TRY(pairs_in.alloc(sizeof(uint64_t) * n, s));
   SCTL_TRY(pairs.alloc(sizeof(uint64_t) * n, s));
-  hipLaunchKernelGGL(k_nl_elements<true>, egrid, dim3(NL_WG), 0, s, nelem, ecnt, edsp, ns, xs, rad, nt, xt,
-                     keys.as<uint64_t>(), idx.as<uint32_t>(), box.as<double>(), nullptr, L->d_edsp,
+  hipLaunchKernelGGL(k_nl_elements<true>, egrid, dim3(NL_WG), 0, s, oidx.as<uint32_t>(), ecnt, edsp, ns, xs, rad,
+                     nt, xt, keys.as<uint64_t>(), idx.as<uint32_t>(), box.as<double>(), nullptr, dsp_w.as<int64_t>(),
                      pairs_in.as<uint64_t>());
   SCTL_HIP_TRY(hipGetLastError());
   keys.release();
