@@ -46,15 +46,18 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X published FP64 vector peak
 FLOP_PER_INT = {0: 11, 1: 19, 2: 29, 3: 30}  # algorithmic flop per interaction
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_eval_mixed_laplace_fxu_sphere_1M.json")
+TRAFFIC_FILES = {False: os.path.join(ROOT, "profiles", "traffic_k_eval_mixed_laplace_fxu_sphere_1M.json"),
+                 True: os.path.join(ROOT, "profiles", "traffic_k_eval_mixed_mfma_laplace_fxu_sphere_1M.json")}
 
 
-def measured_traffic(n):
-    """HBM bytes per launch of the headline kernel from the committed FETCH_SIZE /
-    WRITE_SIZE counter runs (copy-calibrated), for the full-size workload only."""
-    if n != 1 << 20 or not os.path.exists(TRAFFIC_FILE):
+def measured_traffic(n, matrix_cores):
+    """HBM bytes per launch of the headline kernel (VALU or matrix-core far
+    path) from the committed FETCH_SIZE / WRITE_SIZE counter runs
+    (copy-calibrated), for the full-size workload only."""
+    path = TRAFFIC_FILES[bool(matrix_cores)]
+    if n != 1 << 20 or not os.path.exists(path):
         return None
-    with open(TRAFFIC_FILE) as fh:
+    with open(path) as fh:
         return float(json.load(fh)["traffic_bytes"])
 
 
@@ -385,6 +388,8 @@ def main():
                     help="with --headline-only: which path to run")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--near-only", action="store_true",
+                    help="only the near-field operator and near-list lines of config 4 (for kernel traces)")
     ap.add_argument("--shape", default=None, metavar="TPL,CAP,MC",
                     help="pin the mixed headline launch shape (targets per lane, workgroups-per-CU cap, "
                          "matrix cores 0/1) instead of tuning it")
@@ -394,6 +399,16 @@ def main():
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.near_only:
+        stream = ctypes.c_void_p()
+        S.check(S.lib().sctl_amd_stream_create(ctypes.byref(stream)))
+        log("config 4 blocks: near-field matrices and near lists")
+        B = W.torus_blocks(64, 64, seed=0, k0=1)
+        out = {"c4_near_matrix_laplace_dxu_4096": near_ops(S.LAPLACE3D_DXU, B, stream, reps=3),
+               "c4_near_list_4096": near_list_build(B, 0.05, stream, cpu=False)}
+        print(json.dumps({"near_only": True, "configs": out}), flush=True)
+        return
 
     N = args.n
     K = S.LAPLACE3D_FXU
@@ -426,7 +441,9 @@ def main():
                                            stream=stream)
             tuned[name] = {"targets_per_lane": tpl, "wg_per_cu_cap": cap, "matrix_cores": mc, "ms": round(ms, 3)}
         out.upload(np.zeros(N))
-    log("mixed precision: %d warmup + %d timed steps" % (args.warmup, args.steps))
+    hs = S.dense_shape(K, N, N, precision=S.MIXED)
+    headline_kernel = "k_eval_mixed%s<Laplace3D_FxU, TPL %d>" % ("_mfma" if hs[2] else "", hs[0])
+    log("mixed precision: %d warmup + %d timed steps (%s)" % (args.warmup, args.steps, headline_kernel))
     hp = S.FP64 if (args.headline_only and args.headline_precision == "fp64") else S.MIXED
     mx = timed_evals(K, dev, out, hp, -1, args.steps, args.warmup, stream)
     if args.headline_only:
@@ -503,8 +520,8 @@ def main():
         "speedup_vs_fp64_digits14": round(v_mx / v_64b, 3),
         "pct_fp32_valu_peak": round(100 * step_tflops / PEAK_FP32_TFLOPS, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(N),
-                     "kernel": "k_eval_mixed<Laplace3D_FxU>", "mean_launch_ms": round(kern_ms, 3),
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(N, hs[2]),
+                     "kernel": headline_kernel, "mean_launch_ms": round(kern_ms, 3),
                      "flop_per_interaction": FLOP_PER_INT[K]},
         "autotuned_shapes": tuned or None,
         "cpu_baseline": cpu,

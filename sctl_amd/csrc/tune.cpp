@@ -164,8 +164,23 @@ DenseShape pick_dense_shape(int ker, int precision, int newton, int64_t nt, int6
     s.mfma = v.size() >= 4 ? (int)v[3] : 0;
     return s;
   }
-  // Enough target blocks to fill the device with two targets per lane,
-  // otherwise one target per lane and the source range split over the grid.
+  // Mixed Laplace kernels on large target sets: the matrix-core far path with
+  // one target per lane (the fastest shape measured on 2^20 x 2^20).
+  if (precision == SCTL_AMD_MIXED && mfma_supported(ker) && nt >= (1 << 16)) {
+    s.tpl = 1;
+    s.occ = 0;
+    s.mfma = 1;
+    return s;
+  }
+  // FP64 on 2^20 or more targets: four targets per lane (the tuner's choice on
+  // 2^20 x 2^20).  Otherwise enough target blocks to fill the device with two
+  // targets per lane, else one target per lane and the source range split
+  // over the grid.
+  if (precision == SCTL_AMD_FP64 && nt >= (1 << 20)) {
+    s.tpl = 4;
+    s.occ = 0;
+    return s;
+  }
   s.tpl = nt >= 256LL * 2 * 2048 ? 2 : 1;
   s.occ = 0;
   return s;
