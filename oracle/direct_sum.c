@@ -46,7 +46,10 @@
  * compiles each mutant and checks that the pinning checks reject it:
  *   1 Stokes FxU without its rank-one term      2 Laplace DxU with r = x_s - x_t
  *   3 KernelMatrix written transposed           4 Stokes DxU scaled by 1/(4 pi)
- *   5 one Newton step too few (AVX-512 mode)    6 Eval skips the last source */
+ *   5 one Newton step too few (AVX-512 mode)    6 Eval skips the last source
+ *   14 BiotSavart FxU with the sign of its third component flipped
+ *   15 Laplace Fxd2U with 2 for the 3 of its r r^T / r^5 term
+ * (7-13: oracle/near_field.c) */
 #ifndef ORACLE_MUTANT
 #define ORACLE_MUTANT 0
 #endif
@@ -55,13 +58,27 @@
 #include <immintrin.h>
 #endif
 
-enum { LAP_FXU = 0, LAP_DXU = 1, STK_FXU = 2, STK_DXU = 3, NKER = 4 };
+enum {
+  LAP_FXU = 0, LAP_DXU = 1, STK_FXU = 2, STK_DXU = 3,
+  LAP_FXDU = 4, LAP_DXDU = 5, LAP_FXD2U = 6, STK_FXT = 7, STK_FSXU = 8, STK_FXUP = 9, BS_FXU = 10, BS_FXDU = 11,
+  NKER = 12
+};
+#define KMAX0 4 /* largest KDIM0 (Stokes3D_FSxU) */
+#define KMAX1 9 /* largest KDIM1 (Laplace3D_Fxd2U, Stokes3D_FxT, BiotSavart3D_FxdU) */
 enum { ORACLE_EXACT = 0, ORACLE_AVX512 = 1, ORACLE_LDOUBLE = 2 };
 
-static const int KD0[NKER] = {1, 1, 3, 3};
-static const int KD1[NKER] = {1, 1, 3, 3};
-static const int NDIM[NKER] = {0, 3, 0, 3};
-static const int FLOPS[NKER] = {6, 14, 23, 26}; /* kernel_functions.hpp:37,66,130,185 */
+/* uKerMatrix dimensions and FLOPS(), kernel_functions.hpp:32-79 (Laplace FxU,
+ * DxU), 81-123 (FxdU), 125-218 (Stokes FxU, DxU), 220-263 (FxT), 264-324
+ * (FSxU), 325-365 (FxUP), 366-391 (Fxd2U), 392-415 (DxdU), 416-450
+ * (BiotSavart FxU), 451-489 (BiotSavart FxdU) */
+static const int KD0[NKER] = {1, 1, 3, 3, 1, 1, 1, 3, 4, 3, 3, 3};
+static const int KD1[NKER] = {1, 1, 3, 3, 3, 3, 9, 9, 3, 4, 3, 9};
+static const int NDIM[NKER] = {0, 3, 0, 3, 0, 3, 0, 0, 0, 0, 0, 0};
+static const int FLOPS[NKER] = {6, 14, 23, 26, 11, 22, 25, 39, 26, 26, 15, 51};
+/* FUSED_APPLY (kernel_functions.hpp:101,165,204,245,310,350,436): Eval takes
+ * the fused apply; the others go through uKerMatrix + FMA
+ * (generic-kernel.txx:107-121). */
+static const int FUSED[NKER] = {0, 0, 1, 1, 1, 0, 0, 1, 1, 1, 1, 0};
 
 /* const_pi<double>(), math_utils.hpp:43 */
 static double pi_d(void) {
@@ -73,22 +90,32 @@ static long double pi_ld(void) {
          (long double)7089564414062235240ull / (long double)(1ull << 62) / (long double)(1ull << 63);
 }
 
-/* uKerScaleFactor<double>(), kernel_functions.hpp:40,69,133,188 */
+/* uKerScaleFactor<double>(), kernel_functions.hpp:40,69,89,133,188,228,272,333,
+ * 374,400,424,459 */
 double oracle_scale(int ker) {
   switch (ker) {
     case LAP_FXU: return 1 / (4 * pi_d());
     case LAP_DXU: return 1 / (4 * pi_d());
     case STK_FXU: return 1 / (8 * pi_d());
     case STK_DXU: return (ORACLE_MUTANT == 4 ? 1 : 3) / (4 * pi_d());
+    case LAP_FXDU: return -1 / (4 * pi_d());
+    case LAP_DXDU: return 1 / (4 * pi_d());
+    case LAP_FXD2U: return 1 / (4 * pi_d());
+    case STK_FXT: return -3 / (4 * pi_d());
+    case STK_FSXU: return 1 / (8 * pi_d());
+    case STK_FXUP: return 1 / (8 * pi_d());
+    case BS_FXU: return 1 / (4 * pi_d());
+    case BS_FXDU: return -1 / (4 * pi_d());
   }
   return 0;
 }
 static long double scale_ld(int ker) {
   switch (ker) {
-    case LAP_FXU: return 1 / (4 * pi_ld());
-    case LAP_DXU: return 1 / (4 * pi_ld());
-    case STK_FXU: return 1 / (8 * pi_ld());
+    case LAP_FXU: case LAP_DXU: case LAP_DXDU: case LAP_FXD2U: case BS_FXU: return 1 / (4 * pi_ld());
+    case STK_FXU: case STK_FSXU: case STK_FXUP: return 1 / (8 * pi_ld());
     case STK_DXU: return 3 / (4 * pi_ld());
+    case LAP_FXDU: case BS_FXDU: return -1 / (4 * pi_ld());
+    case STK_FXT: return -3 / (4 * pi_ld());
   }
   return 0;
 }
@@ -146,11 +173,42 @@ static inline void interact_exact(int ker, double* vt, const double* r, const do
       const double fdotr_rdotn_rinv5 = (f[0] * r[0] + f[1] * r[1] + f[2] * r[2]) * rdotn_rinv5;
       for (int i = 0; i < 3; i++) vt[i] += r[i] * fdotr_rdotn_rinv5;
     } break;
+    case LAP_FXDU: { /* kernel_functions.hpp:101-109 */
+      const double rinv3 = rinv * rinv * rinv;
+      const double f_rinv3 = f[0] * rinv3;
+      for (int i = 0; i < 3; i++) vt[i] += r[i] * f_rinv3;
+    } break;
+    case STK_FXT: { /* :245-261 */
+      const double rinv2 = rinv * rinv;
+      const double rinv5 = rinv2 * rinv2 * rinv;
+      const double fdotr_rinv5 = (f[0] * r[0] + f[1] * r[1] + f[2] * r[2]) * rinv5;
+      for (int i = 0; i < 3; i++) {
+        const double ri = r[i] * fdotr_rinv5;
+        for (int j = 0; j < 3; j++) vt[i * 3 + j] += ri * r[j];
+      }
+    } break;
+    case STK_FSXU: { /* :310-321 */
+      const double rinv3 = rinv * rinv * rinv;
+      const double t = (f[0] * r[0] + f[1] * r[1] + f[2] * r[2] + f[3]) * rinv3;
+      for (int i = 0; i < 3; i++) vt[i] += f[i] * rinv + r[i] * t;
+    } break;
+    case STK_FXUP: { /* :350-362 */
+      const double rinv3 = rinv * rinv * rinv;
+      const double fdotr_rinv3 = (f[0] * r[0] + f[1] * r[1] + f[2] * r[2]) * rinv3;
+      for (int i = 0; i < 3; i++) vt[i] += f[i] * rinv + r[i] * fdotr_rinv3;
+      vt[3] += fdotr_rinv3;
+    } break;
+    case BS_FXU: { /* :436-448 */
+      const double rinv3 = rinv * rinv * rinv;
+      vt[0] += (f[1] * r[2] - r[1] * f[2]) * rinv3;
+      vt[1] += (f[2] * r[0] - r[2] * f[0]) * rinv3;
+      vt[2] += (f[0] * r[1] - r[0] * f[1]) * rinv3 * (ORACLE_MUTANT == 14 ? -1.0 : 1.0);
+    } break;
   }
 }
 
 /* uKerMatrix for the scalar build: u[k0][k1] (not scaled). */
-static inline void matrix_exact(int ker, double u[3][3], const double* r, const double* n) {
+static inline void matrix_exact(int ker, double u[KMAX0][KMAX1], const double* r, const double* n) {
   const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
   const double rinv = rinv_exact(r2);
   switch (ker) {
@@ -172,7 +230,78 @@ static inline void matrix_exact(int ker, double u[3][3], const double* r, const 
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) u[i][j] = r[i] * r[j] * rdotn_rinv5;
     } break;
+    case LAP_FXDU: { /* kernel_functions.hpp:92-99 */
+      const double rinv3 = rinv * rinv * rinv;
+      for (int i = 0; i < 3; i++) u[0][i] = r[i] * rinv3;
+    } break;
+    case LAP_DXDU: { /* :403-413 */
+      const double rinv2 = rinv * rinv;
+      const double rinv3 = rinv * rinv2;
+      const double rdotn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
+      const double t5 = 3.0 * rdotn * rinv3 * rinv2;
+      for (int i = 0; i < 3; i++) u[0][i] = n[i] * rinv3 - r[i] * t5;
+    } break;
+    case LAP_FXD2U: { /* :377-389 */
+      const double rinv2 = rinv * rinv;
+      const double rinv3 = rinv * rinv2;
+      const double t5 = (ORACLE_MUTANT == 15 ? 2.0 : 3.0) * rinv3 * rinv2;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) u[0][i * 3 + j] = t5 * r[i] * r[j] - (i == j ? rinv3 : 0.0);
+    } break;
+    case STK_FXT: { /* :231-243 */
+      const double rinv2 = rinv * rinv;
+      const double rinv5 = rinv2 * rinv2 * rinv;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          for (int k = 0; k < 3; k++) u[i][j * 3 + k] = r[i] * r[j] * r[k] * rinv5;
+    } break;
+    case STK_FSXU: { /* :275-288 */
+      const double rinv3 = rinv * rinv * rinv;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) u[i][j] = (i == j ? rinv : 0.0) + r[i] * r[j] * rinv3;
+      for (int j = 0; j < 3; j++) u[3][j] = r[j] * rinv3;
+    } break;
+    case STK_FXUP: { /* :336-348 */
+      const double rinv3 = rinv * rinv * rinv;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) u[i][j] = (i == j ? rinv : 0.0) + r[i] * r[j] * rinv3;
+      for (int i = 0; i < 3; i++) u[i][3] = r[i] * rinv3;
+    } break;
+    case BS_FXU: { /* :427-434 */
+      const double rinv3 = rinv * rinv * rinv;
+      u[0][0] = 0.0;           u[0][1] = -r[2] * rinv3; u[0][2] = r[1] * rinv3;
+      u[1][0] = r[2] * rinv3;  u[1][1] = 0.0;           u[1][2] = -r[0] * rinv3;
+      u[2][0] = -r[1] * rinv3; u[2][1] = r[0] * rinv3;  u[2][2] = 0.0;
+    } break;
+    case BS_FXDU: { /* :462-486: the gradient of the Biot-Savart field, column by column */
+      const double rinv2 = rinv * rinv;
+      const double rinv3 = rinv * rinv2;
+      const double t5 = 3.0 * rinv3 * rinv2;
+      u[0][0] = 0.0;                         u[1][0] = t5 * r[2] * r[0];          u[2][0] = -t5 * r[1] * r[0];
+      u[0][1] = 0.0;                         u[1][1] = t5 * r[2] * r[1];          u[2][1] = rinv3 - t5 * r[1] * r[1];
+      u[0][2] = 0.0;                         u[1][2] = -rinv3 + t5 * r[2] * r[2]; u[2][2] = -t5 * r[1] * r[2];
+      u[0][3] = -t5 * r[2] * r[0];           u[1][3] = 0.0;                       u[2][3] = -rinv3 + t5 * r[0] * r[0];
+      u[0][4] = -t5 * r[2] * r[1];           u[1][4] = 0.0;                       u[2][4] = t5 * r[0] * r[1];
+      u[0][5] = rinv3 - t5 * r[2] * r[2];    u[1][5] = 0.0;                       u[2][5] = t5 * r[0] * r[2];
+      u[0][6] = t5 * r[1] * r[0];            u[1][6] = rinv3 - t5 * r[0] * r[0];  u[2][6] = 0.0;
+      u[0][7] = -rinv3 + t5 * r[1] * r[1];   u[1][7] = -t5 * r[0] * r[1];         u[2][7] = 0.0;
+      u[0][8] = t5 * r[1] * r[2];            u[1][8] = -t5 * r[0] * r[2];         u[2][8] = 0.0;
+    } break;
   }
+}
+
+/* One interaction of Eval in the reference's branch for the kernel: the fused
+ * apply, or uKerMatrix followed by FMA(U, vs, vt) (a separate multiply and
+ * add in the scalar build), generic-kernel.txx:107-121. */
+static inline void eval_pair_exact(int ker, double* vt, const double* r, const double* n, const double* f) {
+  if (FUSED[ker] || ker == LAP_FXU || ker == LAP_DXU) {
+    interact_exact(ker, vt, r, n, f);
+    return;
+  }
+  double u[KMAX0][KMAX1];
+  matrix_exact(ker, u, r, n);
+  for (int k0 = 0; k0 < KD0[ker]; k0++)
+    for (int k1 = 0; k1 < KD1[ker]; k1++) vt[k1] = u[k0][k1] * f[k0] + vt[k1];
 }
 
 static void eval_exact(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
@@ -182,11 +311,11 @@ static void eval_exact(int ker, const double* r_trg, long nt, const double* r_sr
   const double zero3[3] = {0, 0, 0};
 #pragma omp parallel for schedule(static) num_threads(nthreads)
   for (long t = 0; t < nt; t++) {
-    double vt[3] = {0, 0, 0};
+    double vt[KMAX1] = {0};
     for (long s = 0; s < ns - (ORACLE_MUTANT == 6 ? 1 : 0); s++) {
       const double r[3] = {r_trg[t * 3 + 0] - r_src[s * 3 + 0], r_trg[t * 3 + 1] - r_src[s * 3 + 1],
                            r_trg[t * 3 + 2] - r_src[s * 3 + 2]};
-      interact_exact(ker, vt, r, ND ? n_src + s * 3 : zero3, v_src + s * K0);
+      eval_pair_exact(ker, vt, r, ND ? n_src + s * 3 : zero3, v_src + s * K0);
     }
     for (int k = 0; k < K1; k++) v_trg[t * K1 + k] += vt[k] * sf;
   }
@@ -199,9 +328,9 @@ static void eval_ldouble(int ker, const double* r_trg, long nt, const double* r_
   const long double sf = scale_ld(ker);
 #pragma omp parallel for schedule(static) num_threads(nthreads)
   for (long t = 0; t < nt; t++) {
-    long double vt[3] = {0, 0, 0};
+    long double vt[KMAX1] = {0};
     for (long s = 0; s < ns; s++) {
-      long double r[3], n[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+      long double r[3], n[3] = {0, 0, 0}, f[KMAX0] = {0};
       for (int k = 0; k < 3; k++) r[k] = (long double)r_trg[t * 3 + k] - (long double)r_src[s * 3 + k];
       if (ND) for (int k = 0; k < 3; k++) n[k] = n_src[s * 3 + k];
       for (int k = 0; k < K0; k++) f[k] = v_src[s * K0 + k];
@@ -220,6 +349,48 @@ static void eval_ldouble(int ker, const double* r_trg, long nt, const double* r_
           const long double fr = f[0] * r[0] + f[1] * r[1] + f[2] * r[2];
           const long double rn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
           for (int i = 0; i < 3; i++) vt[i] += r[i] * fr * rn * rinv3 * rinv * rinv;
+        } break;
+        default: { /* the other kernels: their matrices applied to f, in long double */
+          long double u[KMAX0][KMAX1] = {{0}};
+          const long double rinv5 = rinv3 * rinv * rinv, rn = r[0] * n[0] + r[1] * n[1] + r[2] * n[2];
+          switch (ker) {
+            case LAP_FXDU: for (int i = 0; i < 3; i++) u[0][i] = r[i] * rinv3; break;
+            case LAP_DXDU: for (int i = 0; i < 3; i++) u[0][i] = n[i] * rinv3 - 3 * rn * r[i] * rinv5; break;
+            case LAP_FXD2U:
+              for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) u[0][i * 3 + j] = 3 * r[i] * r[j] * rinv5 - (i == j ? rinv3 : 0);
+              break;
+            case STK_FXT:
+              for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                  for (int k = 0; k < 3; k++) u[i][j * 3 + k] = r[i] * r[j] * r[k] * rinv5;
+              break;
+            case STK_FSXU: case STK_FXUP:
+              for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) u[i][j] = (i == j ? rinv : 0) + r[i] * r[j] * rinv3;
+              for (int j = 0; j < 3; j++) {
+                if (ker == STK_FSXU) u[3][j] = r[j] * rinv3;
+                else u[j][3] = r[j] * rinv3;
+              }
+              break;
+            case BS_FXU: /* f x r / r^3 as a matrix */
+              u[1][0] = r[2] * rinv3; u[2][0] = -r[1] * rinv3; u[0][1] = -r[2] * rinv3;
+              u[2][1] = r[0] * rinv3; u[0][2] = r[1] * rinv3; u[1][2] = -r[0] * rinv3;
+              break;
+            case BS_FXDU: /* d/dx_j of (f x r / r^3)_i, column i*3+j */
+              for (int k = 0; k < 3; k++)       /* source component */
+                for (int i = 0; i < 3; i++)     /* field component */
+                  for (int j = 0; j < 3; j++) { /* derivative direction */
+                    /* (e_k x r)_i = eps_{k m i} r_m; d/dr_j of eps_{kmi} r_m / r^3 */
+                    const int m = 3 - k - i;
+                    if (k == i) continue;
+                    const long double eps = ((k + 1) % 3 == m) ? 1 : -1; /* eps_{k m i} */
+                    u[k][i * 3 + j] = -eps * ((m == j ? rinv3 : 0) - 3 * r[m] * r[j] * rinv5);
+                  }
+              break;
+          }
+          for (int k0 = 0; k0 < K0; k0++)
+            for (int k1 = 0; k1 < K1; k1++) vt[k1] += u[k0][k1] * f[k0];
         } break;
       }
     }
@@ -348,6 +519,7 @@ int oracle_eval(int ker, int mode, int digits, const double* r_trg, long nt, con
     case ORACLE_LDOUBLE: eval_ldouble(ker, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads); return 0;
     case ORACLE_AVX512:
 #if defined(__x86_64__)
+      if (ker > STK_DXU) return -1; /* the AVX-512 restatement covers the four hot-path kernels */
       if (!oracle_have_avx512()) return -2;
       eval_avx512(ker, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads);
       return 0;
@@ -371,7 +543,7 @@ int oracle_kernel_matrix(int ker, const double* r_trg, long nt, const double* r_
     for (long t = 0; t < nt; t++) {
       const double r[3] = {r_trg[t * 3 + 0] - r_src[s * 3 + 0], r_trg[t * 3 + 1] - r_src[s * 3 + 1],
                            r_trg[t * 3 + 2] - r_src[s * 3 + 2]};
-      double u[3][3];
+      double u[KMAX0][KMAX1];
       matrix_exact(ker, u, r, ND ? n_src + s * 3 : zero3);
       for (int k0 = 0; k0 < K0; k0++)
         for (int k1 = 0; k1 < K1; k1++) {

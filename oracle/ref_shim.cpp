@@ -8,7 +8,7 @@
 //   GenericKernel<uKernel>::Eval<Real,omp>(..., digits, self)   generic-kernel.txx:72-100
 //   GenericKernel<uKernel>::KernelMatrix<Real,false,digits>     generic-kernel.txx:213-329
 //   approx_rsqrt<digits>(x, mask)                               vec.txx:379-382
-// for the four micro-kernels of kernel_functions.hpp:32-79,125-218.
+// for the real-valued micro-kernels of kernel_functions.hpp:32-489.
 #include <cstring>
 #include <string>
 
@@ -77,62 +77,64 @@ template <Integer digits> void rsqrt_digits(const double* x, long n, double* y) 
   }
 }
 
+// Calls f(GenericKernel object, kernel_impl tag) for a kernel id:
+//   0 Laplace3D_FxU  1 Laplace3D_DxU  2 Stokes3D_FxU  3 Stokes3D_DxU  4 Laplace3D_FxdU
+//   5 Laplace3D_DxdU 6 Laplace3D_Fxd2U 7 Stokes3D_FxT 8 Stokes3D_FSxU 9 Stokes3D_FxUP
+//   10 BiotSavart3D_FxU 11 BiotSavart3D_FxdU    (kernel_functions.hpp:669-684)
+template <class F> int with_kernel(int kernel, F&& f) {
+  namespace ki = sctl::kernel_impl;
+  switch (kernel) {
+    case 0: f(sctl::Laplace3D_FxU(), ki::Laplace3D_FxU()); return 0;
+    case 1: f(sctl::Laplace3D_DxU(), ki::Laplace3D_DxU()); return 0;
+    case 2: f(sctl::Stokes3D_FxU(), ki::Stokes3D_FxU()); return 0;
+    case 3: f(sctl::Stokes3D_DxU(), ki::Stokes3D_DxU()); return 0;
+    case 4: f(sctl::Laplace3D_FxdU(), ki::Laplace3D_FxdU()); return 0;
+    case 5: f(sctl::Laplace3D_DxdU(), ki::Laplace3D_DxdU()); return 0;
+    case 6: f(sctl::Laplace3D_Fxd2U(), ki::Laplace3D_Fxd2U()); return 0;
+    case 7: f(sctl::Stokes3D_FxT(), ki::Stokes3D_FxT()); return 0;
+    case 8: f(sctl::Stokes3D_FSxU(), ki::Stokes3D_FSxU()); return 0;
+    case 9: f(sctl::Stokes3D_FxUP(), ki::Stokes3D_FxUP()); return 0;
+    case 10: f(sctl::BiotSavart3D_FxU(), ki::BiotSavart3D_FxU()); return 0;
+    case 11: f(sctl::BiotSavart3D_FxdU(), ki::BiotSavart3D_FxdU()); return 0;
+  }
+  return -1;
+}
+
 }  // namespace
 
 extern "C" {
 
-// kernel ids: 0 Laplace3D_FxU, 1 Laplace3D_DxU, 2 Stokes3D_FxU, 3 Stokes3D_DxU
 int sctlref_info(int kernel, int* dims, double* scale, char* name, int name_len) {
-  auto fill = [&](auto ker, double sf) {
+  return with_kernel(kernel, [&](auto ker, auto impl) {
     using K = decltype(ker);
     dims[0] = K::CoordDim(); dims[1] = K::SrcDim(); dims[2] = K::TrgDim(); dims[3] = K::NormalDim();
     dims[4] = (int)K::FLOPS();
-    *scale = sf;
+    *scale = decltype(impl)::template uKerScaleFactor<double>();
     std::strncpy(name, ker.Name().c_str(), name_len);
-    return 0;
-  };
-  switch (kernel) {
-    case 0: return fill(sctl::Laplace3D_FxU(), sctl::kernel_impl::Laplace3D_FxU::uKerScaleFactor<double>());
-    case 1: return fill(sctl::Laplace3D_DxU(), sctl::kernel_impl::Laplace3D_DxU::uKerScaleFactor<double>());
-    case 2: return fill(sctl::Stokes3D_FxU(), sctl::kernel_impl::Stokes3D_FxU::uKerScaleFactor<double>());
-    case 3: return fill(sctl::Stokes3D_DxU(), sctl::kernel_impl::Stokes3D_DxU::uKerScaleFactor<double>());
-  }
-  return -1;
+  });
 }
 
 int sctlref_veclen(void) { return (int)sctl::DefaultVecLen<double>(); }
 
 int sctlref_eval(int kernel, int digits, int omp, const double* r_trg, long nt, const double* r_src,
                  const double* n_src, const double* v_src, long ns, double* v_trg) {
-  switch (kernel) {
-    case 0: eval_one<sctl::Laplace3D_FxU, double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
-    case 1: eval_one<sctl::Laplace3D_DxU, double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
-    case 2: eval_one<sctl::Stokes3D_FxU, double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
-    case 3: eval_one<sctl::Stokes3D_DxU, double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
-  }
-  return -1;
+  return with_kernel(kernel, [&](auto ker, auto) {
+    eval_one<decltype(ker), double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg);
+  });
 }
 
 int sctlref_eval_f32(int kernel, int digits, int omp, const float* r_trg, long nt, const float* r_src,
                      const float* n_src, const float* v_src, long ns, float* v_trg) {
-  switch (kernel) {
-    case 0: eval_one<sctl::Laplace3D_FxU, float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
-    case 1: eval_one<sctl::Laplace3D_DxU, float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
-    case 2: eval_one<sctl::Stokes3D_FxU, float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
-    case 3: eval_one<sctl::Stokes3D_DxU, float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg); return 0;
-  }
-  return -1;
+  return with_kernel(kernel, [&](auto ker, auto) {
+    eval_one<decltype(ker), float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg);
+  });
 }
 
 int sctlref_kernel_matrix(int kernel, int digits, const double* r_trg, long nt, const double* r_src,
                           const double* n_src, long ns, double* M) {
-  switch (kernel) {
-    case 0: kmat_one<sctl::Laplace3D_FxU>(digits, r_trg, nt, r_src, n_src, ns, M); return 0;
-    case 1: kmat_one<sctl::Laplace3D_DxU>(digits, r_trg, nt, r_src, n_src, ns, M); return 0;
-    case 2: kmat_one<sctl::Stokes3D_FxU>(digits, r_trg, nt, r_src, n_src, ns, M); return 0;
-    case 3: kmat_one<sctl::Stokes3D_DxU>(digits, r_trg, nt, r_src, n_src, ns, M); return 0;
-  }
-  return -1;
+  return with_kernel(kernel, [&](auto ker, auto) {
+    kmat_one<decltype(ker)>(digits, r_trg, nt, r_src, n_src, ns, M);
+  });
 }
 
 int sctlref_rsqrt(int digits, const double* x, long n, double* y) {

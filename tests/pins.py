@@ -23,6 +23,11 @@ Pins, with what each rests on:
                     -f inside, 0 outside; Stokes FxU(f) = (2/3) f inside
   linearity         Eval(a f + b g) == a Eval(f) + b Eval(g), superposition
                     of two sources (test-generic-kernel.cpp:89-103)
+  golden_exact_more the other real-valued micro-kernels (ids 4..11) bit-exact
+                    against the reference's scalar build
+  derivative_identities  those kernels tied to the four pinned ones: gradients
+                    by central differences, FxUP / FSxU / FxT / BiotSavart as
+                    combinations of Stokes FxU, DxU and Laplace FxdU
 Near field (oracle/near_field.c):
   near_partition_hand    the block partition worked by hand
   near_layout_hand       K_near_cnt / K_near_dsp worked by hand
@@ -47,6 +52,15 @@ PI = math.pi
 
 def golden():
     return np.load(GOLDEN, allow_pickle=False)
+
+
+GOLDEN_MORE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sctl_ref_more_kernels.npz")
+MORE_KERNELS = range(4, 12)   # Laplace FxdU, DxdU, Fxd2U, Stokes FxT, FSxU, FxUP, BiotSavart FxU, FxdU
+ALL_KERNELS = range(12)
+
+
+def golden_more():
+    return np.load(GOLDEN_MORE, allow_pickle=False)
 
 
 def _nrm(x, ref):
@@ -87,6 +101,88 @@ def golden_exact(orc):
             ref = G["k%d_%s_scalar" % (k, tag)]
             if M.shape != ref.shape or not np.array_equal(M, ref):
                 bad.append("k%d %s" % (k, tag))
+    return not bad, ", ".join(bad)
+
+
+def golden_exact_more(orc):
+    """The other real-valued micro-kernels bit-exact against the reference's
+    scalar build (tests/golden/make_golden_more_kernels.py)."""
+    G = golden_more()
+    bad = []
+    for k in MORE_KERNELS:
+        _, K0, K1, ND, _ = orc.dims(k)
+        Xt, Xs, Xn, Vs = small_inputs(K0, ND)
+        if not np.array_equal(orc.eval(k, Xt, Xs, Xn, Vs), G["k%d_small_eval_scalar" % k]):
+            bad.append("k%d small eval" % k)
+        if not np.array_equal(orc.kernel_matrix(k, Xt, Xs, Xn).ravel(), G["k%d_small_kmat_scalar" % k].ravel()):
+            bad.append("k%d small kmat" % k)
+        xt, xs, n, v = (G["k%d_rand_%s" % (k, a)] for a in ("xt", "xs", "n", "v"))
+        if not np.array_equal(orc.eval(k, xt, xs, n, v), G["k%d_rand_eval_scalar" % k]):
+            bad.append("k%d rand eval" % k)
+        if _nrm(orc.eval(k, xt, xs, n, v), G["k%d_rand_eval_v4" % k]) > 5e-15:
+            bad.append("k%d rand vs AVX-512 build" % k)
+        vt = orc.eval(k, G["k%d_veclen_xt" % k], xs, n, v, v_trg=G["k%d_veclen_vt0" % k])
+        if not np.array_equal(vt, G["k%d_veclen_eval_scalar" % k]):
+            bad.append("k%d veclen accumulate" % k)
+        xt11, xs13, n13 = (G["k%d_kmat_%s" % (k, a)] for a in ("xt", "xs", "n"))
+        for tag, a, b, c in (("kmat", xt11, xs13, n13), ("kmat1t", xt11[:3], xs13, n13),
+                             ("kmat1s", xt11, xs13[:3], n13[:3])):
+            M = orc.kernel_matrix(k, a, b, c)
+            ref = G["k%d_%s_scalar" % (k, tag)]
+            if M.shape != ref.shape or not np.array_equal(M, ref):
+                bad.append("k%d %s" % (k, tag))
+    return not bad, ", ".join(bad)
+
+
+def derivative_identities(orc):
+    """The other micro-kernels tied to the four pinned ones: Laplace FxdU =
+    grad FxU, DxdU = grad DxU, Fxd2U = grad FxdU, BiotSavart FxdU = grad
+    BiotSavart FxU (central differences, h = 1e-5, targets >= 0.5 away);
+    Stokes FxUP = [Stokes FxU, -1/2 sum_k FxdU(f_k)_k]; FSxU(f, g) =
+    Stokes FxU(f) - 1/2 FxdU(g); Stokes FxT contracted with a normal shared by
+    all sources = -Stokes DxU; BiotSavart FxU = f x (r / (4 pi r^3)) written
+    with FxdU (kernel_functions.hpp:81-489)."""
+    rng = np.random.default_rng(17)
+    ns, nt = 40, 25
+    xs = rng.random(ns * 3)
+    xt = (rng.random((nt, 3)) + 1.5).ravel()
+    n1 = np.tile(rng.standard_normal(3), ns)
+    q = rng.uniform(-1, 1, ns)
+    f3 = rng.uniform(-1, 1, ns * 3)
+    f4 = rng.uniform(-1, 1, ns * 4)
+    E = lambda k, dens, nrm=None, pts=xt: orc.eval(k, pts, xs, nrm if nrm is not None else n1, dens)
+    h = 1e-5
+    bad = []
+
+    def grad(k, dens, K1):
+        g = np.zeros((nt, K1, 3))
+        for j in range(3):
+            d = np.zeros((nt, 3))
+            d[:, j] = h
+            g[:, :, j] = (E(k, dens, pts=(xt.reshape(-1, 3) + d).ravel()) -
+                          E(k, dens, pts=(xt.reshape(-1, 3) - d).ravel())).reshape(nt, K1) / (2 * h)
+        return g
+
+    checks = [("FxdU = grad FxU", E(4, q).reshape(nt, 3), grad(0, q, 1)[:, 0, :], 1e-8),
+              ("DxdU = grad DxU", E(5, q).reshape(nt, 3), grad(1, q, 1)[:, 0, :], 1e-8),
+              ("Fxd2U = grad FxdU", E(6, q).reshape(nt, 3, 3), grad(4, q, 3), 1e-8),
+              ("BiotSavart FxdU = grad FxU", E(11, f3).reshape(nt, 3, 3), grad(10, f3, 3), 1e-8)]
+    fr = f3.reshape(ns, 3)
+    up = E(9, f3).reshape(nt, 4)
+    p = -0.5 * sum(E(4, fr[:, k].copy()).reshape(nt, 3)[:, k] for k in range(3))
+    checks.append(("FxUP velocity", up[:, :3], E(2, f3).reshape(nt, 3), 1e-13))
+    checks.append(("FxUP pressure", up[:, 3], p, 1e-13))
+    f4r = f4.reshape(ns, 4)
+    checks.append(("FSxU", E(8, f4), E(2, f4r[:, :3].ravel().copy()) - 0.5 * E(4, f4r[:, 3].copy()), 1e-13))
+    T = E(7, f3).reshape(nt, 3, 3)
+    checks.append(("FxT . n = -DxU", np.einsum("tik,k->ti", T, n1[:3]), -E(3, f3, n1).reshape(nt, 3), 1e-13))
+    g = [E(4, fr[:, k].copy()).reshape(nt, 3) for k in range(3)]  # -(1/4pi) sum f_k r / r^3
+    B = np.stack([-g[1][:, 2] + g[2][:, 1], -g[2][:, 0] + g[0][:, 2], -g[0][:, 1] + g[1][:, 0]], axis=1)
+    checks.append(("BiotSavart FxU = f x r / 4 pi r^3", E(10, f3).reshape(nt, 3), B, 1e-13))
+    for name, a, b, tol in checks:
+        e = _nrm(a, b)
+        if not e < tol:
+            bad.append("%s %.2e" % (name, e))
     return not bad, ", ".join(bad)
 
 
@@ -181,7 +277,7 @@ def closed_forms(orc):
 def long_double(orc):
     rng = np.random.default_rng(11)
     bad, worst = [], 0.0
-    for k in range(4):
+    for k in ALL_KERNELS:
         _, K0, K1, ND, _ = orc.dims(k)
         xs, xt = rng.random(400 * 3), rng.random(300 * 3)
         xt[:30] = xs[:30]
@@ -198,12 +294,12 @@ def long_double(orc):
 
 def eval_vs_matrix(orc):
     bad = []
-    for k in range(4):
+    for k in ALL_KERNELS:
         _, K0, K1, ND, _ = orc.dims(k)
         Xt, Xs, Xn, Vs = small_inputs(K0, ND)
         vt = orc.eval(k, Xt, Xs, Xn, Vs)
         M = orc.kernel_matrix(k, Xt, Xs, Xn)
-        if M.shape != (7 * K0, 5 * K1) or not np.allclose(vt, M.T @ Vs, rtol=1e-10, atol=1e-14):
+        if M.shape != (7 * K0, 5 * K1) or not np.allclose(vt, M.T @ Vs, rtol=1e-10, atol=1e-14 * np.abs(vt).max()):
             bad.append("k%d" % k)
     return not bad, ", ".join(bad)
 
@@ -262,7 +358,7 @@ def surface_identities(orc, p=40):
 def linearity(orc):
     rng = np.random.default_rng(5)
     bad = []
-    for k in range(4):
+    for k in ALL_KERNELS:
         _, K0, K1, ND, _ = orc.dims(k)
         xs, xt, n = rng.random(64 * 3), rng.random(50 * 3), rng.standard_normal(64 * 3)
         f, g = rng.uniform(-1, 1, 64 * K0), rng.uniform(-1, 1, 64 * K0)
@@ -484,4 +580,4 @@ NEAR_CHECKS = [near_partition_hand, near_layout_hand, near_subtract_identity, ne
                near_list_golden, near_list_hand]
 
 CHECKS = [golden_exact, golden_avx512, golden_rsqrt, rsqrt_bounds, closed_forms, long_double, eval_vs_matrix,
-          surface_identities, linearity] + NEAR_CHECKS
+          surface_identities, linearity, golden_exact_more, derivative_identities] + NEAR_CHECKS
