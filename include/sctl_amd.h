@@ -36,7 +36,7 @@ extern "C" {
 typedef enum {
   SCTL_AMD_SUCCESS = 0,
   SCTL_AMD_ERROR_INVALID_VALUE = 1,  /* bad size, pointer, flag, digits or layout */
-  SCTL_AMD_ERROR_UNKNOWN_KERNEL = 2, /* kernel id outside the four micro-kernels */
+  SCTL_AMD_ERROR_UNKNOWN_KERNEL = 2, /* kernel id outside the twelve micro-kernels */
   SCTL_AMD_ERROR_UNSUPPORTED = 3,    /* precision/option combination not provided */
   SCTL_AMD_ERROR_NO_DEVICE = 4,      /* no HIP device visible */
   SCTL_AMD_ERROR_OUT_OF_MEMORY = 5,  /* device allocation failed */
@@ -44,12 +44,22 @@ typedef enum {
   SCTL_AMD_ERROR_QUEUE = 7           /* batched work queue did not drain as planned */
 } sctl_amd_status;
 
-/* Micro-kernel ids (kernel_functions.hpp:669-684 aliases). */
+/* Micro-kernel ids: the real-valued aliases of kernel_functions.hpp:669-684
+ * (the Helmholtz kernels are complex-valued and not provided).  r = x_t - x_s;
+ * K0 densities per source, K1 outputs per target. */
 enum {
-  SCTL_AMD_LAPLACE3D_FXU = 0, /* K = 1/r,                scale 1/(4 pi), K0=K1=1, no normal */
-  SCTL_AMD_LAPLACE3D_DXU = 1, /* K = (r.n)/r^3,          scale 1/(4 pi), K0=K1=1, normal    */
-  SCTL_AMD_STOKES3D_FXU = 2,  /* K = I/r + r r^T/r^3,    scale 1/(8 pi), K0=K1=3, no normal */
-  SCTL_AMD_STOKES3D_DXU = 3   /* K = r r^T (r.n)/r^5,    scale 3/(4 pi), K0=K1=3, normal    */
+  SCTL_AMD_LAPLACE3D_FXU = 0,    /* K = 1/r,                  scale  1/(4 pi), K0=1, K1=1, no normal */
+  SCTL_AMD_LAPLACE3D_DXU = 1,    /* K = (r.n)/r^3,            scale  1/(4 pi), K0=1, K1=1, normal    */
+  SCTL_AMD_STOKES3D_FXU = 2,     /* K = I/r + r r^T/r^3,      scale  1/(8 pi), K0=3, K1=3, no normal */
+  SCTL_AMD_STOKES3D_DXU = 3,     /* K = r r^T (r.n)/r^5,      scale  3/(4 pi), K0=3, K1=3, normal    */
+  SCTL_AMD_LAPLACE3D_FXDU = 4,   /* K = r^T/r^3,              scale -1/(4 pi), K0=1, K1=3, no normal */
+  SCTL_AMD_LAPLACE3D_DXDU = 5,   /* K = n^T/r^3 - 3(r.n)r^T/r^5, scale 1/(4 pi), K0=1, K1=3, normal  */
+  SCTL_AMD_LAPLACE3D_FXD2U = 6,  /* K = 3 r r^T/r^5 - I/r^3 (row-major 9), 1/(4 pi), K0=1, K1=9      */
+  SCTL_AMD_STOKES3D_FXT = 7,     /* K[i][j*3+k] = r_i r_j r_k/r^5, scale -3/(4 pi), K0=3, K1=9      */
+  SCTL_AMD_STOKES3D_FSXU = 8,    /* [f, q] -> f/r + r (r.f + q)/r^3, scale 1/(8 pi), K0=4, K1=3     */
+  SCTL_AMD_STOKES3D_FXUP = 9,    /* f -> [Stokes FxU f, (r.f)/r^3], scale 1/(8 pi), K0=3, K1=4      */
+  SCTL_AMD_BIOTSAVART3D_FXU = 10, /* f -> (f x r)/r^3,         scale  1/(4 pi), K0=3, K1=3           */
+  SCTL_AMD_BIOTSAVART3D_FXDU = 11 /* gradient of BiotSavart FxU, scale -1/(4 pi), K0=3, K1=9          */
 };
 
 /* Arithmetic of an evaluation. */
@@ -144,9 +154,10 @@ int sctl_amd_plan_interactions(sctl_amd_plan plan, int64_t* interactions);
 int sctl_amd_plan_destroy(sctl_amd_plan plan);
 
 /* Launch knobs of a plan: sources staged through LDS per wave-stage
- * (64 * sources_per_lane), targets per lane (1 or 2), and a cap on resident
- * workgroups of 4 waves per CU (0 = no cap).  Chosen at creation from the
- * tuning database, else defaults; sctl_amd_plan_tune searches them. */
+ * (64 * sources_per_lane), targets per lane (1 or 2; kernels with more than
+ * 4 outputs per target always run with 1), and a cap on resident workgroups
+ * of 4 waves per CU (0 = no cap).  Chosen at creation from the tuning
+ * database, else defaults; sctl_amd_plan_tune searches them. */
 int sctl_amd_plan_get_shape(sctl_amd_plan plan, int* sources_per_lane, int* targets_per_lane, int* wg_per_cu_cap);
 int sctl_amd_plan_set_shape(sctl_amd_plan plan, int sources_per_lane, int targets_per_lane, int wg_per_cu_cap);
 
@@ -180,7 +191,8 @@ int sctl_amd_near_blocks(int64_t nelem, const int64_t* near_elem_cnt, const int6
  * block order, which is the reference's layout when the blocks are
  * sctl_amd_near_blocks' blocks of matrix elements
  * (K_near_dsp[e]*KDIM0*KDIM1_ + t0*KDIM1_*src_dof, :1187-1189).
- * trg_normal_dot selects the target-normal contraction (Stokes kernels only).
+ * trg_normal_dot selects the target-normal contraction (kernels with 3 or 9
+ * outputs per target; KDIM1_ = KDIM1/3).
  * *k_len receives the length K_near must have.  Block matrices must not
  * overlap. */
 int sctl_amd_plan_near_layout(sctl_amd_plan plan, const int64_t* blk_off, int trg_normal_dot, int64_t* k_len);
@@ -213,7 +225,7 @@ int sctl_amd_plan_near_gemv(sctl_amd_plan plan, const double* K, const double* F
  * (boundary_integral.txx:1368-1379; comm.txx:1497-1508 on one process):
  *   U[i*dof+k] += sum_{j < near_trg_cnt[i]} U_near[scatter_index[near_trg_dsp[i]+j]*dof + k]
  * with the additions in the reference's order (bit-identical results).
- * dof is 1 or 3; U_near has n_near*dof entries, scatter_index n_scatter.
+ * dof is 1, 3, 4 or 9; U_near has n_near*dof entries, scatter_index n_scatter.
  * With HOST_PTRS out-of-range indices are an error; with device pointers
  * they are skipped. */
 int sctl_amd_near_accumulate(int64_t ntrg, int dof, const int64_t* near_trg_cnt, const int64_t* near_trg_dsp,
@@ -286,9 +298,10 @@ int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_tr
 
 /* The launch shape sctl_amd_eval uses for this (kernel, precision, digits,
  * flags, nt, ns) size class: get resolves it (tuning database, else
- * defaults); set records one in the database (targets_per_lane 1, 2 or 4;
- * wg_per_cu_cap >= 0; matrix_cores 0/1, honoured by the mixed Laplace
- * kernels only).  Host-only. */
+ * defaults); set records one in the database (targets_per_lane 1, 2 or 4,
+ * capped at 2 for 4 outputs per target and at 1 for 9; wg_per_cu_cap >= 0;
+ * matrix_cores 0/1, honoured by the mixed Laplace FxU / DxU kernels only).
+ * Host-only. */
 int sctl_amd_dense_shape_get(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
                              int* targets_per_lane, int* wg_per_cu_cap, int* matrix_cores);
 int sctl_amd_dense_shape_set(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,

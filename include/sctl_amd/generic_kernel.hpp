@@ -23,8 +23,10 @@
 // they are.  sctl_amd/containers.hpp provides minimal owning containers for
 // hosts without SCTL.
 //
-// Only the four micro-kernels of the device library exist here (Laplace3D
-// FxU/DxU, Stokes3D FxU/DxU); any other uKernel is a compile-time error.
+// The twelve real-valued micro-kernels of the device library exist here
+// (Laplace3D FxU/DxU/FxdU/DxdU/Fxd2U, Stokes3D FxU/DxU/FxT/FSxU/FxUP,
+// BiotSavart3D FxU/FxdU); any other uKernel (the complex Helmholtz kernels)
+// is a compile-time error.
 //
 // Precision: Real = double runs FP64 unless 0 <= digits <= 7, which selects
 // the mixed-precision form (FP64 differencing, FP32 pairwise, Kahan FP64
@@ -68,8 +70,12 @@ inline void check(int status, const char* file, int line) {
 
 namespace kernel_impl {
 
-// The four micro-kernels: Name() and FLOPS() as in kernel_functions.hpp,
-// uKerScaleFactor<Real>() with the reference's const_pi (math_utils.hpp:43).
+// The real-valued micro-kernels: Name() and FLOPS() as in
+// kernel_functions.hpp, uKerScaleFactor<Real>() with the reference's const_pi
+// (math_utils.hpp:43).
+// True for the micro-kernel types of this header only.
+template <class T> struct is_device_kernel : std::false_type {};
+
 template <class Real> constexpr Real const_pi() {
   return (Real)14488038916154245684ull / (Real)(1ull << 62) +
          (Real)7089564414062235240ull / (Real)(1ull << 62) / (Real)(1ull << 63);
@@ -119,14 +125,43 @@ struct Stokes3D_DxU {  // kernel_functions.hpp:180-218
   template <class Real> static constexpr Real uKerScaleFactor() { return 3 / (4 * const_pi<Real>()); }
 };
 
+#define SCTL_AMD_UKERNEL(NAME, ID, K0, K1, ND, STR, FLOPS_, NUM, DEN)                    \
+  struct NAME {                                                                       \
+    static constexpr int KERNEL_ID = ID;                                              \
+    static constexpr Integer DIM = 3, KDIM0 = K0, KDIM1 = K1, N_DIM = ND;             \
+    static const std::string& Name() {                                                \
+      static const std::string name = STR;                                            \
+      return name;                                                                    \
+    }                                                                                 \
+    static constexpr Integer FLOPS() { return FLOPS_; }                               \
+    template <class Real> static constexpr Real uKerScaleFactor() {                   \
+      return NUM / (DEN * const_pi<Real>());                                          \
+    }                                                                                 \
+  };                                                                                  \
+  template <> struct is_device_kernel<NAME> : std::true_type {}
+
+SCTL_AMD_UKERNEL(Laplace3D_FxdU, SCTL_AMD_LAPLACE3D_FXDU, 1, 3, 0, "Laplace3D-FxdU", 11, -1, 4);      // :81-123
+SCTL_AMD_UKERNEL(Laplace3D_DxdU, SCTL_AMD_LAPLACE3D_DXDU, 1, 3, 3, "Laplace3D-DxdU", 22, 1, 4);       // :392-414
+SCTL_AMD_UKERNEL(Laplace3D_Fxd2U, SCTL_AMD_LAPLACE3D_FXD2U, 1, 9, 0, "Laplace3D-Fxd2U", 25, 1, 4);    // :366-390
+SCTL_AMD_UKERNEL(Stokes3D_FxT, SCTL_AMD_STOKES3D_FXT, 3, 9, 0, "Stokes3D-FxT", 39, -3, 4);            // :220-263
+SCTL_AMD_UKERNEL(Stokes3D_FSxU, SCTL_AMD_STOKES3D_FSXU, 4, 3, 0, "Stokes3D-FSxU", 26, 1, 8);          // :264-324
+SCTL_AMD_UKERNEL(Stokes3D_FxUP, SCTL_AMD_STOKES3D_FXUP, 3, 4, 0, "Stokes3D-FxUP", 26, 1, 8);          // :325-365
+SCTL_AMD_UKERNEL(BiotSavart3D_FxU, SCTL_AMD_BIOTSAVART3D_FXU, 3, 3, 0, "BiotSavart3D-FxU", 15, 1, 4); // :416-450
+SCTL_AMD_UKERNEL(BiotSavart3D_FxdU, SCTL_AMD_BIOTSAVART3D_FXDU, 3, 9, 0, "BiotSavart3D-FxdU", 51, -1, 4);  // :451-489
+#undef SCTL_AMD_UKERNEL
+template <> struct is_device_kernel<Laplace3D_FxU> : std::true_type {};
+template <> struct is_device_kernel<Laplace3D_DxU> : std::true_type {};
+template <> struct is_device_kernel<Stokes3D_FxU> : std::true_type {};
+template <> struct is_device_kernel<Stokes3D_DxU> : std::true_type {};
+
 }  // namespace kernel_impl
 
 template <class uKernel> class GenericKernel : public uKernel {
-  static_assert(std::is_same<uKernel, kernel_impl::Laplace3D_FxU>::value ||
-                    std::is_same<uKernel, kernel_impl::Laplace3D_DxU>::value ||
-                    std::is_same<uKernel, kernel_impl::Stokes3D_FxU>::value ||
-                    std::is_same<uKernel, kernel_impl::Stokes3D_DxU>::value,
-                "sctl_amd::GenericKernel: only Laplace3D_FxU/DxU and Stokes3D_FxU/DxU run on the device");
+  // Only the kernel_impl types above (which carry a device kernel id) run on
+  // the device; sctl::kernel_impl types (e.g. the complex Helmholtz kernels)
+  // fail here rather than fall back to the CPU.
+  static_assert(kernel_impl::is_device_kernel<uKernel>::value,
+                "sctl_amd::GenericKernel: only the sctl_amd::kernel_impl micro-kernels run on the device");
   static constexpr Integer DIM = uKernel::DIM, KDIM0 = uKernel::KDIM0, KDIM1 = uKernel::KDIM1,
                            N_DIM = uKernel::N_DIM;
 
@@ -232,6 +267,14 @@ using Laplace3D_FxU = GenericKernel<kernel_impl::Laplace3D_FxU>;
 using Laplace3D_DxU = GenericKernel<kernel_impl::Laplace3D_DxU>;
 using Stokes3D_FxU = GenericKernel<kernel_impl::Stokes3D_FxU>;
 using Stokes3D_DxU = GenericKernel<kernel_impl::Stokes3D_DxU>;
+using Laplace3D_FxdU = GenericKernel<kernel_impl::Laplace3D_FxdU>;
+using Laplace3D_DxdU = GenericKernel<kernel_impl::Laplace3D_DxdU>;
+using Laplace3D_Fxd2U = GenericKernel<kernel_impl::Laplace3D_Fxd2U>;
+using Stokes3D_FxT = GenericKernel<kernel_impl::Stokes3D_FxT>;
+using Stokes3D_FSxU = GenericKernel<kernel_impl::Stokes3D_FSxU>;
+using Stokes3D_FxUP = GenericKernel<kernel_impl::Stokes3D_FxUP>;
+using BiotSavart3D_FxU = GenericKernel<kernel_impl::BiotSavart3D_FxU>;
+using BiotSavart3D_FxdU = GenericKernel<kernel_impl::BiotSavart3D_FxdU>;
 
 }  // namespace sctl_amd
 

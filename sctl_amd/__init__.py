@@ -24,7 +24,13 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsctl_amd.so")
 
 LAPLACE3D_FXU, LAPLACE3D_DXU, STOKES3D_FXU, STOKES3D_DXU = 0, 1, 2, 3
-KERNEL_NAMES = {0: "Laplace3D_FxU", 1: "Laplace3D_DxU", 2: "Stokes3D_FxU", 3: "Stokes3D_DxU"}
+LAPLACE3D_FXDU, LAPLACE3D_DXDU, LAPLACE3D_FXD2U = 4, 5, 6
+STOKES3D_FXT, STOKES3D_FSXU, STOKES3D_FXUP = 7, 8, 9
+BIOTSAVART3D_FXU, BIOTSAVART3D_FXDU = 10, 11
+KERNEL_NAMES = {0: "Laplace3D_FxU", 1: "Laplace3D_DxU", 2: "Stokes3D_FxU", 3: "Stokes3D_DxU",
+                4: "Laplace3D_FxdU", 5: "Laplace3D_DxdU", 6: "Laplace3D_Fxd2U", 7: "Stokes3D_FxT",
+                8: "Stokes3D_FSxU", 9: "Stokes3D_FxUP", 10: "BiotSavart3D_FxU", 11: "BiotSavart3D_FxdU"}
+NUM_KERNELS = len(KERNEL_NAMES)
 FP64, MIXED, AUTO = 0, 1, 2
 HOST_PTRS, MIXED_NEWTON = 0x1, 0x2
 NEAR_SUBTRACT, NEAR_STORE = 0, 1

@@ -131,7 +131,7 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
 #pragma unroll 2
       for (int i = g; i < STAGE; i += G) {
         const double* rec = lds + i * R;
-        double n[3] = {0, 0, 0}, f[3];
+        double n[3] = {0, 0, 0}, f[K0];
 #pragma unroll
         for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
 #pragma unroll
@@ -152,7 +152,7 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
           for (int k = 0; k < K1; k++) acc[j][k] = saved[j][k];
         for (int i = g; i < len; i += G) {
           const double* rec = lds + i * R;
-          double n[3] = {0, 0, 0}, f[3];
+          double n[3] = {0, 0, 0}, f[K0];
           for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
           for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
 #pragma unroll
@@ -251,7 +251,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
       for (int i = g; i < STAGE; i += G) {
         const double* pr = pos + i * 4;
         const float* ar = aux + i * AW;
-        float n[3] = {0, 0, 0}, f[3];
+        float n[3] = {0, 0, 0}, f[K0];
 #pragma unroll
         for (int k = 0; k < ND; k++) n[k] = ar[k];
 #pragma unroll
@@ -274,7 +274,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
         for (int i = g; i < len; i += G) {
           const double* pr = pos + i * 4;
           const float* ar = aux + i * AW;
-          float n[3] = {0, 0, 0}, f[3];
+          float n[3] = {0, 0, 0}, f[K0];
           for (int k = 0; k < ND; k++) n[k] = ar[k];
           for (int k = 0; k < K0; k++) f[k] = ar[ND + k];
 #pragma unroll
@@ -312,39 +312,42 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
 using BlocksFn = void (*)(const Item*, int, int*, const int64_t*, const int64_t*, const int32_t*, const double*,
                           const double*, const double*, const double*, double*, double);
 
+// Targets per lane: 1 or 2 (2 only where max_tpl allows it).
 template <int KER, int NEWTON, int SPL> BlocksFn f64_blocks_t(int tpl) {
-  return tpl >= 2 ? k_blocks_f64<KER, NEWTON, SPL, 2> : k_blocks_f64<KER, NEWTON, SPL, 1>;
+  if constexpr (max_tpl<KER>() >= 2) {
+    if (tpl >= 2) return k_blocks_f64<KER, NEWTON, SPL, 2>;
+  }
+  return k_blocks_f64<KER, NEWTON, SPL, 1>;
 }
 template <int KER, int NEWTON> BlocksFn f64_blocks_s(int spl, int tpl) {
   return spl >= 4 ? f64_blocks_t<KER, NEWTON, 4>(tpl)
                   : (spl >= 2 ? f64_blocks_t<KER, NEWTON, 2>(tpl) : f64_blocks_t<KER, NEWTON, 1>(tpl));
 }
-template <int KER> BlocksFn f64_blocks_n(int newton, int spl, int tpl) {
-  return newton == 0 ? f64_blocks_s<KER, 0>(spl, tpl)
-                     : (newton == 1 ? f64_blocks_s<KER, 1>(spl, tpl) : f64_blocks_s<KER, 2>(spl, tpl));
-}
 BlocksFn f64_blocks(int ker, int newton, int spl, int tpl) {
-  switch (ker) {
-    case LAP_FXU: return f64_blocks_n<LAP_FXU>(newton, spl, tpl);
-    case LAP_DXU: return f64_blocks_n<LAP_DXU>(newton, spl, tpl);
-    case STK_FXU: return f64_blocks_n<STK_FXU>(newton, spl, tpl);
-    default: return f64_blocks_n<STK_DXU>(newton, spl, tpl);
-  }
+  return with_kernel(ker, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    return newton == 0 ? f64_blocks_s<K, 0>(spl, tpl)
+                       : (newton == 1 ? f64_blocks_s<K, 1>(spl, tpl) : f64_blocks_s<K, 2>(spl, tpl));
+  });
 }
 template <int KER, int NEWTON, int SPL> BlocksFn mix_blocks_t(int tpl) {
-  return tpl >= 2 ? k_blocks_mixed<KER, NEWTON, SPL, 2> : k_blocks_mixed<KER, NEWTON, SPL, 1>;
+  if constexpr (max_tpl<KER>() >= 2) {
+    if (tpl >= 2) return k_blocks_mixed<KER, NEWTON, SPL, 2>;
+  }
+  return k_blocks_mixed<KER, NEWTON, SPL, 1>;
 }
 template <int KER, int NEWTON> BlocksFn mix_blocks_s(int spl, int tpl) {
   return spl >= 4 ? mix_blocks_t<KER, NEWTON, 4>(tpl)
                   : (spl >= 2 ? mix_blocks_t<KER, NEWTON, 2>(tpl) : mix_blocks_t<KER, NEWTON, 1>(tpl));
 }
 BlocksFn mix_blocks(int ker, int newton, int spl, int tpl) {
-  switch (ker) {
-    case LAP_FXU: return newton ? mix_blocks_s<LAP_FXU, 1>(spl, tpl) : mix_blocks_s<LAP_FXU, 0>(spl, tpl);
-    case LAP_DXU: return newton ? mix_blocks_s<LAP_DXU, 1>(spl, tpl) : mix_blocks_s<LAP_DXU, 0>(spl, tpl);
-    case STK_FXU: return newton ? mix_blocks_s<STK_FXU, 1>(spl, tpl) : mix_blocks_s<STK_FXU, 0>(spl, tpl);
-    default: return newton ? mix_blocks_s<STK_DXU, 1>(spl, tpl) : mix_blocks_s<STK_DXU, 0>(spl, tpl);
-  }
+  return with_kernel(ker, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    return newton ? mix_blocks_s<K, 1>(spl, tpl) : mix_blocks_s<K, 0>(spl, tpl);
+  });
+}
+int plan_max_tpl(int ker) {
+  return with_kernel(ker, [](auto kc) { return max_tpl<decltype(kc)::value>(); });
 }
 // Static LDS bytes of one workgroup of a blocks kernel.
 size_t blocks_lds(int ker, int precision, int spl) {
@@ -354,15 +357,6 @@ size_t blocks_lds(int ker, int precision, int spl) {
   if (precision == SCTL_AMD_MIXED) return stage * (4 * sizeof(double) + (((ND + K0) + 3) & ~3) * sizeof(float));
   return stage * (((3 + ND + K0) + 1) & ~1) * sizeof(double);
 }
-double fold(int ker, int newton) {
-  switch (ker) {
-    case LAP_FXU: return newton == 0 ? 1.0 : (newton == 1 ? fold_factor<LAP_FXU, 1>() : fold_factor<LAP_FXU, 2>());
-    case LAP_DXU: return newton == 0 ? 1.0 : (newton == 1 ? fold_factor<LAP_DXU, 1>() : fold_factor<LAP_DXU, 2>());
-    case STK_DXU: return newton == 0 ? 1.0 : (newton == 1 ? fold_factor<STK_DXU, 1>() : fold_factor<STK_DXU, 2>());
-    default: return 1.0;
-  }
-}
-
 }  // namespace
 }  // namespace sctl_amd
 
@@ -409,7 +403,7 @@ int plan_configure(sctl_amd_plan_s* p, PlanShape s) {
   DeviceInfo dev;
   SCTL_TRY(device_info(&dev));
   s.spl = s.spl >= 4 ? 4 : (s.spl >= 2 ? 2 : 1);
-  s.tpl = s.tpl >= 2 ? 2 : 1;
+  s.tpl = s.tpl >= 2 && plan_max_tpl(p->kernel) >= 2 ? 2 : 1;
   s.occ = std::max(0, s.occ);
   if (!p->d_items || s.tpl != p->shape.tpl) {
     const std::vector<Item> items = build_items(p, s.tpl);
@@ -448,7 +442,7 @@ int plan_launch(sctl_amd_plan_s* p, const double* v_src, double* v_trg, hipStrea
   if (p->nitems == 0) return SCTL_AMD_SUCCESS;
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, p->d_counter);
   SCTL_HIP_TRY(hipGetLastError());
-  const double scale = kernel_scale(p->kernel) * fold(p->kernel, p->newton);
+  const double scale = kernel_scale(p->kernel) * fold_factor(p->kernel, p->newton);
   const int grid = (int)std::min<int64_t>(p->grid, ((int64_t)p->nitems + 3) / 4);
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   SCTL_HIP_TRY(hipStreamIsCapturing(s, &cs));

@@ -20,13 +20,15 @@ struct KernelMeta {
   const char* name;
 };
 // GenericKernel dims (generic-kernel.txx:48-62) and FLOPS()/Name()
-// (kernel_functions.hpp:33-39,62-67,126-131,181-186).
+// (kernel_functions.hpp:33-39,62-67,82-88,126-131,181-186,221-227,265-271,
+// 326-332,367-373,393-399,417-423,452-458).
 constexpr KernelMeta META[NUM_KERNELS] = {
-    {1, 1, 0, 6, "Laplace3D-FxU"},
-    {1, 1, 3, 14, "Laplace3D-DxU"},
-    {3, 3, 0, 23, "Stokes3D-FxU"},
-    {3, 3, 3, 26, "Stokes3D-DxU"},
+    {1, 1, 0, 6, "Laplace3D-FxU"},      {1, 1, 3, 14, "Laplace3D-DxU"},    {3, 3, 0, 23, "Stokes3D-FxU"},
+    {3, 3, 3, 26, "Stokes3D-DxU"},      {1, 3, 0, 11, "Laplace3D-FxdU"},   {1, 3, 3, 22, "Laplace3D-DxdU"},
+    {1, 9, 0, 25, "Laplace3D-Fxd2U"},   {3, 9, 0, 39, "Stokes3D-FxT"},     {4, 3, 0, 26, "Stokes3D-FSxU"},
+    {3, 4, 0, 26, "Stokes3D-FxUP"},     {3, 3, 0, 15, "BiotSavart3D-FxU"}, {3, 9, 0, 51, "BiotSavart3D-FxdU"},
 };
+static_assert(Ker<LAP_FXD2U>::K1 == 9 && Ker<STK_FSXU>::K0 == 4 && Ker<LAP_DXDU>::ND == 3, "tables disagree");
 
 // const_pi<double>() of math_utils.hpp:43, so the scale factors round as the
 // reference's uKerScaleFactor<double>() does.
@@ -52,14 +54,29 @@ int kernel_dims(int ker, int* K0, int* K1, int* ND) {
   return SCTL_AMD_SUCCESS;
 }
 
+// uKerScaleFactor<double>() (kernel_functions.hpp:40,69,89,133,188,228,272,
+// 333,374,400,424,459).
 double kernel_scale(int ker) {
   switch (ker) {
     case LAP_FXU:
-    case LAP_DXU: return 1 / (4 * const_pi());
-    case STK_FXU: return 1 / (8 * const_pi());
+    case LAP_DXU:
+    case LAP_DXDU:
+    case LAP_FXD2U:
+    case BS_FXU: return 1 / (4 * const_pi());
+    case STK_FXU:
+    case STK_FSXU:
+    case STK_FXUP: return 1 / (8 * const_pi());
     case STK_DXU: return 3 / (4 * const_pi());
+    case LAP_FXDU:
+    case BS_FXDU: return -1 / (4 * const_pi());
+    case STK_FXT: return -3 / (4 * const_pi());
   }
   return 0;
+}
+
+int kernel_max_tpl(int ker) {
+  if (ker < 0 || ker >= NUM_KERNELS) return 1;
+  return with_kernel(ker, [](auto kc) { return max_tpl<decltype(kc)::value>(); });
 }
 
 // v_rsq_f64 is good to about 2^-24 (7 digits); each Newton step doubles the

@@ -91,7 +91,7 @@ k_eval_f64(const double* __restrict__ xt, int64_t nt, const double* __restrict__
     for (int i = 0; i < TILE; i++) {
       const double* rec = lds + i * R;
       const double sx = rec[0], sy = rec[1], sz = rec[2];
-      double n[3] = {0, 0, 0}, f[3];
+      double n[3] = {0, 0, 0}, f[K0];
 #pragma unroll
       for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
 #pragma unroll
@@ -112,7 +112,7 @@ k_eval_f64(const double* __restrict__ xt, int64_t nt, const double* __restrict__
         for (int k = 0; k < K1; k++) acc[j][k] = saved[j][k];
       for (int i = 0; i < len; i++) {
         const double* rec = lds + i * R;
-        double n[3] = {0, 0, 0}, f[3];
+        double n[3] = {0, 0, 0}, f[K0];
         for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
         for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
 #pragma unroll
@@ -162,7 +162,7 @@ k_kernel_matrix(const double* __restrict__ xt, int64_t nt, const double* __restr
     double n[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < ND; k++) n[k] = xn[s * 3 + k];
-    double u[3][3];
+    double u[K0][K1];
     kmatrix<KER, NEWTON>(u, tx - xs[s * 3 + 0], ty - xs[s * 3 + 1], tz - xs[s * 3 + 2], n);
 #pragma unroll
     for (int k0 = 0; k0 < K0; k0++)
@@ -174,8 +174,6 @@ k_kernel_matrix(const double* __restrict__ xt, int64_t nt, const double* __restr
 // --------------------------------------------------------------- dispatch
 namespace {
 
-template <class F> struct Table;
-
 using EvalF64Fn = void (*)(const double*, int64_t, const double*, const double*, const double*, int64_t, int64_t,
                            double*, int, double);
 using KMatFn = void (*)(const double*, int64_t, const double*, const double*, int64_t, double*, double);
@@ -184,62 +182,28 @@ constexpr int TPLS[3] = {1, 2, 4};
 int tpl_index(int tpl) { return tpl >= 4 ? 2 : (tpl >= 2 ? 1 : 0); }
 
 template <int KER, int NEWTON> EvalF64Fn f64_fn(int ti) {
-  switch (ti) {
-    case 0: return k_eval_f64<KER, 1, NEWTON>;
-    case 1: return k_eval_f64<KER, 2, NEWTON>;
-    default: return k_eval_f64<KER, 4, NEWTON>;
-  }
-}
-template <int KER> EvalF64Fn f64_fn(int newton, int ti) {
-  switch (newton) {
-    case 0: return f64_fn<KER, 0>(ti);
-    case 1: return f64_fn<KER, 1>(ti);
-    default: return f64_fn<KER, 2>(ti);
-  }
+  constexpr int MT = max_tpl<KER>();
+  if (ti >= 2 && MT >= 4) return k_eval_f64<KER, (MT >= 4 ? 4 : 1), NEWTON>;
+  if (ti >= 1 && MT >= 2) return k_eval_f64<KER, (MT >= 2 ? 2 : 1), NEWTON>;
+  return k_eval_f64<KER, 1, NEWTON>;
 }
 EvalF64Fn f64_fn(int ker, int newton, int ti) {
-  switch (ker) {
-    case LAP_FXU: return f64_fn<LAP_FXU>(newton, ti);
-    case LAP_DXU: return f64_fn<LAP_DXU>(newton, ti);
-    case STK_FXU: return f64_fn<STK_FXU>(newton, ti);
-    default: return f64_fn<STK_DXU>(newton, ti);
-  }
+  return with_kernel(ker, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    return newton == 0 ? f64_fn<K, 0>(ti) : (newton == 1 ? f64_fn<K, 1>(ti) : f64_fn<K, 2>(ti));
+  });
 }
-
 int rec_words_f64(int ker) {
-  switch (ker) {
-    case LAP_FXU: return rec_f64<LAP_FXU>();
-    case LAP_DXU: return rec_f64<LAP_DXU>();
-    case STK_FXU: return rec_f64<STK_FXU>();
-    default: return rec_f64<STK_DXU>();
-  }
+  return with_kernel(ker, [](auto kc) { return rec_f64<decltype(kc)::value>(); });
 }
-template <int KER> KMatFn kmat_fn(int newton) {
-  switch (newton) {
-    case 0: return k_kernel_matrix<KER, 0>;
-    case 1: return k_kernel_matrix<KER, 1>;
-    default: return k_kernel_matrix<KER, 2>;
-  }
+int max_tpl_index(int ker) {
+  return with_kernel(ker, [](auto kc) { return tpl_index(max_tpl<decltype(kc)::value>()); });
 }
 KMatFn kmat_fn(int ker, int newton) {
-  switch (ker) {
-    case LAP_FXU: return kmat_fn<LAP_FXU>(newton);
-    case LAP_DXU: return kmat_fn<LAP_DXU>(newton);
-    case STK_FXU: return kmat_fn<STK_FXU>(newton);
-    default: return kmat_fn<STK_DXU>(newton);
-  }
-}
-double fold(int ker, int newton) {
-  auto f = [&](auto kc) -> double {
+  return with_kernel(ker, [&](auto kc) -> KMatFn {
     constexpr int K = decltype(kc)::value;
-    return newton == 0 ? fold_factor<K, 0>() : (newton == 1 ? fold_factor<K, 1>() : fold_factor<K, 2>());
-  };
-  switch (ker) {
-    case LAP_FXU: return f(std::integral_constant<int, LAP_FXU>());
-    case LAP_DXU: return f(std::integral_constant<int, LAP_DXU>());
-    case STK_FXU: return f(std::integral_constant<int, STK_FXU>());
-    default: return f(std::integral_constant<int, STK_DXU>());
-  }
+    return newton == 0 ? k_kernel_matrix<K, 0> : (newton == 1 ? k_kernel_matrix<K, 1> : k_kernel_matrix<K, 2>);
+  });
 }
 
 // Splits the source range over gridDim.y when the target blocks alone cannot
@@ -264,13 +228,13 @@ int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const 
   SCTL_TRY(device_info(&dev));
   int K0, K1, ND;
   kernel_dims(ker, &K0, &K1, &ND);
-  const int ti = tpl_index(shape.tpl), tpl = TPLS[ti];
+  const int ti = std::min(tpl_index(shape.tpl), max_tpl_index(ker)), tpl = TPLS[ti];
   const int64_t tb = (nt + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
   SCTL_CHECK_ARG(tb <= 0x7fffffffLL, "too many targets for one launch");
   const int64_t ntiles = (ns + TILE - 1) / TILE;
   int64_t nsplit, tps;
   split_sources(tb, ntiles, dev.num_cus, &nsplit, &tps);
-  const double scale = kernel_scale(ker) * fold(ker, newton);
+  const double scale = kernel_scale(ker) * fold_factor(ker, newton);
   const EvalF64Fn fn = f64_fn(ker, newton, ti);
   size_t dyn = 0;
   SCTL_TRY(occupancy_lds((const void*)fn, shape.occ, sizeof(double) * TILE * rec_words_f64(ker), &dyn));

@@ -140,6 +140,9 @@ int plan_kernel(sctl_amd_plan p);
 
 int kernel_dims(int ker, int* K0, int* K1, int* ND);
 double kernel_scale(int ker);
+// Largest targets-per-lane count the dense and batched kernels take for a
+// kernel (larger requests run with this many).
+int kernel_max_tpl(int ker);
 int newton_steps_f64(int digits);
 
 }  // namespace sctl_amd

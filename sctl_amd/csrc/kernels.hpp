@@ -1,19 +1,26 @@
 // Device-side micro-kernels for the direct summation on gfx950.
 //
-// Each micro-kernel is the per-interaction body of one of SCTL's uKernels
-// (kernel_functions.hpp:32-79 Laplace3D_FxU/DxU, :125-178 Stokes3D_FxU,
-// :180-218 Stokes3D_DxU).  Eval uses the fused apply form (K * f accumulated
-// without forming K, kernel_functions.hpp:165-176,204-217); KernelMatrix uses
-// the matrix form.  One lane owns one target; the source record is the same
-// for every lane of the wave (broadcast from LDS).
+// Each micro-kernel is the per-interaction body of one of SCTL's real-valued
+// uKernels (kernel_functions.hpp:32-79 Laplace3D_FxU/DxU, :81-123 Laplace3D_FxdU,
+// :125-178 Stokes3D_FxU, :180-218 Stokes3D_DxU, :220-263 Stokes3D_FxT,
+// :264-324 Stokes3D_FSxU, :325-365 Stokes3D_FxUP, :366-390 Laplace3D_Fxd2U,
+// :392-414 Laplace3D_DxdU, :416-450 BiotSavart3D_FxU, :451-489
+// BiotSavart3D_FxdU).  Eval uses a fused apply form (K * f accumulated without
+// forming K; the reference's own for the kernels that have one, e.g.
+// kernel_functions.hpp:165-176,204-217, and an algebraically equal one written
+// here for Laplace DxdU / Fxd2U and BiotSavart FxdU, whose reference Eval
+// forms the matrix); KernelMatrix uses the matrix form.  One lane owns one
+// target; the source record is the same for every lane of the wave
+// (broadcast from LDS).
 //
 // Reciprocal square root.  The hardware estimate (v_rsq_f64 / v_rsq_f32) is
 // refined by N deferred-scaling Newton steps, the scheme of
 // intrin-wrapper.hpp:540-559: z <- z * (x*z*z + c1_j) with c1_j =
 // -3*2^(3^j-1), and the constant (-1/2)^((3^(N-1)-1)*3/2+1) applied once.
 // Here that constant is folded into the per-target scale wherever the kernel
-// value is a single power of 1/r (Laplace FxU: r^-1, Laplace DxU: r^-3,
-// Stokes DxU: r^-5), so a Newton step costs three VALU operations.
+// value is a single power of 1/r (Laplace FxU: r^-1, Laplace DxU and FxdU,
+// BiotSavart FxU: r^-3, Stokes DxU and FxT: r^-5), so a Newton step costs
+// three VALU operations; the other kernels scale 1/r once per pair.
 //
 // Masking.  The reference masks r2 > 0 so that coincident points contribute
 // exactly zero (kernel_functions.hpp:45, vec-test.hpp:449-463).  The fast
@@ -23,16 +30,56 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace sctl_amd {
 
-enum KernelId { LAP_FXU = 0, LAP_DXU = 1, STK_FXU = 2, STK_DXU = 3, NUM_KERNELS = 4 };
+enum KernelId {
+  LAP_FXU = 0, LAP_DXU = 1, STK_FXU = 2, STK_DXU = 3,
+  LAP_FXDU = 4, LAP_DXDU = 5, LAP_FXD2U = 6, STK_FXT = 7, STK_FSXU = 8, STK_FXUP = 9, BS_FXU = 10, BS_FXDU = 11,
+  NUM_KERNELS = 12
+};
 
+// K0 densities per source, K1 outputs per target, ND normal words per source
+// (generic-kernel.txx:48-62), POW: the single power of 1/r of the kernel
+// value (0 if it mixes powers).
 template <int KER> struct Ker;
 template <> struct Ker<LAP_FXU> { static constexpr int K0 = 1, K1 = 1, ND = 0, POW = 1; };
 template <> struct Ker<LAP_DXU> { static constexpr int K0 = 1, K1 = 1, ND = 3, POW = 3; };
 template <> struct Ker<STK_FXU> { static constexpr int K0 = 3, K1 = 3, ND = 0, POW = 0; };
 template <> struct Ker<STK_DXU> { static constexpr int K0 = 3, K1 = 3, ND = 3, POW = 5; };
+template <> struct Ker<LAP_FXDU> { static constexpr int K0 = 1, K1 = 3, ND = 0, POW = 3; };
+template <> struct Ker<LAP_DXDU> { static constexpr int K0 = 1, K1 = 3, ND = 3, POW = 0; };
+template <> struct Ker<LAP_FXD2U> { static constexpr int K0 = 1, K1 = 9, ND = 0, POW = 0; };
+template <> struct Ker<STK_FXT> { static constexpr int K0 = 3, K1 = 9, ND = 0, POW = 5; };
+template <> struct Ker<STK_FSXU> { static constexpr int K0 = 4, K1 = 3, ND = 0, POW = 0; };
+template <> struct Ker<STK_FXUP> { static constexpr int K0 = 3, K1 = 4, ND = 0, POW = 0; };
+template <> struct Ker<BS_FXU> { static constexpr int K0 = 3, K1 = 3, ND = 0, POW = 3; };
+template <> struct Ker<BS_FXDU> { static constexpr int K0 = 3, K1 = 9, ND = 0, POW = 0; };
+constexpr int KMAX0 = 4, KMAX1 = 9;
+
+// Calls f(std::integral_constant<int, KER>()) for a kernel id (host side
+// dispatch to the kernel templates; ids are checked by the C-ABI first).
+template <class F> inline auto with_kernel(int ker, F&& f) {
+  switch (ker) {
+    case LAP_FXU: return f(std::integral_constant<int, LAP_FXU>());
+    case LAP_DXU: return f(std::integral_constant<int, LAP_DXU>());
+    case STK_FXU: return f(std::integral_constant<int, STK_FXU>());
+    case STK_DXU: return f(std::integral_constant<int, STK_DXU>());
+    case LAP_FXDU: return f(std::integral_constant<int, LAP_FXDU>());
+    case LAP_DXDU: return f(std::integral_constant<int, LAP_DXDU>());
+    case LAP_FXD2U: return f(std::integral_constant<int, LAP_FXD2U>());
+    case STK_FXT: return f(std::integral_constant<int, STK_FXT>());
+    case STK_FSXU: return f(std::integral_constant<int, STK_FSXU>());
+    case STK_FXUP: return f(std::integral_constant<int, STK_FXUP>());
+    case BS_FXU: return f(std::integral_constant<int, BS_FXU>());
+    default: return f(std::integral_constant<int, BS_FXDU>());
+  }
+}
+
+// Largest targets-per-lane count instantiated for a kernel: TPL * K1
+// accumulators (plus their saved copies) stay in registers.
+template <int KER> constexpr int max_tpl() { return Ker<KER>::K1 <= 3 ? 4 : (Ker<KER>::K1 <= 4 ? 2 : 1); }
 
 // Words per packed source record: position, normal (if any), density.
 template <int KER> constexpr int src_words() { return 3 + Ker<KER>::ND + Ker<KER>::K0; }
@@ -60,9 +107,15 @@ template <int NEWTON, class T> __device__ __forceinline__ T rsq_deferred(T x) {
 }
 
 // Factor that turns the accumulated (unscaled) sum into the kernel value:
-// newton_c2^POW for single-power kernels, 1 for Stokes FxU (scaled in-loop).
+// newton_c2^POW for single-power kernels, 1 for the others (scaled in-loop).
 template <int KER, int NEWTON> __host__ __device__ constexpr double fold_factor() {
   return Ker<KER>::POW ? ipow(newton_c2(NEWTON), Ker<KER>::POW) : 1.0;
+}
+inline double fold_factor(int ker, int newton) {
+  return with_kernel(ker, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    return newton == 0 ? fold_factor<K, 0>() : (newton == 1 ? fold_factor<K, 1>() : fold_factor<K, 2>());
+  });
 }
 
 // acc[K1] += K(r, n) * f  (fused apply, unscaled).  r = x_t - x_s.
@@ -85,7 +138,7 @@ __device__ __forceinline__ void interact(T* acc, T rx, T ry, T rz, const T* n, c
     acc[0] = fmad(f[0], z, fmad(rx, fr, acc[0]));
     acc[1] = fmad(f[1], z, fmad(ry, fr, acc[1]));
     acc[2] = fmad(f[2], z, fmad(rz, fr, acc[2]));
-  } else {  // STK_DXU
+  } else if constexpr (KER == STK_DXU) {
     const T z2 = z * z;
     const T z5 = z2 * z2 * z;
     const T rn5 = fmad(rz, n[2], fmad(ry, n[1], rx * n[0])) * z5;
@@ -93,17 +146,86 @@ __device__ __forceinline__ void interact(T* acc, T rx, T ry, T rz, const T* n, c
     acc[0] = fmad(rx, q, acc[0]);
     acc[1] = fmad(ry, q, acc[1]);
     acc[2] = fmad(rz, q, acc[2]);
+  } else {
+    static_assert(KER >= LAP_FXDU && KER < NUM_KERNELS, "unknown kernel");
+    const T r[3] = {rx, ry, rz};
+    if constexpr (Ker<KER>::POW == 0 && NEWTON > 0) z = z * (T)newton_c2(NEWTON);
+    if constexpr (KER == LAP_FXDU) {  // f r / r^3
+      const T g = f[0] * (z * z * z);
+#pragma unroll
+      for (int i = 0; i < 3; i++) acc[i] = fmad(r[i], g, acc[i]);
+    } else if constexpr (KER == LAP_DXDU) {  // f (n / r^3 - 3 (r.n) r / r^5)
+      const T z2 = z * z;
+      const T g = f[0] * (z2 * z);
+      const T t = (T)3 * fmad(rz, n[2], fmad(ry, n[1], rx * n[0])) * z2;
+#pragma unroll
+      for (int i = 0; i < 3; i++) acc[i] = fmad(g, fmad(-t, r[i], n[i]), acc[i]);
+    } else if constexpr (KER == LAP_FXD2U) {  // f (3 r r^T / r^5 - I / r^3)
+      const T z2 = z * z;
+      const T g = f[0] * (z2 * z);
+      const T t = (T)3 * g * z2;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const T a = t * r[i];
+#pragma unroll
+        for (int j = 0; j < 3; j++) acc[i * 3 + j] = fmad(a, r[j], acc[i * 3 + j]);
+        acc[i * 3 + i] -= g;
+      }
+    } else if constexpr (KER == STK_FXT) {  // r (x) r (r.f) / r^5
+      const T z2 = z * z;
+      const T q = fmad(f[2], rz, fmad(f[1], ry, f[0] * rx)) * (z2 * z2 * z);
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const T a = r[i] * q;
+#pragma unroll
+        for (int j = 0; j < 3; j++) acc[i * 3 + j] = fmad(a, r[j], acc[i * 3 + j]);
+      }
+    } else if constexpr (KER == STK_FSXU) {  // f/r + r (r.f + f3) / r^3
+      const T t = (fmad(f[2], rz, fmad(f[1], ry, f[0] * rx)) + f[3]) * (z * z * z);
+#pragma unroll
+      for (int i = 0; i < 3; i++) acc[i] = fmad(f[i], z, fmad(r[i], t, acc[i]));
+    } else if constexpr (KER == STK_FXUP) {  // [f/r + r (r.f) / r^3, (r.f) / r^3]
+      const T t = fmad(f[2], rz, fmad(f[1], ry, f[0] * rx)) * (z * z * z);
+#pragma unroll
+      for (int i = 0; i < 3; i++) acc[i] = fmad(f[i], z, fmad(r[i], t, acc[i]));
+      acc[3] += t;
+    } else if constexpr (KER == BS_FXU) {  // (f x r) / r^3
+      const T z3 = z * z * z;
+      acc[0] = fmad(fmad(f[1], rz, -ry * f[2]), z3, acc[0]);
+      acc[1] = fmad(fmad(f[2], rx, -rz * f[0]), z3, acc[1]);
+      acc[2] = fmad(fmad(f[0], ry, -rx * f[1]), z3, acc[2]);
+    } else {  // BS_FXDU, output i*3+j: 3 (f x r)_i r_j / r^5 - (f x e_j)_i / r^3
+      const T z2 = z * z;
+      const T z3 = z2 * z;
+      const T t = (T)3 * z3 * z2;
+      const T c[3] = {fmad(f[1], rz, -ry * f[2]), fmad(f[2], rx, -rz * f[0]), fmad(f[0], ry, -rx * f[1])};
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const T a = t * c[i];
+#pragma unroll
+        for (int j = 0; j < 3; j++) acc[i * 3 + j] = fmad(a, r[j], acc[i * 3 + j]);
+      }
+      acc[1] = fmad(z3, f[2], acc[1]);   // -(f x e_1)_0 =  f2
+      acc[2] = fmad(-z3, f[1], acc[2]);  // -(f x e_2)_0 = -f1
+      acc[3] = fmad(-z3, f[2], acc[3]);  // -(f x e_0)_1 = -f2
+      acc[5] = fmad(z3, f[0], acc[5]);   // -(f x e_2)_1 =  f0
+      acc[6] = fmad(z3, f[1], acc[6]);   // -(f x e_0)_2 =  f1
+      acc[7] = fmad(-z3, f[0], acc[7]);  // -(f x e_1)_2 = -f0
+    }
   }
 }
 
 // Matrix form (uKerMatrix): u[k0][k1], unscaled, masked; kernel_functions.hpp
-// :43-47, :72-78, :136-145, :191-201.  Used by KernelMatrix.
+// :43-47, :72-78, :92-99, :136-145, :191-201, :231-243, :275-288, :336-348,
+// :377-389, :403-413, :427-434, :462-486.  Used by KernelMatrix.
 template <int KER, int NEWTON>
-__device__ __forceinline__ void kmatrix(double (&u)[3][3], double rx, double ry, double rz, const double* n) {
+__device__ __forceinline__ void kmatrix(double (&u)[Ker<KER>::K0][Ker<KER>::K1], double rx, double ry, double rz,
+                                        const double* n) {
   const double r2 = fmad(rz, rz, fmad(ry, ry, rx * rx));
   double z = rsq_deferred<NEWTON>(r2) * newton_c2(NEWTON);
   z = (r2 > 0.0) ? z : 0.0;
   const double r[3] = {rx, ry, rz};
+  const double z2 = z * z, z3 = z2 * z, z5 = z3 * z2;
   if constexpr (KER == LAP_FXU) {
     u[0][0] = z;
   } else if constexpr (KER == LAP_DXU) {
@@ -115,13 +237,65 @@ __device__ __forceinline__ void kmatrix(double (&u)[3][3], double rx, double ry,
     for (int i = 0; i < 3; i++)
 #pragma unroll
       for (int j = 0; j < 3; j++) u[i][j] = (i == j ? z : 0.0) + r[i] * r[j] * z3;
-  } else {
-    const double z2 = z * z;
+  } else if constexpr (KER == STK_DXU) {
     const double rn5 = fmad(rz, n[2], fmad(ry, n[1], rx * n[0])) * (z2 * z2 * z);
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
       for (int j = 0; j < 3; j++) u[i][j] = r[i] * r[j] * rn5;
+  } else if constexpr (KER == LAP_FXDU) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) u[0][i] = r[i] * z3;
+  } else if constexpr (KER == LAP_DXDU) {
+    const double t5 = 3.0 * fmad(rz, n[2], fmad(ry, n[1], rx * n[0])) * z5;
+#pragma unroll
+    for (int i = 0; i < 3; i++) u[0][i] = fmad(-r[i], t5, n[i] * z3);
+  } else if constexpr (KER == LAP_FXD2U) {
+    const double t5 = 3.0 * z5;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) u[0][i * 3 + j] = t5 * r[i] * r[j] - (i == j ? z3 : 0.0);
+  } else if constexpr (KER == STK_FXT) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) u[i][j * 3 + k] = r[i] * r[j] * r[k] * z5;
+  } else if constexpr (KER == STK_FSXU || KER == STK_FXUP) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) u[i][j] = (i == j ? z : 0.0) + r[i] * r[j] * z3;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if constexpr (KER == STK_FSXU) u[3][j] = r[j] * z3;
+      else u[j][3] = r[j] * z3;
+    }
+  } else if constexpr (KER == BS_FXU) {
+    u[0][0] = 0.0;
+    u[0][1] = -r[2] * z3;
+    u[0][2] = r[1] * z3;
+    u[1][0] = r[2] * z3;
+    u[1][1] = 0.0;
+    u[1][2] = -r[0] * z3;
+    u[2][0] = -r[1] * z3;
+    u[2][1] = r[0] * z3;
+    u[2][2] = 0.0;
+  } else {  // BS_FXDU: source component k, column i*3+j = d/dr_j of (e_k x r)_i / r^3
+    const double t5 = 3.0 * z5;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          // (e_k x r)_i = eps(k, m, i) r_m with m the third index
+          const int m = 3 - k - i;
+          const double eps = (k == i) ? 0.0 : (((k + 1) % 3 == m) ? 1.0 : -1.0);
+          u[k][i * 3 + j] = (k == i) ? 0.0 : eps * (t5 * r[m] * r[j] - (m == j ? z3 : 0.0));
+        }
   }
 }
 

@@ -41,13 +41,16 @@ constexpr float PADF = 1e18f;
 // Record layout (floats) per source; far-path words first.
 //   LAP_FXU  [s'x s'y s'z S | q s''x s''y s''z]                         S = |s'|^2
 //   LAP_DXU  [s'x s'y s'z S | nq_x nq_y nq_z -s'.nq | s''x s''y s''z 0]   nq = q n
+//   others   [s'x s'y s'z | f_0 .. f_{K0-1} | n (if any) | s''x s''y s''z], padded
+//            to whole float4s, e.g.
 //   STK_FXU  [s'x s'y s'z f0 | f1 f2 s''x s''y | s''z 0 0 0]
 //   STK_DXU  [s'x s'y s'z f0 | f1 f2 n_x n_y | n_z s''x s''y s''z]
-template <int KER> struct MixLayout;
+// FOFF / NOFF / LO: word offsets of the densities, the normal and s''.
+template <int KER> struct MixLayout {
+  static constexpr int FOFF = 3, NOFF = 3 + Ker<KER>::K0, LO = NOFF + Ker<KER>::ND, Q = (LO + 3 + 3) / 4;
+};
 template <> struct MixLayout<LAP_FXU> { static constexpr int Q = 2, LO = 5; };
 template <> struct MixLayout<LAP_DXU> { static constexpr int Q = 3, LO = 8; };
-template <> struct MixLayout<STK_FXU> { static constexpr int Q = 3, LO = 6; };
-template <> struct MixLayout<STK_DXU> { static constexpr int Q = 3, LO = 9; };
 
 template <int KER>
 __device__ __forceinline__ void fill_record(float* r, const double* d, const double* n, const double* f) {
@@ -73,18 +76,12 @@ __device__ __forceinline__ void fill_record(float* r, const double* d, const dou
     r[6] = nq[2];
     r[7] = (float)(-((double)hi[0] * nq[0] + (double)hi[1] * nq[1] + (double)hi[2] * nq[2]));
     r[11] = 0.0f;
-  } else if constexpr (KER == STK_FXU) {
-    r[3] = (float)f[0];
-    r[4] = (float)f[1];
-    r[5] = (float)f[2];
-    r[9] = r[10] = r[11] = 0.0f;
   } else {
-    r[3] = (float)f[0];
-    r[4] = (float)f[1];
-    r[5] = (float)f[2];
-    r[6] = (float)n[0];
-    r[7] = (float)n[1];
-    r[8] = (float)n[2];
+    using L = MixLayout<KER>;
+#pragma unroll
+    for (int k = 0; k < Ker<KER>::K0; k++) r[L::FOFF + k] = (float)f[k];
+#pragma unroll
+    for (int k = 0; k < Ker<KER>::ND; k++) r[L::NOFF + k] = (float)n[k];
   }
   r[LO + 0] = lo[0];
   r[LO + 1] = lo[1];
@@ -102,12 +99,9 @@ __device__ __forceinline__ void mix_far(float* p, const float* th, const float* 
     const float rnq = fmad(th[2], r[6], fmad(th[1], r[5], fmad(th[0], r[4], r[7])));
     const float z = rsq_deferred<NEWTON>(r2);
     p[0] = fmad(rnq, z * z * z, p[0]);
-  } else if constexpr (KER == STK_FXU) {
-    const float f[3] = {r[3], r[4], r[5]};
-    interact<STK_FXU, NEWTON, false>(p, th[0] - r[0], th[1] - r[1], th[2] - r[2], f, f);
   } else {
-    const float f[3] = {r[3], r[4], r[5]}, n[3] = {r[6], r[7], r[8]};
-    interact<STK_DXU, NEWTON, false>(p, th[0] - r[0], th[1] - r[1], th[2] - r[2], n, f);
+    using L = MixLayout<KER>;
+    interact<KER, NEWTON, false>(p, th[0] - r[0], th[1] - r[1], th[2] - r[2], r + L::NOFF, r + L::FOFF);
   }
 }
 
@@ -124,12 +118,9 @@ __device__ __forceinline__ void mix_near(float* p, const float* th, const float*
   } else if constexpr (KER == LAP_DXU) {
     const float one[1] = {1.0f}, nq[3] = {r[4], r[5], r[6]};
     interact<LAP_DXU, NEWTON, true>(p, dx, dy, dz, nq, one);
-  } else if constexpr (KER == STK_FXU) {
-    const float f[3] = {r[3], r[4], r[5]};
-    interact<STK_FXU, NEWTON, true>(p, dx, dy, dz, f, f);
   } else {
-    const float f[3] = {r[3], r[4], r[5]}, n[3] = {r[6], r[7], r[8]};
-    interact<STK_DXU, NEWTON, true>(p, dx, dy, dz, n, f);
+    using L = MixLayout<KER>;
+    interact<KER, NEWTON, true>(p, dx, dy, dz, r + L::NOFF, r + L::FOFF);
   }
 }
 
@@ -174,7 +165,7 @@ k_pack_tiles(const double* __restrict__ xs, const double* __restrict__ xn, const
   float h2 = 0.0f;
   if (ok) {
     const double d[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
-    double n[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+    double n[3] = {0, 0, 0}, f[KMAX0] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < ND; k++) n[k] = xn[src * 3 + k];
 #pragma unroll
@@ -512,11 +503,10 @@ constexpr int TPLS[3] = {1, 2, 4};
 int tpl_index(int tpl) { return tpl >= 4 ? 2 : (tpl >= 2 ? 1 : 0); }
 
 template <int KER, int NEWTON> EvalMixFn mix_fn(int ti) {
-  switch (ti) {
-    case 0: return k_eval_mixed<KER, 1, NEWTON>;
-    case 1: return k_eval_mixed<KER, 2, NEWTON>;
-    default: return k_eval_mixed<KER, 4, NEWTON>;
-  }
+  constexpr int MT = max_tpl<KER>();
+  if (ti >= 2 && MT >= 4) return k_eval_mixed<KER, (MT >= 4 ? 4 : 1), NEWTON>;
+  if (ti >= 1 && MT >= 2) return k_eval_mixed<KER, (MT >= 2 ? 2 : 1), NEWTON>;
+  return k_eval_mixed<KER, 1, NEWTON>;
 }
 template <int KER, int NEWTON> EvalMixFn mfma_fn(int ti) {
   switch (ti) {
@@ -525,42 +515,23 @@ template <int KER, int NEWTON> EvalMixFn mfma_fn(int ti) {
     default: return k_eval_mixed_mfma<KER, 4, NEWTON>;
   }
 }
-template <int KER> EvalMixFn mix_fn(int newton, int ti) { return newton ? mix_fn<KER, 1>(ti) : mix_fn<KER, 0>(ti); }
-template <int KER> EvalMixFn mfma_fn(int newton, int ti) {
-  return newton ? mfma_fn<KER, 1>(ti) : mfma_fn<KER, 0>(ti);
-}
 EvalMixFn mix_fn(int ker, int newton, int ti, bool mfma) {
-  switch (ker) {
-    case LAP_FXU: return mfma ? mfma_fn<LAP_FXU>(newton, ti) : mix_fn<LAP_FXU>(newton, ti);
-    case LAP_DXU: return mfma ? mfma_fn<LAP_DXU>(newton, ti) : mix_fn<LAP_DXU>(newton, ti);
-    case STK_FXU: return mix_fn<STK_FXU>(newton, ti);
-    default: return mix_fn<STK_DXU>(newton, ti);
-  }
+  return with_kernel(ker, [&](auto kc) -> EvalMixFn {
+    constexpr int K = decltype(kc)::value;
+    if constexpr (K == LAP_FXU || K == LAP_DXU) {
+      if (mfma) return newton ? mfma_fn<K, 1>(ti) : mfma_fn<K, 0>(ti);
+    }
+    return newton ? mix_fn<K, 1>(ti) : mix_fn<K, 0>(ti);
+  });
 }
 PackFn pack_fn(int ker) {
-  switch (ker) {
-    case LAP_FXU: return k_pack_tiles<LAP_FXU>;
-    case LAP_DXU: return k_pack_tiles<LAP_DXU>;
-    case STK_FXU: return k_pack_tiles<STK_FXU>;
-    default: return k_pack_tiles<STK_DXU>;
-  }
+  return with_kernel(ker, [](auto kc) -> PackFn { return k_pack_tiles<decltype(kc)::value>; });
 }
 int layout_q(int ker) {
-  switch (ker) {
-    case LAP_FXU: return MixLayout<LAP_FXU>::Q;
-    case LAP_DXU: return MixLayout<LAP_DXU>::Q;
-    case STK_FXU: return MixLayout<STK_FXU>::Q;
-    default: return MixLayout<STK_DXU>::Q;
-  }
+  return with_kernel(ker, [](auto kc) { return MixLayout<decltype(kc)::value>::Q; });
 }
-double mixed_fold(int ker, int newton) {
-  if (!newton) return 1.0;
-  switch (ker) {
-    case LAP_FXU: return fold_factor<LAP_FXU, 1>();
-    case LAP_DXU: return fold_factor<LAP_DXU, 1>();
-    case STK_FXU: return fold_factor<STK_FXU, 1>();
-    default: return fold_factor<STK_DXU, 1>();
-  }
+int max_tpl_index(int ker) {
+  return with_kernel(ker, [](auto kc) { return tpl_index(max_tpl<decltype(kc)::value>()); });
 }
 
 }  // namespace
@@ -574,7 +545,8 @@ int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, cons
   int K0, K1, ND;
   kernel_dims(ker, &K0, &K1, &ND);
   SCTL_CHECK_ARG(ns <= 0xffffffffLL && nt <= 0xffffffffLL, "mixed precision supports at most 2^32-1 points");
-  const int ti = tpl_index(shape.tpl), tpl = TPLS[ti];
+  const bool mfma = shape.mfma && mfma_supported(ker);
+  const int ti = mfma ? tpl_index(shape.tpl) : std::min(tpl_index(shape.tpl), max_tpl_index(ker)), tpl = TPLS[ti];
   const int64_t tb = (nt + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
   SCTL_CHECK_ARG(tb <= 0x7fffffffLL, "too many targets for one launch");
   const int64_t ntiles = (ns + TILE - 1) / TILE;
@@ -600,8 +572,8 @@ int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, cons
   const int64_t tps = (ntiles + nsplit - 1) / nsplit;
   nsplit = (ntiles + tps - 1) / tps;
 
-  const double scale = kernel_scale(ker) * mixed_fold(ker, newton);
-  const EvalMixFn fn = mix_fn(ker, newton, ti, shape.mfma && mfma_supported(ker));
+  const double scale = kernel_scale(ker) * fold_factor(ker, newton ? 1 : 0);
+  const EvalMixFn fn = mix_fn(ker, newton, ti, mfma);
   size_t dyn = 0;
   SCTL_TRY(occupancy_lds((const void*)fn, shape.occ, sizeof(float4) * TILE * Q, &dyn));
   Scratch part;

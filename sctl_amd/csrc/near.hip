@@ -78,7 +78,8 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
               double* __restrict__ K, double scale) {
   using Kt = Ker<KER>;
   constexpr int K0 = Kt::K0, K1 = Kt::K1, ND = Kt::ND;
-  constexpr int DOF = NDOT ? 1 : K1;
+  constexpr int DOF = NDOT ? K1 / 3 : K1;
+  static_assert(!NDOT || K1 % 3 == 0, "the target-normal contraction needs 3 outputs per row");
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (NWG / 64);
   for (int64_t tile = (int64_t)blockIdx.x * (NWG / 64) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
@@ -106,20 +107,23 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
       const double ws = wts ? wts[gs] : 1.0;
       double* Kc = Ks + (int64_t)s * K0;
       for (int t = 0; t < nts; t++) {
-        double u[3][3];
+        double u[K0][K1];
         kmatrix<KER, NEAR_NEWTON>(u, readlane_d(mx, t) - sx, readlane_d(my, t) - sy, readlane_d(mz, t) - sz, n);
         if (!active) continue;
         if constexpr (NDOT) {
           const double n0 = readlane_d(mn[0], t), n1 = readlane_d(mn[1], t), n2 = readlane_d(mn[2], t);
-          double* row = Kc + (int64_t)t * ld;
 #pragma unroll
-          for (int k0 = 0; k0 < K0; k0++) {
-            double v = 0.0;
-            v += (u[k0][0] * scale) * ws * n0;
-            v += (u[k0][1] * scale) * ws * n1;
-            v += (u[k0][2] * scale) * ws * n2;
-            if constexpr (SUB) row[k0] -= v;
-            else row[k0] = v;
+          for (int k1 = 0; k1 < DOF; k1++) {
+            double* row = Kc + (int64_t)(t * DOF + k1) * ld;
+#pragma unroll
+            for (int k0 = 0; k0 < K0; k0++) {
+              double v = 0.0;
+              v += (u[k0][k1 * 3 + 0] * scale) * ws * n0;
+              v += (u[k0][k1 * 3 + 1] * scale) * ws * n1;
+              v += (u[k0][k1 * 3 + 2] * scale) * ws * n2;
+              if constexpr (SUB) row[k0] -= v;
+              else row[k0] = v;
+            }
           }
         } else {
 #pragma unroll
@@ -224,12 +228,13 @@ template <int KER, bool NDOT> NearMatFn near_mat_fn(bool sub) {
   return sub ? k_near_matrix<KER, NDOT, true> : k_near_matrix<KER, NDOT, false>;
 }
 NearMatFn near_mat_fn(int ker, bool ndot, bool sub) {
-  switch (ker) {
-    case LAP_FXU: return near_mat_fn<LAP_FXU, false>(sub);
-    case LAP_DXU: return near_mat_fn<LAP_DXU, false>(sub);
-    case STK_FXU: return ndot ? near_mat_fn<STK_FXU, true>(sub) : near_mat_fn<STK_FXU, false>(sub);
-    default: return ndot ? near_mat_fn<STK_DXU, true>(sub) : near_mat_fn<STK_DXU, false>(sub);
-  }
+  return with_kernel(ker, [&](auto kc) -> NearMatFn {
+    constexpr int K = decltype(kc)::value;
+    if constexpr (Ker<K>::K1 % 3 == 0) {
+      if (ndot) return near_mat_fn<K, true>(sub);
+    }
+    return near_mat_fn<K, false>(sub);
+  });
 }
 
 // Flattens per-block tile counts into a device tile list, blocks in
@@ -355,7 +360,7 @@ int sctl_amd_near_blocks(int64_t nelem, const int64_t* near_elem_cnt, const int6
 int sctl_amd_plan_near_layout(sctl_amd_plan p, const int64_t* blk_off, int trg_normal_dot, int64_t* k_len) {
   SCTL_CHECK_ARG(p, "plan is NULL");
   SCTL_CHECK_ARG(!trg_normal_dot || p->K1 % 3 == 0,
-                 "the target-normal contraction needs a kernel with 3 outputs per target (Stokes)");
+                 "the target-normal contraction needs a kernel whose outputs per target are a multiple of 3");
   const int dof = trg_normal_dot ? p->K1 / 3 : p->K1;
   const int64_t nb = p->nblocks;
   std::vector<int64_t> off(nb), size(nb);
@@ -455,7 +460,7 @@ int sctl_amd_plan_near_gemv(sctl_amd_plan p, const double* K, const double* F, d
 int sctl_amd_near_accumulate(int64_t ntrg, int dof, const int64_t* near_trg_cnt, const int64_t* near_trg_dsp,
                              int64_t n_scatter, const int64_t* scatter_index, int64_t n_near, const double* U_near,
                              double* U, uint32_t flags, void* stream) {
-  SCTL_CHECK_ARG(dof == 1 || dof == 3, "dof must be 1 or 3");
+  SCTL_CHECK_ARG(dof == 1 || dof == 3 || dof == 4 || dof == 9, "dof must be 1, 3, 4 or 9");
   SCTL_CHECK_ARG(ntrg >= 0 && n_scatter >= 0 && n_near >= 0, "negative size");
   SCTL_CHECK_ARG(ntrg == 0 || (near_trg_cnt && near_trg_dsp && U), "NULL target array");
   SCTL_CHECK_ARG(n_scatter == 0 || scatter_index, "scatter_index is NULL");
@@ -467,12 +472,9 @@ int sctl_amd_near_accumulate(int64_t ntrg, int dof, const int64_t* near_trg_cnt,
   SCTL_CHECK_ARG(grid <= 0x7fffffffLL, "too many targets for one launch");
   auto launch = [&](const int64_t* cnt, const int64_t* dsp, const int64_t* sc, const double* un, double* u) -> int {
     ProfileScope prof(s);
-    if (dof == 1)
-      hipLaunchKernelGGL(k_near_accumulate<1>, dim3((unsigned)grid), dim3(NWG), 0, s, ntrg, cnt, dsp, n_scatter, sc,
-                         n_near, un, u);
-    else
-      hipLaunchKernelGGL(k_near_accumulate<3>, dim3((unsigned)grid), dim3(NWG), 0, s, ntrg, cnt, dsp, n_scatter, sc,
-                         n_near, un, u);
+    auto fn = dof == 1 ? k_near_accumulate<1>
+                       : (dof == 3 ? k_near_accumulate<3> : (dof == 4 ? k_near_accumulate<4> : k_near_accumulate<9>));
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(NWG), 0, s, ntrg, cnt, dsp, n_scatter, sc, n_near, un, u);
     SCTL_HIP_TRY(hipGetLastError());
     return SCTL_AMD_SUCCESS;
   };

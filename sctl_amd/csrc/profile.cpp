@@ -10,8 +10,10 @@
 namespace sctl_amd {
 
 namespace {
-// FLOPS() of kernel_functions.hpp:37,66,130,185.
-constexpr int64_t REF_FLOPS[4] = {6, 14, 23, 26};
+// FLOPS() of kernel_functions.hpp:37,66,86,130,185,225,269,330,371,397,421,456
+// (kernel ids 0..11).
+constexpr int64_t REF_FLOPS[] = {6, 14, 23, 26, 11, 22, 25, 39, 26, 26, 15, 51};
+constexpr int N_REF_FLOPS = (int)(sizeof(REF_FLOPS) / sizeof(REF_FLOPS[0]));
 std::atomic<int64_t> g_inter{0}, g_flops{0};
 std::atomic<bool> g_timing{false};
 std::mutex g_mu;
@@ -22,7 +24,7 @@ double g_ms = 0;
 
 void profile_count(int ker, int64_t interactions) {
   g_inter += interactions;
-  if (ker >= 0 && ker < 4) g_flops += interactions * REF_FLOPS[ker];
+  if (ker >= 0 && ker < N_REF_FLOPS) g_flops += interactions * REF_FLOPS[ker];
 }
 
 ProfileScope::ProfileScope(hipStream_t s) : s_(s) {

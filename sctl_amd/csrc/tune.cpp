@@ -278,6 +278,7 @@ int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_tr
   for (int mfma = 0; mfma <= (with_mfma ? 1 : 0); mfma++) {
     for (int tpl : {1, 2, 4}) {
       if (mfma && tpl == 4) continue;  // 16 row tiles per wave do not fit the register file
+      if (!mfma && tpl > kernel_max_tpl(kernel)) continue;
       for (int occ : {0, 3, 2}) {
         DenseShape sh;
         sh.tpl = tpl;

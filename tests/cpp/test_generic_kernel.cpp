@@ -133,6 +133,14 @@ int main() {
   eval_vs_matrix<sctl_amd::Laplace3D_DxU>(1, 1, 3);
   eval_vs_matrix<sctl_amd::Stokes3D_FxU>(3, 3, 0);
   eval_vs_matrix<sctl_amd::Stokes3D_DxU>(3, 3, 3);
+  eval_vs_matrix<sctl_amd::Laplace3D_FxdU>(1, 3, 0);
+  eval_vs_matrix<sctl_amd::Laplace3D_DxdU>(1, 3, 3);
+  eval_vs_matrix<sctl_amd::Laplace3D_Fxd2U>(1, 9, 0);
+  eval_vs_matrix<sctl_amd::Stokes3D_FxT>(3, 9, 0);
+  eval_vs_matrix<sctl_amd::Stokes3D_FSxU>(4, 3, 0);
+  eval_vs_matrix<sctl_amd::Stokes3D_FxUP>(3, 4, 0);
+  eval_vs_matrix<sctl_amd::BiotSavart3D_FxU>(3, 3, 0);
+  eval_vs_matrix<sctl_amd::BiotSavart3D_FxdU>(3, 9, 0);
 
   std::printf("\n%s (%d check(s) failed)\n", failures == 0 ? "PASS" : "FAIL", failures);
   return failures == 0 ? 0 : 1;

@@ -35,7 +35,7 @@ def test_dropin_rejects_other_kernels():
         src = os.path.join(d, "reject.cpp")
         open(src, "w").write(code)
         r = _syntax(src)
-    assert r.returncode != 0 and "only Laplace3D_FxU/DxU and Stokes3D_FxU/DxU" in r.stderr
+    assert r.returncode != 0 and "only the sctl_amd::kernel_impl micro-kernels run on the device" in r.stderr
 
 
 @pytest.mark.gpu

@@ -36,14 +36,21 @@ def rand_case(rng, k, nt, ns, coincide=0):
     return xt, xs, n, v
 
 
-KERNELS = [0, 1, 2, 3]
-IDS = ["LaplaceFxU", "LaplaceDxU", "StokesFxU", "StokesDxU"]
+KERNELS = list(range(12))
+IDS = ["LaplaceFxU", "LaplaceDxU", "StokesFxU", "StokesDxU", "LaplaceFxdU", "LaplaceDxdU", "LaplaceFxd2U",
+       "StokesFxT", "StokesFSxU", "StokesFxUP", "BiotSavartFxU", "BiotSavartFxdU"]
+
+
+def golden_for(k):
+    """The reference's outputs for kernel k (the first four kernels and the
+    other eight are in two fixture files with the same keys)."""
+    return pins.golden() if k < 4 else pins.golden_more()
 
 
 @pytest.mark.parametrize("k", KERNELS, ids=IDS)
 def test_eval_fp64_matches_reference_golden(gpu, k):
     """Against the reference's own scalar-build outputs (exact 1/sqrt)."""
-    G = pins.golden()
+    G = golden_for(k)
     _, K0, K1, ND, _ = oracle.dims(k)
     Xt, Xs, Xn, Vs = pins.small_inputs(K0, ND)
     assert nerr(gpu.eval(k, Xt, Xs, Xn, Vs), G["k%d_small_eval_scalar" % k]) < FP64_TOL
@@ -71,7 +78,7 @@ def test_eval_fp64_matches_oracle(gpu, k, nt, ns, coincide):
 
 @pytest.mark.parametrize("k", KERNELS, ids=IDS)
 def test_eval_fp64_config1_size(gpu, k):
-    """The 20,000 x 20,000 uniform-cube case (seed 0), all four kernels."""
+    """The 20,000 x 20,000 uniform-cube case (seed 0), every kernel."""
     _, K0, K1, ND, _ = oracle.dims(k)
     xt, xs, f = W.cube(20000, 20000, seed=0, k0=K0)
     n = W.normal(0, 3, 20000 * 3) if ND else None
@@ -104,7 +111,7 @@ def test_eval_mixed_matches_oracle(gpu, k, newton, nt, ns, coincide):
     assert e_inf < MIXED_TOL and e_rms < MIXED_TOL
 
 
-@pytest.mark.parametrize("k", [2, 3], ids=["StokesFxU", "StokesDxU"])
+@pytest.mark.parametrize("k", KERNELS[2:], ids=IDS[2:])
 def test_eval_mixed_clusters(gpu, k):
     """Gaussian clusters (sigma 0.02): the case that needs tile centring."""
     _, K0, K1, ND, _ = oracle.dims(k)
@@ -162,7 +169,7 @@ def test_python_generic_kernel_mirror(gpu):
 
 @pytest.mark.parametrize("k", KERNELS, ids=IDS)
 def test_kernel_matrix_matches_reference(gpu, k):
-    G = pins.golden()
+    G = golden_for(k)
     _, K0, K1, ND, _ = oracle.dims(k)
     Xt, Xs, Xn, Vs = pins.small_inputs(K0, ND)
     M = gpu.kernel_matrix(k, Xt, Xs, Xn)

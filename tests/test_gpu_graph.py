@@ -21,41 +21,45 @@ def test_device_generator_matches_host(gpu):
         assert np.allclose(got, want, rtol=0, atol=1e-13 * max(1.0, np.abs(want).max()))
 
 
+@pytest.mark.parametrize("k", [3, 8], ids=["StokesDxU", "StokesFSxU"])
 @pytest.mark.parametrize("prec,tol", [(0, 1e-13), (1, 1e-6)], ids=["fp64", "mixed"])
-def test_graph_replays_match_eager(gpu, prec, tol):
-    k = 3  # Stokes DxU: 3-vector densities, normals
-    B = W.torus_blocks(8, 8, seed=1, k0=3, ns_range=(64, 1024), nt_range=(16, 300))
+def test_graph_replays_match_eager(gpu, k, prec, tol):
+    """Stokes DxU (3-vector densities, normals) and Stokes FSxU (4 densities
+    per source)."""
+    _, K0, K1, ND, _ = oracle.dims(k)
+    B = W.torus_blocks(8, 8, seed=1, k0=K0, ns_range=(64, 1024), nt_range=(16, 300))
     nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
     P = gpu.Plan(k, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=prec)
-    P.set_geometry(B["r_trg"], B["r_src"], B["n_src"])
+    P.set_geometry(B["r_trg"], B["r_src"], B["n_src"] if ND else None)
     nsteps, seed = 5, 100
     # eager: the same steps one call at a time
-    v_src = gpu.DeviceArray(ns * 3)
-    eager = gpu.DeviceArray.from_host(np.zeros(nt * 3))
+    v_src = gpu.DeviceArray(ns * K0)
+    eager = gpu.DeviceArray.from_host(np.zeros(nt * K1))
     for s in range(nsteps):
         gpu.gen_normal(seed + s, DENS_STREAM, v_src)
         P.apply(v_src, eager)
     ref = eager.download()
     # the oracle on the summed densities (linearity), per block
-    dens = sum(W.normal(seed + s, DENS_STREAM, ns * 3) for s in range(nsteps))
-    want = np.zeros(nt * 3)
+    dens = sum(W.normal(seed + s, DENS_STREAM, ns * K0) for s in range(nsteps))
+    want = np.zeros(nt * K1)
     for b in range(B["trg_cnt"].size):
         t0, tc, s0, sc = (int(B[a][b]) for a in ("trg_dsp", "trg_cnt", "src_dsp", "src_cnt"))
-        want[3 * t0:3 * (t0 + tc)] = oracle.eval(k, B["r_trg"][3 * t0:3 * (t0 + tc)], B["r_src"][3 * s0:3 * (s0 + sc)],
-                                                 B["n_src"][3 * s0:3 * (s0 + sc)], dens[3 * s0:3 * (s0 + sc)])
+        want[K1 * t0:K1 * (t0 + tc)] = oracle.eval(
+            k, B["r_trg"][3 * t0:3 * (t0 + tc)], B["r_src"][3 * s0:3 * (s0 + sc)],
+            B["n_src"][3 * s0:3 * (s0 + sc)] if ND else None, dens[K0 * s0:K0 * (s0 + sc)])
     assert W.rel_errors(ref, want)[0] < tol
     # graph: replay several times, eager work in between
-    v_trg = gpu.DeviceArray.from_host(np.zeros(nt * 3))
+    v_trg = gpu.DeviceArray.from_host(np.zeros(nt * K1))
     G = P.graph(nsteps, seed, DENS_STREAM, v_src, v_trg)
     assert G.interactions() == nsteps * P.interactions()
     for rep in range(3):
-        v_trg.upload(np.zeros(nt * 3))
+        v_trg.upload(np.zeros(nt * K1))
         G.launch()
         gpu.synchronize()
         got = v_trg.download()
         assert W.rel_errors(got, ref)[0] < (1e-15 if prec == 0 else 1e-12), "replay %d" % rep
         # eager work between replays
-        tmp = gpu.DeviceArray.from_host(np.zeros(nt * 3))
+        tmp = gpu.DeviceArray.from_host(np.zeros(nt * K1))
         P.apply(v_src, tmp)
         gpu.synchronize()
     # accumulation across two launches without clearing

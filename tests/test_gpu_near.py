@@ -17,8 +17,12 @@ from sctl_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-KERNEL_CASES = [(0, False), (1, False), (2, False), (3, False), (2, True), (3, True)]
-IDS = ["LaplaceFxU", "LaplaceDxU", "StokesFxU", "StokesDxU", "StokesFxU-ndot", "StokesDxU-ndot"]
+KERNEL_CASES = [(0, False), (1, False), (2, False), (3, False), (2, True), (3, True),
+                (4, False), (5, False), (5, True), (6, False), (6, True), (7, False), (7, True), (8, False),
+                (9, False), (10, False), (11, True)]
+IDS = ["LaplaceFxU", "LaplaceDxU", "StokesFxU", "StokesDxU", "StokesFxU-ndot", "StokesDxU-ndot",
+       "LaplaceFxdU", "LaplaceDxdU", "LaplaceDxdU-ndot", "LaplaceFxd2U", "LaplaceFxd2U-ndot", "StokesFxT",
+       "StokesFxT-ndot", "StokesFSxU", "StokesFxUP", "BiotSavartFxU", "BiotSavartFxdU-ndot"]
 
 
 def _case(kernel, ndot, seed=1, n_u=6, n_v=5, n_trg=200, matrix_free=None, nthreads=5):
@@ -126,7 +130,7 @@ def test_near_gemv_matches_oracle(gpu, kernel, ndot):
     assert np.all(got[e0:e1] == 0)
 
 
-@pytest.mark.parametrize("dof", [1, 3])
+@pytest.mark.parametrize("dof", [1, 3, 4, 9])
 @pytest.mark.parametrize("device", [False, True], ids=["host", "device"])
 def test_near_accumulate_bit_identical(gpu, dof, device):
     S = gpu
@@ -159,7 +163,8 @@ def test_near_accumulate_rejects_bad_indices(gpu):
         S.near_accumulate(np.zeros(4), 2, [1, 1], [0, 1], [0, 1], np.zeros(4))
 
 
-@pytest.mark.parametrize("kernel,ndot", [(1, False), (2, True)], ids=["LaplaceDxU", "StokesFxU-ndot"])
+@pytest.mark.parametrize("kernel,ndot", [(1, False), (2, True), (6, True), (9, False)],
+                         ids=["LaplaceDxU", "StokesFxU-ndot", "LaplaceFxd2U-ndot", "StokesFxUP"])
 def test_compute_near_interac_end_to_end(gpu, kernel, ndot):
     """SetupNear's subtract then ComputeNearInterac (GEMV, ScatterForward,
     per-target sum) on the device against the oracle's chain of the same
@@ -180,7 +185,7 @@ def test_compute_near_interac_end_to_end(gpu, kernel, ndot):
     assert W.rel_errors(U, U_ref)[0] < 1e-12
 
 
-@pytest.mark.parametrize("kernel", [0, 3], ids=["LaplaceFxU", "StokesDxU"])
+@pytest.mark.parametrize("kernel", [0, 3, 7], ids=["LaplaceFxU", "StokesDxU", "StokesFxT"])
 def test_gemv_of_stored_matrix_equals_weighted_apply(gpu, kernel):
     """A size-independent property that ties the two routes together:
     near_gemv(K = w*M^T, F) == plan.apply(w*F), block by block."""
@@ -217,6 +222,9 @@ def test_near_layout_errors(gpu):
     with pytest.raises(S.SctlAmdError):
         P.near_layout(np.zeros(4, dtype=np.int64))  # overlapping block matrices
     assert P.near_layout() == int(np.sum(B["trg_cnt"] * B["src_cnt"]))
+    P4 = S.Plan(9, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns)
+    with pytest.raises(S.SctlAmdError):
+        P4.near_layout(None, trg_normal_dot=True)  # Stokes FxUP: 4 outputs per target
 
 
 def test_near_empty_blocks(gpu):

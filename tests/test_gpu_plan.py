@@ -30,7 +30,12 @@ def small_torus(k, nu=8, nv=8, seed=0):
     return W.torus_blocks(nu, nv, seed=seed, k0=K0, ns_range=(64, 1024), nt_range=(1, 300))
 
 
-@pytest.mark.parametrize("k", [0, 1, 2, 3], ids=["LaplaceFxU", "LaplaceDxU", "StokesFxU", "StokesDxU"])
+KERNELS = list(range(12))
+IDS = ["LaplaceFxU", "LaplaceDxU", "StokesFxU", "StokesDxU", "LaplaceFxdU", "LaplaceDxdU", "LaplaceFxd2U",
+       "StokesFxT", "StokesFSxU", "StokesFxUP", "BiotSavartFxU", "BiotSavartFxdU"]
+
+
+@pytest.mark.parametrize("k", KERNELS, ids=IDS)
 @pytest.mark.parametrize("prec,tol", [(0, 1e-13), (1, 1e-6)], ids=["fp64", "mixed"])
 def test_plan_matches_per_block_oracle(gpu, k, prec, tol):
     B = small_torus(k)
@@ -61,7 +66,7 @@ def test_plan_ragged_and_empty_blocks(gpu, prec, tol):
     src_dsp = np.array([0, 100, 200, 0, 900, 1000, 1500], dtype=np.int64)
     src_cnt = np.array([100, 1, 700, 10, 33, 1000, 500], dtype=np.int64)
     B = dict(r_trg=xt, r_src=xs, n_src=n, trg_dsp=trg_dsp, trg_cnt=trg_cnt, src_dsp=src_dsp, src_cnt=src_cnt)
-    for k in (1, 2):
+    for k in (1, 2, 5, 8, 11):
         _, K0, K1, ND, _ = oracle.dims(k)
         v = rng.uniform(-1, 1, ns * K0)
         P = gpu.Plan(k, trg_dsp, trg_cnt, src_dsp, src_cnt, nt, ns, precision=prec)

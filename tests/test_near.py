@@ -50,7 +50,8 @@ def test_near_blocks_empty_and_errors():
         S.near_blocks([3, -1], [0, 3], 2)
 
 
-@pytest.mark.parametrize("kernel,ndot", [(0, False), (1, False), (2, False), (3, False), (2, True), (3, True)])
+@pytest.mark.parametrize("kernel,ndot", [(0, False), (1, False), (2, False), (3, False), (2, True), (3, True),
+                                         (5, True), (8, False), (9, False), (11, False)])
 def test_near_offsets_match_reference_layout(kernel, ndot):
     """Block offsets of the product's NearField against K_near_dsp of the
     oracle (boundary_integral.txx:1039-1057, :1187-1189)."""
