@@ -11,7 +11,11 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 The product library never depends on it.
 
 Kernel ids follow the product's C-ABI: 0 Laplace3D_FxU, 1 Laplace3D_DxU,
-2 Stokes3D_FxU, 3 Stokes3D_DxU.
+2 Stokes3D_FxU, 3 Stokes3D_DxU, 4 Laplace3D_FxdU, 5 Laplace3D_DxdU,
+6 Laplace3D_Fxd2U, 7 Stokes3D_FxT, 8 Stokes3D_FSxU, 9 Stokes3D_FxUP,
+10 BiotSavart3D_FxU, 11 BiotSavart3D_FxdU, 12 Helmholtz3D_FxU,
+13 Helmholtz3D_DxU, 14 Helmholtz3D_FxdU, 15 HelmholtzDiff3D_FxdU.  The
+Helmholtz kernels take the wavenumber as ``ctx``.
 """
 import ctypes
 import os
@@ -51,6 +55,13 @@ def _i(a):
     return np.ascontiguousarray(a, dtype=np.int64)
 
 
+def _ctx(ctx):
+    """Pointer to the kernel context (one double: the Helmholtz wavenumber), or NULL."""
+    if ctx is None:
+        return None
+    return ctypes.pointer(ctypes.c_double(float(ctx)))
+
+
 def _ip(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_long)) if a is not None and a.size else None
 
@@ -88,6 +99,8 @@ class Oracle:
         L.oracle_eval.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _L, _dp, _dp, _dp, _L, _dp,
                                   ctypes.c_int]
         L.oracle_kernel_matrix.argtypes = [ctypes.c_int, _dp, _L, _dp, _dp, _L, _dp, ctypes.c_int]
+        L.oracle_eval_ctx.argtypes = L.oracle_eval.argtypes + [_dp]
+        L.oracle_kernel_matrix_ctx.argtypes = L.oracle_kernel_matrix.argtypes + [_dp]
         L.oracle_rsqrt.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _L, _dp]
         L.oracle_dims.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.oracle_scale.argtypes = [ctypes.c_int]
@@ -99,7 +112,7 @@ class Oracle:
         L.oracle_near_blocks.restype = _L
         L.oracle_near_layout.argtypes = [ctypes.c_int, ctypes.c_int, _L, _lp, _lp, _ip, _lp, _lp]
         L.oracle_near_layout.restype = _L
-        L.oracle_near_subtract.argtypes = [ctypes.c_int, ctypes.c_int, _L] + [_lp] * 9 + [_ip] + [_dp] * 6
+        L.oracle_near_subtract.argtypes = [ctypes.c_int, ctypes.c_int, _L] + [_lp] * 9 + [_ip] + [_dp] * 7
         L.oracle_near_gemv.argtypes = [ctypes.c_int, ctypes.c_int, _L] + [_lp] * 8 + [_dp] * 3
         L.oracle_near_accumulate.argtypes = [_L, ctypes.c_int, _lp, _lp, _L, _lp, _dp, _dp]
         L.oracle_near_list.argtypes = [_dp, _L, _dp, _dp, _L, _dp, _L, _lp, _lp, _L, _dp, _dp, _lp, _lp, _lp, _lp, _lp]
@@ -119,8 +132,9 @@ class Oracle:
     def have_avx512(self):
         return bool(self.lib.oracle_have_avx512())
 
-    def eval(self, kernel, r_trg, r_src, n_src, v_src, v_trg=None, mode=EXACT, digits=-1, nthreads=None):
-        """v_trg += Eval(...); returns v_trg (a fresh zero array when v_trg is None)."""
+    def eval(self, kernel, r_trg, r_src, n_src, v_src, v_trg=None, mode=EXACT, digits=-1, nthreads=None, ctx=None):
+        """v_trg += Eval(...); returns v_trg (a fresh zero array when v_trg is None).
+        ctx: the wavenumber of the Helmholtz kernels."""
         _, K0, K1, ND, _ = self.dims(kernel)
         r_trg, r_src, v_src = _c(r_trg), _c(r_src), _c(v_src)
         n_src = _c(n_src) if ND else None
@@ -129,20 +143,20 @@ class Oracle:
         assert not ND or n_src.size == 3 * ns
         out = np.zeros(nt * K1) if v_trg is None else np.ascontiguousarray(v_trg, dtype=np.float64).copy()
         nth = nthreads or os.cpu_count() or 1
-        rc = self.lib.oracle_eval(kernel, mode, digits, _p(r_trg), nt, _p(r_src), _p(n_src), _p(v_src), ns,
-                                  _p(out), nth)
+        rc = self.lib.oracle_eval_ctx(kernel, mode, digits, _p(r_trg), nt, _p(r_src), _p(n_src), _p(v_src), ns,
+                                      _p(out), nth, _ctx(ctx))
         if rc != 0:
             raise RuntimeError("oracle_eval failed rc=%d" % rc)
         return out
 
-    def kernel_matrix(self, kernel, r_trg, r_src, n_src, nthreads=None):
+    def kernel_matrix(self, kernel, r_trg, r_src, n_src, nthreads=None, ctx=None):
         _, K0, K1, ND, _ = self.dims(kernel)
         r_trg, r_src = _c(r_trg), _c(r_src)
         n_src = _c(n_src) if ND else None
         nt, ns = r_trg.size // 3, r_src.size // 3
         M = np.zeros((ns * K0, nt * K1))
-        rc = self.lib.oracle_kernel_matrix(kernel, _p(r_trg), nt, _p(r_src), _p(n_src), ns, _p(M),
-                                           nthreads or os.cpu_count() or 1)
+        rc = self.lib.oracle_kernel_matrix_ctx(kernel, _p(r_trg), nt, _p(r_src), _p(n_src), ns, _p(M),
+                                               nthreads or os.cpu_count() or 1, _ctx(ctx))
         if rc != 0:
             raise RuntimeError("oracle_kernel_matrix failed rc=%d" % rc)
         return M
@@ -171,7 +185,7 @@ class Oracle:
 
     def near_subtract(self, kernel, normal_dot, blocks, near_elem_dsp, elem_nds_cnt, elem_nds_cnt_far,
                       elem_nds_dsp_far, K_near_cnt, K_near_dsp, Xtrg_near, Xn_trg_near, X_far, Xn_far, wts_far,
-                      K_near, matrix_free=None):
+                      K_near, matrix_free=None, ctx=None):
         """KNearSubtract (boundary_integral.txx:1165-1240) on a copy of K_near."""
         be, bt, bc = (_i(a) for a in blocks)
         arrs = [_i(a) for a in (near_elem_dsp, elem_nds_cnt, elem_nds_cnt_far, elem_nds_dsp_far, K_near_cnt,
@@ -182,7 +196,7 @@ class Oracle:
         rc = self.lib.oracle_near_subtract(kernel, int(normal_dot), be.size, _ip(be), _ip(bt), _ip(bc),
                                            *[_ip(a) for a in arrs],
                                            None if mf is None else mf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                           *[_p(a) for a in f], _p(K))
+                                           *[_p(a) for a in f], _p(K), _ctx(ctx))
         if rc != 0:
             raise RuntimeError("oracle_near_subtract failed rc=%d" % rc)
         return K
@@ -278,9 +292,10 @@ class Reference:
         if not os.path.exists(path):
             raise FileNotFoundError(path)
         L = ctypes.CDLL(path)
-        L.sctlref_eval.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _L, _dp, _dp, _dp, _L, _dp]
-        L.sctlref_eval_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _L, _fp, _fp, _fp, _L, _fp]
-        L.sctlref_kernel_matrix.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _L, _dp, _dp, _L, _dp]
+        L.sctlref_eval.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _L, _dp, _dp, _dp, _L, _dp, _dp]
+        L.sctlref_eval_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _L, _fp, _fp, _fp, _L, _fp,
+                                       _dp]
+        L.sctlref_kernel_matrix.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _L, _dp, _dp, _L, _dp, _dp]
         L.sctlref_rsqrt.argtypes = [ctypes.c_int, _dp, _L, _dp]
         L.sctlref_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, ctypes.c_char_p, ctypes.c_int]
         self.lib = L
@@ -300,17 +315,19 @@ class Reference:
         self.lib.sctlref_info(kernel, d, ctypes.byref(s), name, 64)
         return tuple(d), s.value, name.value.decode()
 
-    def eval(self, kernel, r_trg, r_src, n_src, v_src, v_trg=None, digits=-1, omp=True):
+    def eval(self, kernel, r_trg, r_src, n_src, v_src, v_trg=None, digits=-1, omp=True, ctx=None):
         (_, K0, K1, ND, _), _, _ = self.info(kernel)
         r_trg, r_src, v_src = _c(r_trg), _c(r_src), _c(v_src)
         n_src = _c(n_src) if ND else None
         nt, ns = r_trg.size // 3, r_src.size // 3
         out = np.zeros(nt * K1) if v_trg is None else np.ascontiguousarray(v_trg, dtype=np.float64).copy()
-        self.lib.sctlref_eval(kernel, digits, int(omp), _p(r_trg), nt, _p(r_src), _p(n_src), _p(v_src), ns,
-                              _p(out))
+        rc = self.lib.sctlref_eval(kernel, digits, int(omp), _p(r_trg), nt, _p(r_src), _p(n_src), _p(v_src), ns,
+                                   _p(out), _ctx(ctx))
+        if rc != 0:
+            raise RuntimeError("sctlref_eval failed rc=%d" % rc)
         return out
 
-    def eval_f32(self, kernel, r_trg, r_src, n_src, v_src, digits=-1, omp=True):
+    def eval_f32(self, kernel, r_trg, r_src, n_src, v_src, digits=-1, omp=True, ctx=None):
         (_, K0, K1, ND, _), _, _ = self.info(kernel)
         f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
         r_trg, r_src, v_src = f(r_trg), f(r_src), f(v_src)
@@ -318,16 +335,22 @@ class Reference:
         nt, ns = r_trg.size // 3, r_src.size // 3
         out = np.zeros(nt * K1, dtype=np.float32)
         q = lambda a: a.ctypes.data_as(_fp) if a is not None and a.size else None
-        self.lib.sctlref_eval_f32(kernel, digits, int(omp), q(r_trg), nt, q(r_src), q(n_src), q(v_src), ns, q(out))
+        rc = self.lib.sctlref_eval_f32(kernel, digits, int(omp), q(r_trg), nt, q(r_src), q(n_src), q(v_src), ns,
+                                       q(out), _ctx(ctx))
+        if rc != 0:
+            raise RuntimeError("sctlref_eval_f32 failed rc=%d" % rc)
         return out
 
-    def kernel_matrix(self, kernel, r_trg, r_src, n_src, digits=-1):
+    def kernel_matrix(self, kernel, r_trg, r_src, n_src, digits=-1, ctx=None):
         (_, K0, K1, ND, _), _, _ = self.info(kernel)
         r_trg, r_src = _c(r_trg), _c(r_src)
         n_src = _c(n_src) if ND else None
         nt, ns = r_trg.size // 3, r_src.size // 3
         M = np.zeros((ns * K0, nt * K1))
-        self.lib.sctlref_kernel_matrix(kernel, digits, _p(r_trg), nt, _p(r_src), _p(n_src), ns, _p(M))
+        rc = self.lib.sctlref_kernel_matrix(kernel, digits, _p(r_trg), nt, _p(r_src), _p(n_src), ns, _p(M),
+                                            _ctx(ctx))
+        if rc != 0:
+            raise RuntimeError("sctlref_kernel_matrix failed rc=%d" % rc)
         return M
 
     def rsqrt(self, x, digits=-1):

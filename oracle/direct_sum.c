@@ -49,6 +49,8 @@
  *   5 one Newton step too few (AVX-512 mode)    6 Eval skips the last source
  *   14 BiotSavart FxU with the sign of its third component flipped
  *   15 Laplace Fxd2U with 2 for the 3 of its r r^T / r^5 term
+ *   16 Helmholtz FxU with e^{-i mu r} for e^{i mu r}
+ *   17 HelmholtzDiff FxdU with cos(x) - 1 = -sin^2(x/2) (factor 2 dropped)
  * (7-13: oracle/near_field.c) */
 #ifndef ORACLE_MUTANT
 #define ORACLE_MUTANT 0
@@ -61,8 +63,13 @@
 enum {
   LAP_FXU = 0, LAP_DXU = 1, STK_FXU = 2, STK_DXU = 3,
   LAP_FXDU = 4, LAP_DXDU = 5, LAP_FXD2U = 6, STK_FXT = 7, STK_FSXU = 8, STK_FXUP = 9, BS_FXU = 10, BS_FXDU = 11,
-  NKER = 12
+  HLM_FXU = 12, HLM_DXU = 13, HLM_FXDU = 14, HLMD_FXDU = 15,
+  NKER = 16
 };
+/* The Helmholtz kernels (ids 12..15) read the wavenumber mu through the
+ * kernel's context pointer (kernel_functions.hpp:485-489); complex values
+ * are adjacent (re, im) pairs. */
+#define NEEDS_CTX(ker) ((ker) >= HLM_FXU)
 #define KMAX0 4 /* largest KDIM0 (Stokes3D_FSxU) */
 #define KMAX1 9 /* largest KDIM1 (Laplace3D_Fxd2U, Stokes3D_FxT, BiotSavart3D_FxdU) */
 enum { ORACLE_EXACT = 0, ORACLE_AVX512 = 1, ORACLE_LDOUBLE = 2 };
@@ -70,15 +77,17 @@ enum { ORACLE_EXACT = 0, ORACLE_AVX512 = 1, ORACLE_LDOUBLE = 2 };
 /* uKerMatrix dimensions and FLOPS(), kernel_functions.hpp:32-79 (Laplace FxU,
  * DxU), 81-123 (FxdU), 125-218 (Stokes FxU, DxU), 220-263 (FxT), 264-324
  * (FSxU), 325-365 (FxUP), 366-391 (Fxd2U), 392-415 (DxdU), 416-450
- * (BiotSavart FxU), 451-489 (BiotSavart FxdU) */
-static const int KD0[NKER] = {1, 1, 3, 3, 1, 1, 1, 3, 4, 3, 3, 3};
-static const int KD1[NKER] = {1, 1, 3, 3, 3, 3, 9, 9, 3, 4, 3, 9};
-static const int NDIM[NKER] = {0, 3, 0, 3, 0, 3, 0, 0, 0, 0, 0, 0};
-static const int FLOPS[NKER] = {6, 14, 23, 26, 11, 22, 25, 39, 26, 26, 15, 51};
-/* FUSED_APPLY (kernel_functions.hpp:101,165,204,245,310,350,436): Eval takes
- * the fused apply; the others go through uKerMatrix + FMA
+ * (BiotSavart FxU), 451-489 (BiotSavart FxdU), 491-527 (Helmholtz FxU),
+ * 529-571 (Helmholtz DxU), 573-620 (Helmholtz FxdU), 627-659 (HelmholtzDiff
+ * FxdU) */
+static const int KD0[NKER] = {1, 1, 3, 3, 1, 1, 1, 3, 4, 3, 3, 3, 2, 2, 2, 2};
+static const int KD1[NKER] = {1, 1, 3, 3, 3, 3, 9, 9, 3, 4, 3, 9, 2, 2, 6, 6};
+static const int NDIM[NKER] = {0, 3, 0, 3, 0, 3, 0, 0, 0, 0, 0, 0, 0, 3, 0, 0};
+static const int FLOPS[NKER] = {6, 14, 23, 26, 11, 22, 25, 39, 26, 26, 15, 51, 20, 29, 35, 41};
+/* FUSED_APPLY (kernel_functions.hpp:101,165,204,245,310,350,436,513,552,596):
+ * Eval takes the fused apply; the others go through uKerMatrix + FMA
  * (generic-kernel.txx:107-121). */
-static const int FUSED[NKER] = {0, 0, 1, 1, 1, 0, 0, 1, 1, 1, 1, 0};
+static const int FUSED[NKER] = {0, 0, 1, 1, 1, 0, 0, 1, 1, 1, 1, 0, 1, 1, 1, 0};
 
 /* const_pi<double>(), math_utils.hpp:43 */
 static double pi_d(void) {
@@ -106,15 +115,20 @@ double oracle_scale(int ker) {
     case STK_FXUP: return 1 / (8 * pi_d());
     case BS_FXU: return 1 / (4 * pi_d());
     case BS_FXDU: return -1 / (4 * pi_d());
+    case HLM_FXU: return 1 / (4 * pi_d());
+    case HLM_DXU: return -1 / (4 * pi_d());
+    case HLM_FXDU: return 1 / (4 * pi_d());
+    case HLMD_FXDU: return 1 / (4 * pi_d());
   }
   return 0;
 }
 static long double scale_ld(int ker) {
   switch (ker) {
     case LAP_FXU: case LAP_DXU: case LAP_DXDU: case LAP_FXD2U: case BS_FXU: return 1 / (4 * pi_ld());
+    case HLM_FXU: case HLM_FXDU: case HLMD_FXDU: return 1 / (4 * pi_ld());
     case STK_FXU: case STK_FSXU: case STK_FXUP: return 1 / (8 * pi_ld());
     case STK_DXU: return 3 / (4 * pi_ld());
-    case LAP_FXDU: case BS_FXDU: return -1 / (4 * pi_ld());
+    case LAP_FXDU: case BS_FXDU: case HLM_DXU: return -1 / (4 * pi_ld());
     case STK_FXT: return -3 / (4 * pi_ld());
   }
   return 0;
@@ -147,7 +161,7 @@ static inline double rinv_exact(double r2) {
 
 /* One interaction of Eval for the scalar build: vt += K * vs (fused apply for
  * the Stokes kernels, uKerMatrix + FMA for Laplace, generic-kernel.txx:107-121). */
-static inline void interact_exact(int ker, double* vt, const double* r, const double* n, const double* f) {
+static inline void interact_exact(int ker, double* vt, const double* r, const double* n, const double* f, double mu) {
   const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
   const double rinv = rinv_exact(r2);
   switch (ker) {
@@ -204,11 +218,41 @@ static inline void interact_exact(int ker, double* vt, const double* r, const do
       vt[1] += (f[2] * r[0] - r[2] * f[0]) * rinv3;
       vt[2] += (f[0] * r[1] - r[0] * f[1]) * rinv3 * (ORACLE_MUTANT == 14 ? -1.0 : 1.0);
     } break;
+    case HLM_FXU: { /* :513-526: G = e^{i mu r} / r applied to the complex density */
+      const double x = r2 * rinv * mu;
+      const double sn = (ORACLE_MUTANT == 16 ? -1.0 : 1.0) * sin(x), cs = cos(x);
+      const double G0 = cs * rinv, G1 = sn * rinv;
+      vt[0] += f[0] * G0 - f[1] * G1;
+      vt[1] += f[0] * G1 + f[1] * G0;
+    } break;
+    case HLM_DXU: { /* :552-570 */
+      const double rinv2 = rinv * rinv;
+      const double ndotr = n[0] * r[0] + n[1] * r[1] + n[2] * r[2];
+      const double x = r2 * rinv * mu;
+      const double sn = sin(x), cs = cos(x);
+      const double G0 = (-mu * sn - cs * rinv) * rinv2 * ndotr;
+      const double G1 = (mu * cs - sn * rinv) * rinv2 * ndotr;
+      vt[0] += f[0] * G0 - f[1] * G1;
+      vt[1] += f[0] * G1 + f[1] * G0;
+    } break;
+    case HLM_FXDU: { /* :596-619 */
+      const double rinv2 = rinv * rinv;
+      const double x = r2 * rinv * mu;
+      const double sn = sin(x), cs = cos(x);
+      const double G0 = (-mu * sn - cs * rinv) * rinv2;
+      const double G1 = (mu * cs - sn * rinv) * rinv2;
+      const double fG0 = f[0] * G0 - f[1] * G1;
+      const double fG1 = f[0] * G1 + f[1] * G0;
+      for (int i = 0; i < 3; i++) {
+        vt[i * 2 + 0] += fG0 * r[i];
+        vt[i * 2 + 1] += fG1 * r[i];
+      }
+    } break;
   }
 }
 
 /* uKerMatrix for the scalar build: u[k0][k1] (not scaled). */
-static inline void matrix_exact(int ker, double u[KMAX0][KMAX1], const double* r, const double* n) {
+static inline void matrix_exact(int ker, double u[KMAX0][KMAX1], const double* r, const double* n, double mu) {
   const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
   const double rinv = rinv_exact(r2);
   switch (ker) {
@@ -287,25 +331,63 @@ static inline void matrix_exact(int ker, double u[KMAX0][KMAX1], const double* r
       u[0][7] = -rinv3 + t5 * r[1] * r[1];   u[1][7] = -t5 * r[0] * r[1];         u[2][7] = 0.0;
       u[0][8] = t5 * r[1] * r[2];            u[1][8] = -t5 * r[0] * r[2];         u[2][8] = 0.0;
     } break;
+    case HLM_FXU: { /* :501-511 */
+      const double x = r2 * rinv * mu;
+      const double sn = (ORACLE_MUTANT == 16 ? -1.0 : 1.0) * sin(x), cs = cos(x);
+      u[0][0] = cs * rinv; u[0][1] = sn * rinv;
+      u[1][0] = -u[0][1];  u[1][1] = u[0][0];
+    } break;
+    case HLM_DXU: { /* :539-550 */
+      const double rinv2 = rinv * rinv;
+      const double ndotr = n[0] * r[0] + n[1] * r[1] + n[2] * r[2];
+      const double x = r2 * rinv * mu;
+      const double sn = sin(x), cs = cos(x);
+      u[0][0] = (-mu * sn - cs * rinv) * rinv2 * ndotr;
+      u[0][1] = (mu * cs - sn * rinv) * rinv2 * ndotr;
+      u[1][0] = -u[0][1]; u[1][1] = u[0][0];
+    } break;
+    case HLM_FXDU:
+    case HLMD_FXDU: { /* :583-594; HelmholtzDiff :637-656 (half-angle form of cos(mu r) - 1) */
+      const double rinv2 = rinv * rinv;
+      double G0, G1;
+      if (ker == HLM_FXDU) {
+        const double x = r2 * rinv * mu;
+        const double sn = sin(x), cs = cos(x);
+        G0 = (-mu * sn - cs * rinv) * rinv2;
+        G1 = (mu * cs - sn * rinv) * rinv2;
+      } else {
+        const double xh = r2 * rinv * mu * 0.5;
+        const double sh = sin(xh), ch = cos(xh);
+        const double sn = 2.0 * sh * ch;
+        const double cm1 = (ORACLE_MUTANT == 17 ? -1.0 : -2.0) * sh * sh;
+        const double cs = 1.0 + cm1;
+        G0 = (-mu * sn - cm1 * rinv) * rinv2;
+        G1 = (mu * cs - sn * rinv) * rinv2;
+      }
+      for (int i = 0; i < 3; i++) {
+        u[0][i * 2 + 0] = G0 * r[i]; u[0][i * 2 + 1] = G1 * r[i];
+        u[1][i * 2 + 0] = -u[0][i * 2 + 1]; u[1][i * 2 + 1] = u[0][i * 2 + 0];
+      }
+    } break;
   }
 }
 
 /* One interaction of Eval in the reference's branch for the kernel: the fused
  * apply, or uKerMatrix followed by FMA(U, vs, vt) (a separate multiply and
  * add in the scalar build), generic-kernel.txx:107-121. */
-static inline void eval_pair_exact(int ker, double* vt, const double* r, const double* n, const double* f) {
+static inline void eval_pair_exact(int ker, double* vt, const double* r, const double* n, const double* f, double mu) {
   if (FUSED[ker] || ker == LAP_FXU || ker == LAP_DXU) {
-    interact_exact(ker, vt, r, n, f);
+    interact_exact(ker, vt, r, n, f, mu);
     return;
   }
   double u[KMAX0][KMAX1];
-  matrix_exact(ker, u, r, n);
+  matrix_exact(ker, u, r, n, mu);
   for (int k0 = 0; k0 < KD0[ker]; k0++)
     for (int k1 = 0; k1 < KD1[ker]; k1++) vt[k1] = u[k0][k1] * f[k0] + vt[k1];
 }
 
 static void eval_exact(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
-                       const double* v_src, long ns, double* v_trg, int nthreads) {
+                       const double* v_src, long ns, double* v_trg, int nthreads, double mu) {
   const int K0 = KD0[ker], K1 = KD1[ker], ND = NDIM[ker];
   const double sf = oracle_scale(ker);
   const double zero3[3] = {0, 0, 0};
@@ -315,7 +397,7 @@ static void eval_exact(int ker, const double* r_trg, long nt, const double* r_sr
     for (long s = 0; s < ns - (ORACLE_MUTANT == 6 ? 1 : 0); s++) {
       const double r[3] = {r_trg[t * 3 + 0] - r_src[s * 3 + 0], r_trg[t * 3 + 1] - r_src[s * 3 + 1],
                            r_trg[t * 3 + 2] - r_src[s * 3 + 2]};
-      eval_pair_exact(ker, vt, r, ND ? n_src + s * 3 : zero3, v_src + s * K0);
+      eval_pair_exact(ker, vt, r, ND ? n_src + s * 3 : zero3, v_src + s * K0, mu);
     }
     for (int k = 0; k < K1; k++) v_trg[t * K1 + k] += vt[k] * sf;
   }
@@ -323,7 +405,8 @@ static void eval_exact(int ker, const double* r_trg, long nt, const double* r_sr
 
 /* ------------------------------------------------------------ long double */
 static void eval_ldouble(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
-                         const double* v_src, long ns, double* v_trg, int nthreads) {
+                         const double* v_src, long ns, double* v_trg, int nthreads, double mu_d) {
+  const long double mu = mu_d;
   const int K0 = KD0[ker], K1 = KD1[ker], ND = NDIM[ker];
   const long double sf = scale_ld(ker);
 #pragma omp parallel for schedule(static) num_threads(nthreads)
@@ -377,6 +460,32 @@ static void eval_ldouble(int ker, const double* r_trg, long nt, const double* r_
               u[1][0] = r[2] * rinv3; u[2][0] = -r[1] * rinv3; u[0][1] = -r[2] * rinv3;
               u[2][1] = r[0] * rinv3; u[0][2] = r[1] * rinv3; u[1][2] = -r[0] * rinv3;
               break;
+            case HLM_FXU: case HLM_DXU: case HLM_FXDU: case HLMD_FXDU: {
+              /* e^{i mu r}/r and its derivatives as 2x2 real blocks of complex numbers */
+              const long double rr = sqrtl(r2), sn = sinl(mu * rr), cs = cosl(mu * rr);
+              long double G0, G1;
+              if (ker == HLM_FXU) {
+                G0 = cs * rinv;
+                G1 = sn * rinv;
+              } else {
+                /* d/dr (e^{i mu r}/r) / r = (i mu r - 1) e^{i mu r} / r^3 */
+                G0 = (-mu * rr * sn - cs) * rinv3;
+                G1 = (mu * rr * cs - sn) * rinv3;
+                if (ker == HLMD_FXDU) G0 += rinv3; /* minus Laplace: (cos - 1) term */
+                if (ker == HLM_DXU) {
+                  G0 *= rn;
+                  G1 *= rn;
+                }
+              }
+              if (ker == HLM_FXU || ker == HLM_DXU) {
+                u[0][0] = G0; u[0][1] = G1; u[1][0] = -G1; u[1][1] = G0;
+              } else {
+                for (int i = 0; i < 3; i++) {
+                  u[0][i * 2 + 0] = G0 * r[i]; u[0][i * 2 + 1] = G1 * r[i];
+                  u[1][i * 2 + 0] = -G1 * r[i]; u[1][i * 2 + 1] = G0 * r[i];
+                }
+              }
+            } break;
             case BS_FXDU: /* d/dx_j of (f x r / r^3)_i, column i*3+j */
               for (int k = 0; k < 3; k++)       /* source component */
                 for (int i = 0; i < 3; i++)     /* field component */
@@ -510,13 +619,18 @@ int oracle_have_avx512(void) {
  * Return 0 on success, -1 on a bad kernel id or mode, -2 when the AVX-512
  * mode is requested on a CPU without AVX-512.
  */
-int oracle_eval(int ker, int mode, int digits, const double* r_trg, long nt, const double* r_src,
-                const double* n_src, const double* v_src, long ns, double* v_trg, int nthreads) {
+/* ctx: the kernel's context (the wavenumber for the Helmholtz kernels, which
+ * need it; ignored by the others). */
+int oracle_eval_ctx(int ker, int mode, int digits, const double* r_trg, long nt, const double* r_src,
+                    const double* n_src, const double* v_src, long ns, double* v_trg, int nthreads,
+                    const double* ctx) {
   if (ker < 0 || ker >= NKER) return -1;
+  if (NEEDS_CTX(ker) && !ctx) return -1;
+  const double mu = NEEDS_CTX(ker) ? *ctx : 0.0;
   if (nthreads < 1) nthreads = 1;
   switch (mode) {
-    case ORACLE_EXACT: eval_exact(ker, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads); return 0;
-    case ORACLE_LDOUBLE: eval_ldouble(ker, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads); return 0;
+    case ORACLE_EXACT: eval_exact(ker, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads, mu); return 0;
+    case ORACLE_LDOUBLE: eval_ldouble(ker, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads, mu); return 0;
     case ORACLE_AVX512:
 #if defined(__x86_64__)
       if (ker > STK_DXU) return -1; /* the AVX-512 restatement covers the four hot-path kernels */
@@ -530,10 +644,17 @@ int oracle_eval(int ker, int mode, int digits, const double* r_trg, long nt, con
   return -1;
 }
 
+int oracle_eval(int ker, int mode, int digits, const double* r_trg, long nt, const double* r_src,
+                const double* n_src, const double* v_src, long ns, double* v_trg, int nthreads) {
+  return oracle_eval_ctx(ker, mode, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg, nthreads, 0);
+}
+
 /* KernelMatrix, exact mode: M is (ns*K0) x (nt*K1) row-major, overwritten. */
-int oracle_kernel_matrix(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
-                         long ns, double* M, int nthreads) {
+int oracle_kernel_matrix_ctx(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
+                             long ns, double* M, int nthreads, const double* ctx) {
   if (ker < 0 || ker >= NKER) return -1;
+  if (NEEDS_CTX(ker) && !ctx) return -1;
+  const double mu = NEEDS_CTX(ker) ? *ctx : 0.0;
   const int K0 = KD0[ker], K1 = KD1[ker], ND = NDIM[ker];
   const double sf = oracle_scale(ker);
   const double zero3[3] = {0, 0, 0};
@@ -544,7 +665,7 @@ int oracle_kernel_matrix(int ker, const double* r_trg, long nt, const double* r_
       const double r[3] = {r_trg[t * 3 + 0] - r_src[s * 3 + 0], r_trg[t * 3 + 1] - r_src[s * 3 + 1],
                            r_trg[t * 3 + 2] - r_src[s * 3 + 2]};
       double u[KMAX0][KMAX1];
-      matrix_exact(ker, u, r, ND ? n_src + s * 3 : zero3);
+      matrix_exact(ker, u, r, ND ? n_src + s * 3 : zero3, mu);
       for (int k0 = 0; k0 < K0; k0++)
         for (int k1 = 0; k1 < K1; k1++) {
           if (ORACLE_MUTANT == 3) M[(t * K1 + k1) * (ns * K0) + s * K0 + k0] = u[k0][k1] * sf;
@@ -553,6 +674,10 @@ int oracle_kernel_matrix(int ker, const double* r_trg, long nt, const double* r_
     }
   }
   return 0;
+}
+int oracle_kernel_matrix(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
+                         long ns, double* M, int nthreads) {
+  return oracle_kernel_matrix_ctx(ker, r_trg, nt, r_src, n_src, ns, M, nthreads, 0);
 }
 
 /* Masked rsqrt of one array: mode 0 exact, mode 1 AVX-512 estimate + Newton. */

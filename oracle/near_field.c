@@ -31,6 +31,8 @@
 
 int oracle_kernel_matrix(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
                          long ns, double* M, int nthreads);
+int oracle_kernel_matrix_ctx(int ker, const double* r_trg, long nt, const double* r_src, const double* n_src,
+                             long ns, double* M, int nthreads, const double* ctx);
 int oracle_dims(int ker, int* dims);
 
 static long min_l(long a, long b) { return a < b ? a : b; }
@@ -81,7 +83,7 @@ int oracle_near_subtract(int ker, int normal_dot, long nblk, const long* near_bl
                          const long* elem_nds_cnt_far, const long* elem_nds_dsp_far, const long* K_near_cnt,
                          const long* K_near_dsp, const int* matrix_free, const double* Xtrg_near,
                          const double* Xn_trg_near, const double* X_far, const double* Xn_far, const double* wts_far,
-                         double* K_near) {
+                         double* K_near, const double* ctx) {
   int d[5];
   if (oracle_dims(ker, d) != 0) return -1;
   const int K0 = d[1], K1 = d[2], ND = d[3];
@@ -106,8 +108,12 @@ int oracle_near_subtract(int ker, int normal_dot, long nblk, const long* near_bl
       free(Mker);
       return -3;
     }
-    oracle_kernel_matrix(ker, Xtrg_near + trg_dsp * 3, trg_cnt, X_far + far_src_dsp * 3,
-                         ND ? Xn_far + far_src_dsp * 3 : NULL, far_src_cnt, Mk, 1);
+    if (oracle_kernel_matrix_ctx(ker, Xtrg_near + trg_dsp * 3, trg_cnt, X_far + far_src_dsp * 3,
+                                 ND ? Xn_far + far_src_dsp * 3 : NULL, far_src_cnt, Mk, 1, ctx) != 0) {
+      free(Mk);
+      free(Mker);
+      return -1;
+    }
     const long ldk = trg_cnt * K1, ldm = trg_cnt * K1_;
     for (long s = 0; s < far_src_cnt; s++) {
       const double w = ORACLE_MUTANT == 8 ? 1.0 : wts_far[far_src_dsp + s];

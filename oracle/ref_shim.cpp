@@ -8,7 +8,9 @@
 //   GenericKernel<uKernel>::Eval<Real,omp>(..., digits, self)   generic-kernel.txx:72-100
 //   GenericKernel<uKernel>::KernelMatrix<Real,false,digits>     generic-kernel.txx:213-329
 //   approx_rsqrt<digits>(x, mask)                               vec.txx:379-382
-// for the real-valued micro-kernels of kernel_functions.hpp:32-489.
+// for the micro-kernels of kernel_functions.hpp:32-659 (the Helmholtz kernels
+// with their wavenumber set through GenericKernel::SetCtxPtr,
+// generic-kernel.txx:64-66).
 #include <cstring>
 #include <string>
 
@@ -32,8 +34,10 @@ template <class Real> using Vec = sctl::Vector<Real>;
 
 template <class K, class Real>
 void eval_one(int omp, int digits, const Real* r_trg, long nt, const Real* r_src, const Real* n_src,
-              const Real* v_src, long ns, Real* v_trg) {
+              const Real* v_src, long ns, Real* v_trg, const double* ctx) {
   K ker;
+  Real mu = ctx ? (Real)*ctx : (Real)0;  // the kernels read *(const Real*)ctx_ptr
+  if (ctx) ker.SetCtxPtr(&mu);
   const Long nd = K::NormalDim();
   Vec<Real> Xt(nt * 3, (sctl::Iterator<Real>)r_trg, false);
   Vec<Real> Xs(ns * 3, (sctl::Iterator<Real>)r_src, false);
@@ -47,8 +51,10 @@ void eval_one(int omp, int digits, const Real* r_trg, long nt, const Real* r_src
 
 template <class K>
 void kmat_one(int digits, const double* r_trg, long nt, const double* r_src, const double* n_src,
-              long ns, double* M_out) {
+              long ns, double* M_out, const double* ctx) {
   K ker;
+  double mu = ctx ? *ctx : 0.0;
+  if (ctx) ker.SetCtxPtr(&mu);
   const Long nd = K::NormalDim();
   Vec<double> Xt(nt * 3, (sctl::Iterator<double>)r_trg, false);
   Vec<double> Xs(ns * 3, (sctl::Iterator<double>)r_src, false);
@@ -80,7 +86,8 @@ template <Integer digits> void rsqrt_digits(const double* x, long n, double* y) 
 // Calls f(GenericKernel object, kernel_impl tag) for a kernel id:
 //   0 Laplace3D_FxU  1 Laplace3D_DxU  2 Stokes3D_FxU  3 Stokes3D_DxU  4 Laplace3D_FxdU
 //   5 Laplace3D_DxdU 6 Laplace3D_Fxd2U 7 Stokes3D_FxT 8 Stokes3D_FSxU 9 Stokes3D_FxUP
-//   10 BiotSavart3D_FxU 11 BiotSavart3D_FxdU    (kernel_functions.hpp:669-684)
+//   10 BiotSavart3D_FxU 11 BiotSavart3D_FxdU 12 Helmholtz3D_FxU 13 Helmholtz3D_DxU
+//   14 Helmholtz3D_FxdU 15 HelmholtzDiff3D_FxdU    (kernel_functions.hpp:669-684)
 template <class F> int with_kernel(int kernel, F&& f) {
   namespace ki = sctl::kernel_impl;
   switch (kernel) {
@@ -96,6 +103,10 @@ template <class F> int with_kernel(int kernel, F&& f) {
     case 9: f(sctl::Stokes3D_FxUP(), ki::Stokes3D_FxUP()); return 0;
     case 10: f(sctl::BiotSavart3D_FxU(), ki::BiotSavart3D_FxU()); return 0;
     case 11: f(sctl::BiotSavart3D_FxdU(), ki::BiotSavart3D_FxdU()); return 0;
+    case 12: f(sctl::Helmholtz3D_FxU(), ki::Helmholtz3D_FxU()); return 0;
+    case 13: f(sctl::Helmholtz3D_DxU(), ki::Helmholtz3D_DxU()); return 0;
+    case 14: f(sctl::Helmholtz3D_FxdU(), ki::Helmholtz3D_FxdU()); return 0;
+    case 15: f(sctl::HelmholtzDiff3D_FxdU(), ki::HelmholtzDiff3D_FxdU()); return 0;
   }
   return -1;
 }
@@ -116,24 +127,28 @@ int sctlref_info(int kernel, int* dims, double* scale, char* name, int name_len)
 
 int sctlref_veclen(void) { return (int)sctl::DefaultVecLen<double>(); }
 
+// ctx: the wavenumber for the Helmholtz kernels (required by them), else NULL.
 int sctlref_eval(int kernel, int digits, int omp, const double* r_trg, long nt, const double* r_src,
-                 const double* n_src, const double* v_src, long ns, double* v_trg) {
+                 const double* n_src, const double* v_src, long ns, double* v_trg, const double* ctx) {
+  if (kernel >= 12 && !ctx) return -1;
   return with_kernel(kernel, [&](auto ker, auto) {
-    eval_one<decltype(ker), double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg);
+    eval_one<decltype(ker), double>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg, ctx);
   });
 }
 
 int sctlref_eval_f32(int kernel, int digits, int omp, const float* r_trg, long nt, const float* r_src,
-                     const float* n_src, const float* v_src, long ns, float* v_trg) {
+                     const float* n_src, const float* v_src, long ns, float* v_trg, const double* ctx) {
+  if (kernel >= 12 && !ctx) return -1;
   return with_kernel(kernel, [&](auto ker, auto) {
-    eval_one<decltype(ker), float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg);
+    eval_one<decltype(ker), float>(omp, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg, ctx);
   });
 }
 
 int sctlref_kernel_matrix(int kernel, int digits, const double* r_trg, long nt, const double* r_src,
-                          const double* n_src, long ns, double* M) {
+                          const double* n_src, long ns, double* M, const double* ctx) {
+  if (kernel >= 12 && !ctx) return -1;
   return with_kernel(kernel, [&](auto ker, auto) {
-    kmat_one<decltype(ker)>(digits, r_trg, nt, r_src, n_src, ns, M);
+    kmat_one<decltype(ker)>(digits, r_trg, nt, r_src, n_src, ns, M, ctx);
   });
 }
 

@@ -28,6 +28,11 @@ Pins, with what each rests on:
   derivative_identities  those kernels tied to the four pinned ones: gradients
                     by central differences, FxUP / FSxU / FxT / BiotSavart as
                     combinations of Stokes FxU, DxU and Laplace FxdU
+  golden_exact_helmholtz the Helmholtz kernels (ids 12..15) bit-exact against
+                    the reference's scalar build at four wavenumbers
+  helmholtz_identities  closed form of e^{i mu r}/(4 pi r), the mu = 0 limits
+                    (Laplace), gradient and normal-derivative differences, the
+                    Helmholtz equation, HelmholtzDiff = Helmholtz - Laplace
 Near field (oracle/near_field.c):
   near_partition_hand    the block partition worked by hand
   near_layout_hand       K_near_cnt / K_near_dsp worked by hand
@@ -56,11 +61,20 @@ def golden():
 
 GOLDEN_MORE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sctl_ref_more_kernels.npz")
 MORE_KERNELS = range(4, 12)   # Laplace FxdU, DxdU, Fxd2U, Stokes FxT, FSxU, FxUP, BiotSavart FxU, FxdU
-ALL_KERNELS = range(12)
+ALL_KERNELS = range(16)
+MU_TEST = 1.7  # wavenumber for the Helmholtz kernels (ignored by the others)
 
 
 def golden_more():
     return np.load(GOLDEN_MORE, allow_pickle=False)
+
+
+GOLDEN_HELMHOLTZ = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sctl_ref_helmholtz.npz")
+HELMHOLTZ_KERNELS = range(12, 16)  # Helmholtz3D FxU, DxU, FxdU, HelmholtzDiff3D FxdU (wavenumber in ctx)
+
+
+def golden_helmholtz():
+    return np.load(GOLDEN_HELMHOLTZ, allow_pickle=False)
 
 
 def _nrm(x, ref):
@@ -131,6 +145,106 @@ def golden_exact_more(orc):
             ref = G["k%d_%s_scalar" % (k, tag)]
             if M.shape != ref.shape or not np.array_equal(M, ref):
                 bad.append("k%d %s" % (k, tag))
+    return not bad, ", ".join(bad)
+
+
+def golden_exact_helmholtz(orc):
+    """The Helmholtz micro-kernels bit-exact against the reference's scalar
+    build at four wavenumbers (tests/golden/make_golden_helmholtz.py)."""
+    G = golden_helmholtz()
+    bad = []
+    for k in HELMHOLTZ_KERNELS:
+        _, K0, K1, ND, _ = orc.dims(k)
+        for j, mu in enumerate(G["mu"]):
+            p = "k%d_m%d_" % (k, j)
+            Xt, Xs, Xn, Vs = small_inputs(K0, ND)
+            if not np.array_equal(orc.eval(k, Xt, Xs, Xn, Vs, ctx=mu), G[p + "small_eval_scalar"]):
+                bad.append(p + "small eval")
+            if not np.array_equal(orc.kernel_matrix(k, Xt, Xs, Xn, ctx=mu).ravel(), G[p + "small_kmat_scalar"].ravel()):
+                bad.append(p + "small kmat")
+            xt, xs, n, v = (G[p + "rand_" + a] for a in ("xt", "xs", "n", "v"))
+            out = orc.eval(k, xt, xs, n, v, ctx=mu)
+            if not np.array_equal(out, G[p + "rand_eval_scalar"]):
+                bad.append(p + "rand eval")
+            if _nrm(out, G[p + "rand_eval_v4"]) > 1e-14:
+                bad.append(p + "rand vs AVX-512 build")
+            vt = orc.eval(k, G[p + "veclen_xt"], xs, n, v, v_trg=G[p + "veclen_vt0"], ctx=mu)
+            if not np.array_equal(vt, G[p + "veclen_eval_scalar"]):
+                bad.append(p + "veclen accumulate")
+            xt11, xs13, n13 = (G[p + "kmat_" + a] for a in ("xt", "xs", "n"))
+            for tag, a, b, c in (("kmat", xt11, xs13, n13), ("kmat1t", xt11[:3], xs13, n13),
+                                 ("kmat1s", xt11, xs13[:3], n13[:3])):
+                M = orc.kernel_matrix(k, a, b, c, ctx=mu)
+                ref = G[p + tag + "_scalar"]
+                if M.shape != ref.shape or not np.array_equal(M, ref):
+                    bad.append(p + tag)
+    return not bad, ", ".join(bad)
+
+
+def helmholtz_identities(orc):
+    """The Helmholtz kernels fixed by mathematics, independently of the
+    reference's code (kernel_functions.hpp:485-659):
+      * closed form of the outgoing Green's function: one unit source, FxU =
+        (cos(mu d), sin(mu d)) / (4 pi d) at distance d;
+      * mu = 0: FxU / DxU / FxdU are Laplace FxU / DxU / FxdU on the real and
+        imaginary parts, HelmholtzDiff FxdU vanishes;
+      * FxdU = grad FxU and DxU = d/dh FxU(x; y + h n) (central differences);
+      * (Laplacian + mu^2) FxU = 0 away from the sources (second differences);
+      * HelmholtzDiff FxdU = Helmholtz FxdU - Laplace FxdU (real parts)."""
+    rng = np.random.default_rng(23)
+    ns, nt = 30, 20
+    xs = rng.random(ns * 3)
+    xt = (rng.random((nt, 3)) + 1.5).ravel()
+    n = rng.standard_normal(ns * 3)
+    f = rng.uniform(-1, 1, ns * 2)
+    fre, fim = f[0::2].copy(), f[1::2].copy()
+    bad = []
+    mu = 2.3
+    for d in (0.5, 1.0, 4.0):
+        got = orc.eval(12, [d, 0, 0], [0, 0, 0], None, [1.0, 0.0], ctx=mu)
+        want = np.array([np.cos(mu * d), np.sin(mu * d)]) / (4 * np.pi * d)
+        if _nrm(got, want) > 1e-15:
+            bad.append("closed form d=%g" % d)
+    E = lambda k, pts, m, dens=f, src=xs, nrm=n: orc.eval(k, pts, src, nrm, dens, ctx=m)
+    pair = lambda a, b: np.stack([a, b], axis=-1).reshape(nt, -1)
+    checks = [("mu=0 FxU", E(12, xt, 0.0).reshape(nt, 2),
+               pair(orc.eval(0, xt, xs, None, fre), orc.eval(0, xt, xs, None, fim)), 1e-15),
+              ("mu=0 DxU", E(13, xt, 0.0).reshape(nt, 2),
+               pair(orc.eval(1, xt, xs, n, fre), orc.eval(1, xt, xs, n, fim)), 1e-14),
+              ("mu=0 FxdU", E(14, xt, 0.0).reshape(nt, 3, 2),
+               np.stack([orc.eval(4, xt, xs, None, fre).reshape(nt, 3),
+                         orc.eval(4, xt, xs, None, fim).reshape(nt, 3)], axis=-1), 1e-15)]
+    if np.abs(E(15, xt, 0.0)).max() != 0:
+        bad.append("mu=0 HelmholtzDiff FxdU not zero")
+    h = 1e-5
+    P = xt.reshape(-1, 3)
+    grad = np.zeros((nt, 3, 2))
+    for j in range(3):
+        dd = np.zeros(3)
+        dd[j] = h
+        grad[:, j, :] = ((E(12, (P + dd).ravel(), mu) - E(12, (P - dd).ravel(), mu)).reshape(nt, 2)) / (2 * h)
+    checks.append(("FxdU = grad FxU", E(14, xt, mu).reshape(nt, 3, 2), grad, 1e-8))
+    # DxU: moving each source along its own normal (kernel value per unit density, summed)
+    Ysrc = xs.reshape(-1, 3)
+    dx = ((E(12, xt, mu, src=(Ysrc + h * n.reshape(-1, 3)).ravel()) -
+           E(12, xt, mu, src=(Ysrc - h * n.reshape(-1, 3)).ravel())) / (2 * h)).reshape(nt, 2)
+    checks.append(("DxU = d/dh FxU(y + h n)", E(13, xt, mu).reshape(nt, 2), dx, 1e-8))
+    h2 = 1e-3
+    lap = -6 * E(12, xt, mu)
+    for j in range(3):
+        dd = np.zeros(3)
+        dd[j] = h2
+        lap = lap + E(12, (P + dd).ravel(), mu) + E(12, (P - dd).ravel(), mu)
+    lap /= h2 * h2
+    checks.append(("(Laplacian + mu^2) FxU = 0", lap + mu * mu * E(12, xt, mu), np.zeros(nt * 2), 1e-5))
+    diff = E(14, xt, mu).reshape(nt, 3, 2).copy()
+    diff[:, :, 0] -= orc.eval(4, xt, xs, None, fre).reshape(nt, 3)
+    diff[:, :, 1] -= orc.eval(4, xt, xs, None, fim).reshape(nt, 3)
+    checks.append(("HelmholtzDiff = Helmholtz - Laplace", E(15, xt, mu).reshape(nt, 3, 2), diff, 1e-12))
+    for name, a, b, tol in checks:
+        e = np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(E(12, xt, mu)).max(), 1e-300)
+        if not e < tol:
+            bad.append("%s %.2e" % (name, e))
     return not bad, ", ".join(bad)
 
 
@@ -283,8 +397,8 @@ def long_double(orc):
         xt[:30] = xs[:30]
         n = rng.standard_normal(400 * 3)
         v = rng.uniform(-1, 1, 400 * K0)
-        a = orc.eval(k, xt, xs, n, v)
-        b = orc.eval(k, xt, xs, n, v, mode=oracle.LDOUBLE)
+        a = orc.eval(k, xt, xs, n, v, ctx=MU_TEST)
+        b = orc.eval(k, xt, xs, n, v, mode=oracle.LDOUBLE, ctx=MU_TEST)
         e = _nrm(a, b)
         worst = max(worst, e)
         if e > 1e-13:
@@ -297,8 +411,8 @@ def eval_vs_matrix(orc):
     for k in ALL_KERNELS:
         _, K0, K1, ND, _ = orc.dims(k)
         Xt, Xs, Xn, Vs = small_inputs(K0, ND)
-        vt = orc.eval(k, Xt, Xs, Xn, Vs)
-        M = orc.kernel_matrix(k, Xt, Xs, Xn)
+        vt = orc.eval(k, Xt, Xs, Xn, Vs, ctx=MU_TEST)
+        M = orc.kernel_matrix(k, Xt, Xs, Xn, ctx=MU_TEST)
         if M.shape != (7 * K0, 5 * K1) or not np.allclose(vt, M.T @ Vs, rtol=1e-10, atol=1e-14 * np.abs(vt).max()):
             bad.append("k%d" % k)
     return not bad, ", ".join(bad)
@@ -362,8 +476,8 @@ def linearity(orc):
         _, K0, K1, ND, _ = orc.dims(k)
         xs, xt, n = rng.random(64 * 3), rng.random(50 * 3), rng.standard_normal(64 * 3)
         f, g = rng.uniform(-1, 1, 64 * K0), rng.uniform(-1, 1, 64 * K0)
-        a = orc.eval(k, xt, xs, n, 2.0 * f - 0.5 * g)
-        b = 2.0 * orc.eval(k, xt, xs, n, f) - 0.5 * orc.eval(k, xt, xs, n, g)
+        a = orc.eval(k, xt, xs, n, 2.0 * f - 0.5 * g, ctx=MU_TEST)
+        b = 2.0 * orc.eval(k, xt, xs, n, f, ctx=MU_TEST) - 0.5 * orc.eval(k, xt, xs, n, g, ctx=MU_TEST)
         if _nrm(a, b) > 1e-13:
             bad.append("k%d" % k)
     return not bad, ", ".join(bad)
@@ -580,4 +694,5 @@ NEAR_CHECKS = [near_partition_hand, near_layout_hand, near_subtract_identity, ne
                near_list_golden, near_list_hand]
 
 CHECKS = [golden_exact, golden_avx512, golden_rsqrt, rsqrt_bounds, closed_forms, long_double, eval_vs_matrix,
-          surface_identities, linearity, golden_exact_more, derivative_identities] + NEAR_CHECKS
+          surface_identities, linearity, golden_exact_more, derivative_identities, golden_exact_helmholtz,
+          helmholtz_identities] + NEAR_CHECKS

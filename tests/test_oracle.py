@@ -26,7 +26,7 @@ def mutant_dir():
         yield d
 
 
-@pytest.mark.parametrize("mutant", range(1, 16))
+@pytest.mark.parametrize("mutant", range(1, 18))
 def test_pins_reject_mutant(mutant, mutant_dir):
     orc = oracle.Oracle(oracle.build_mutant(mutant, mutant_dir))
     failed = [c.__name__ for c in pins.CHECKS if not c(orc)[0]]
