@@ -369,6 +369,26 @@ def other_configs(stream, quick=False):
     out["c5_stokes_dxu_graph_2048x100"] = graph_pair(S.STOKES3D_DXU, B, stream, nsteps=20 if quick else 100)
     log("config 5 blocks: near-field matrices (KNearSubtract, GEMV), Stokes DxU")
     out["c5_near_matrix_stokes_dxu_2048"] = near_ops(S.STOKES3D_DXU, B, stream, reps=2)
+    log("other micro-kernels: FP64 and mixed Eval, 2^17 x 2^17 uniform cube")
+    out["other_kernels_cube_128k"] = other_kernels(stream, reps=1 if quick else 2)
+    return out
+
+
+def other_kernels(stream, reps):
+    """FP64 and mixed Eval of the eight other real-valued micro-kernels on a
+    2^17-target x 2^17-source uniform cube: rates, the reference's FLOPS()
+    count per interaction times the rate, and mixed-vs-FP64 errors."""
+    N = 1 << 17
+    out = {}
+    for k in range(4, S.NUM_KERNELS):
+        _, K0, K1, ND, flops, _, name = S.kernel_info(k)
+        xt, xs, f = W.cube(N, N, seed=k, k0=K0)
+        nrm = W.normal(k, 3, N * 3) if ND else None
+        r, _ = dense_pair(k, xt, xs, nrm, f, stream, reps=reps)
+        for prec in ("fp64", "mixed"):
+            r[prec]["ref_TFLOP_per_s"] = round(r[prec]["G_int_per_s"] * flops / 1e3, 2)
+        r["ref_flops_per_interaction"] = flops
+        out[name] = r
     return out
 
 
@@ -390,6 +410,8 @@ def main():
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--near-only", action="store_true",
                     help="only the near-field operator and near-list lines of config 4 (for kernel traces)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the other micro-kernels' FP64 and mixed Eval lines (for kernel traces)")
     ap.add_argument("--shape", default=None, metavar="TPL,CAP,MC",
                     help="pin the mixed headline launch shape (targets per lane, workgroups-per-CU cap, "
                          "matrix cores 0/1) instead of tuning it")
@@ -408,6 +430,14 @@ def main():
         out = {"c4_near_matrix_laplace_dxu_4096": near_ops(S.LAPLACE3D_DXU, B, stream, reps=3),
                "c4_near_list_4096": near_list_build(B, 0.05, stream, cpu=False)}
         print(json.dumps({"near_only": True, "configs": out}), flush=True)
+        return
+
+    if args.kernels_only:
+        stream = ctypes.c_void_p()
+        S.check(S.lib().sctl_amd_stream_create(ctypes.byref(stream)))
+        log("other micro-kernels: FP64 and mixed Eval, 2^17 x 2^17 uniform cube")
+        print(json.dumps({"kernels_only": True, "configs": {"other_kernels_cube_128k": other_kernels(stream, 2)}}),
+              flush=True)
         return
 
     N = args.n

@@ -44,9 +44,12 @@ typedef enum {
   SCTL_AMD_ERROR_QUEUE = 7           /* batched work queue did not drain as planned */
 } sctl_amd_status;
 
-/* Micro-kernel ids: the real-valued aliases of kernel_functions.hpp:669-684
- * (the Helmholtz kernels are complex-valued and not provided).  r = x_t - x_s;
- * K0 densities per source, K1 outputs per target. */
+/* Micro-kernel ids: the aliases of kernel_functions.hpp:669-684.  r = x_t - x_s;
+ * K0 densities per source, K1 outputs per target.  The Helmholtz kernels
+ * (ids 12..15) take the wavenumber mu through ctx (the *_ctx entry points,
+ * sctl_amd_plan_set_ctx), as the reference reads it through ctx_ptr
+ * (kernel_functions.hpp:485-489); their complex densities and outputs are
+ * (re, im) pairs. */
 enum {
   SCTL_AMD_LAPLACE3D_FXU = 0,    /* K = 1/r,                  scale  1/(4 pi), K0=1, K1=1, no normal */
   SCTL_AMD_LAPLACE3D_DXU = 1,    /* K = (r.n)/r^3,            scale  1/(4 pi), K0=1, K1=1, normal    */
@@ -59,7 +62,11 @@ enum {
   SCTL_AMD_STOKES3D_FSXU = 8,    /* [f, q] -> f/r + r (r.f + q)/r^3, scale 1/(8 pi), K0=4, K1=3     */
   SCTL_AMD_STOKES3D_FXUP = 9,    /* f -> [Stokes FxU f, (r.f)/r^3], scale 1/(8 pi), K0=3, K1=4      */
   SCTL_AMD_BIOTSAVART3D_FXU = 10, /* f -> (f x r)/r^3,         scale  1/(4 pi), K0=3, K1=3           */
-  SCTL_AMD_BIOTSAVART3D_FXDU = 11 /* gradient of BiotSavart FxU, scale -1/(4 pi), K0=3, K1=9          */
+  SCTL_AMD_BIOTSAVART3D_FXDU = 11, /* gradient of BiotSavart FxU, scale -1/(4 pi), K0=3, K1=9         */
+  SCTL_AMD_HELMHOLTZ3D_FXU = 12,  /* K = e^{i mu r}/r,           scale  1/(4 pi), K0=2, K1=2           */
+  SCTL_AMD_HELMHOLTZ3D_DXU = 13,  /* K = (i mu r - 1) e^{i mu r} (r.n)/r^3, scale -1/(4 pi), K0=2, K1=2, normal */
+  SCTL_AMD_HELMHOLTZ3D_FXDU = 14, /* K = (i mu r - 1) e^{i mu r} r/r^3, scale 1/(4 pi), K0=2, K1=6      */
+  SCTL_AMD_HELMHOLTZDIFF3D_FXDU = 15 /* Helmholtz FxdU minus Laplace's, scale 1/(4 pi), K0=2, K1=6     */
 };
 
 /* Arithmetic of an evaluation. */
@@ -87,7 +94,8 @@ const char* sctl_amd_version(void);
 
 /* Kernel metadata: CoordDim/SrcDim/TrgDim/NormalDim (generic-kernel.hpp:48-62),
  * the reference's FLOPS() Profile count and uKerScaleFactor<double>()
- * (kernel_functions.hpp:37-41,66-70,130-134,185-189), and Name().
+ * (kernel_functions.hpp:37-41,66-70,130-134,185-189 and the other kernels'
+ * FLOPS()/uKerScaleFactor()), and Name().
  * Any output pointer may be NULL. */
 int sctl_amd_kernel_info(int kernel, int* coord_dim, int* src_dim, int* trg_dim, int* normal_dim, int* flops,
                          double* scale, const char** name);
@@ -108,6 +116,14 @@ int sctl_amd_kernel_info(int kernel, int* coord_dim, int* src_dim, int* trg_dim,
 int sctl_amd_eval(int kernel, int precision, int digits, const double* r_trg, int64_t nt, const double* r_src,
                   const double* n_src, const double* v_src, int64_t ns, double* v_trg, uint32_t flags, void* stream);
 
+/* sctl_amd_eval with the kernel's context (GenericKernel::SetCtxPtr,
+ * generic-kernel.txx:64-66): ctx points to the wavenumber (one double, read
+ * at the call) for the Helmholtz kernels, which fail without it; the other
+ * kernels ignore ctx.  sctl_amd_eval(...) is sctl_amd_eval_ctx(kernel, NULL, ...). */
+int sctl_amd_eval_ctx(int kernel, const double* ctx, int precision, int digits, const double* r_trg, int64_t nt,
+                      const double* r_src, const double* n_src, const double* v_src, int64_t ns, double* v_trg,
+                      uint32_t flags, void* stream);
+
 /* ------------------------------------------------------------ KernelMatrix
  * Replaces GenericKernel<uKernel>::KernelMatrix<double,omp,digits>
  * (generic-kernel.hpp:124, generic-kernel.txx:213-329):
@@ -117,6 +133,11 @@ int sctl_amd_eval(int kernel, int precision, int digits, const double* r_trg, in
  * M is row-major (ns*K0) x (nt*K1) and is overwritten. FP64. */
 int sctl_amd_kernel_matrix(int kernel, int digits, const double* r_trg, int64_t nt, const double* r_src,
                            const double* n_src, int64_t ns, double* M, uint32_t flags, void* stream);
+
+/* sctl_amd_kernel_matrix with the kernel's context (as sctl_amd_eval_ctx). */
+int sctl_amd_kernel_matrix_ctx(int kernel, const double* ctx, int digits, const double* r_trg, int64_t nt,
+                               const double* r_src, const double* n_src, int64_t ns, double* M, uint32_t flags,
+                               void* stream);
 
 /* ------------------------------------------------------------ batched blocks
  * A plan holds nblocks (target-set, source-set) blocks over one target array
@@ -143,6 +164,11 @@ int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int dig
  * The plan keeps its own device copy, so the caller's buffers may be reused. */
 int sctl_amd_plan_set_geometry(sctl_amd_plan plan, const double* r_trg, const double* r_src, const double* n_src,
                                uint32_t flags, void* stream);
+
+/* The kernel's context for later applies and near-field matrices: the
+ * wavenumber (*ctx, copied now) of the Helmholtz kernels, which need it
+ * (GenericKernel::SetCtxPtr, generic-kernel.txx:64-66).  NULL clears it. */
+int sctl_amd_plan_set_ctx(sctl_amd_plan plan, const double* ctx);
 
 /* v_trg (nt_total*K1) += block sums for densities v_src (ns_total*K0).
  * Allocates nothing; may be captured into a hipGraph. */

@@ -23,10 +23,11 @@
 // they are.  sctl_amd/containers.hpp provides minimal owning containers for
 // hosts without SCTL.
 //
-// The twelve real-valued micro-kernels of the device library exist here
-// (Laplace3D FxU/DxU/FxdU/DxdU/Fxd2U, Stokes3D FxU/DxU/FxT/FSxU/FxUP,
-// BiotSavart3D FxU/FxdU); any other uKernel (the complex Helmholtz kernels)
-// is a compile-time error.
+// All sixteen micro-kernels of kernel_functions.hpp exist here (Laplace3D
+// FxU/DxU/FxdU/DxdU/Fxd2U, Stokes3D FxU/DxU/FxT/FSxU/FxUP, BiotSavart3D
+// FxU/FxdU, Helmholtz3D FxU/DxU/FxdU, HelmholtzDiff3D FxdU; the Helmholtz
+// kernels read their wavenumber through SetCtxPtr); any other uKernel is a
+// compile-time error.
 //
 // Precision: Real = double runs FP64 unless 0 <= digits <= 7, which selects
 // the mixed-precision form (FP64 differencing, FP32 pairwise, Kahan FP64
@@ -148,6 +149,12 @@ SCTL_AMD_UKERNEL(Stokes3D_FSxU, SCTL_AMD_STOKES3D_FSXU, 4, 3, 0, "Stokes3D-FSxU"
 SCTL_AMD_UKERNEL(Stokes3D_FxUP, SCTL_AMD_STOKES3D_FXUP, 3, 4, 0, "Stokes3D-FxUP", 26, 1, 8);          // :325-365
 SCTL_AMD_UKERNEL(BiotSavart3D_FxU, SCTL_AMD_BIOTSAVART3D_FXU, 3, 3, 0, "BiotSavart3D-FxU", 15, 1, 4); // :416-450
 SCTL_AMD_UKERNEL(BiotSavart3D_FxdU, SCTL_AMD_BIOTSAVART3D_FXDU, 3, 9, 0, "BiotSavart3D-FxdU", 51, -1, 4);  // :451-489
+// The Helmholtz kernels read the wavenumber through SetCtxPtr (a pointer to one Real).
+SCTL_AMD_UKERNEL(Helmholtz3D_FxU, SCTL_AMD_HELMHOLTZ3D_FXU, 2, 2, 0, "Helmholtz3D-FxU", 20, 1, 4);    // :491-527
+SCTL_AMD_UKERNEL(Helmholtz3D_DxU, SCTL_AMD_HELMHOLTZ3D_DXU, 2, 2, 3, "Helmholtz3D-DxU", 29, -1, 4);   // :529-571
+SCTL_AMD_UKERNEL(Helmholtz3D_FxdU, SCTL_AMD_HELMHOLTZ3D_FXDU, 2, 6, 0, "Helmholtz3D-FxdU", 35, 1, 4);  // :573-620
+SCTL_AMD_UKERNEL(HelmholtzDiff3D_FxdU, SCTL_AMD_HELMHOLTZDIFF3D_FXDU, 2, 6, 0, "HelmholtzDiff3D-FxdU", 41, 1,
+                 4);  // :627-659
 #undef SCTL_AMD_UKERNEL
 template <> struct is_device_kernel<Laplace3D_FxU> : std::true_type {};
 template <> struct is_device_kernel<Laplace3D_DxU> : std::true_type {};
@@ -158,8 +165,8 @@ template <> struct is_device_kernel<Stokes3D_DxU> : std::true_type {};
 
 template <class uKernel> class GenericKernel : public uKernel {
   // Only the kernel_impl types above (which carry a device kernel id) run on
-  // the device; sctl::kernel_impl types (e.g. the complex Helmholtz kernels)
-  // fail here rather than fall back to the CPU.
+  // the device; sctl::kernel_impl types fail here rather than fall back to
+  // the CPU.
   static_assert(kernel_impl::is_device_kernel<uKernel>::value,
                 "sctl_amd::GenericKernel: only the sctl_amd::kernel_impl micro-kernels run on the device");
   static constexpr Integer DIM = uKernel::DIM, KDIM0 = uKernel::KDIM0, KDIM1 = uKernel::KDIM1,
@@ -181,14 +188,15 @@ template <class uKernel> class GenericKernel : public uKernel {
   template <class Real, bool enable_openmp, class VectorT>
   static void Eval(VectorT& v_trg, const VectorT& r_trg, const VectorT& r_src, const VectorT& n_src,
                    const VectorT& v_src, Integer digits, const char* self) {
-    (void)self;  // the four kernels carry no context
-    EvalImpl<Real>(v_trg, r_trg, r_src, n_src, v_src, (digits >= 16 ? -1 : digits));
+    // self is the kernel object (generic-kernel.txx:75): its context travels along.
+    const void* ctx = self ? reinterpret_cast<const GenericKernel*>(self)->ctx_ptr : nullptr;
+    EvalImpl<Real>(v_trg, r_trg, r_src, n_src, v_src, (digits >= 16 ? -1 : digits), ctx);
   }
 
   template <class Real, bool enable_openmp = false, Integer digits = -1, class VectorT>
   void Eval(VectorT& v_trg, const VectorT& r_trg, const VectorT& r_src, const VectorT& n_src,
             const VectorT& v_src) const {
-    EvalImpl<Real>(v_trg, r_trg, r_src, n_src, v_src, digits);
+    EvalImpl<Real>(v_trg, r_trg, r_src, n_src, v_src, digits, ctx_ptr);
   }
 
   template <class Real, bool enable_openmp = false, Integer digits = -1, class MatrixT, class VectorT>
@@ -206,19 +214,30 @@ template <class uKernel> class GenericKernel : public uKernel {
     const double* ps = as_double(Xs, xs);
     const double* pn = N_DIM ? as_double(Xn, xn) : nullptr;
     const Long n = Ns * KDIM0 * Nt * KDIM1;
+    double mu;
+    const double* ctx = ctx_value<Real>(ctx_ptr, &mu);
     if (std::is_same<Real, double>::value) {
-      SCTL_AMD_CHECK(sctl_amd_kernel_matrix(uKernel::KERNEL_ID, (int)(digits >= 16 ? -1 : digits), pt, Nt, ps, pn,
-                                            Ns, reinterpret_cast<double*>(&M[0][0]), SCTL_AMD_HOST_PTRS, nullptr));
+      SCTL_AMD_CHECK(sctl_amd_kernel_matrix_ctx(uKernel::KERNEL_ID, ctx, (int)(digits >= 16 ? -1 : digits), pt, Nt,
+                                                ps, pn, Ns, reinterpret_cast<double*>(&M[0][0]), SCTL_AMD_HOST_PTRS,
+                                                nullptr));
     } else {
       m.resize(n);
-      SCTL_AMD_CHECK(sctl_amd_kernel_matrix(uKernel::KERNEL_ID, (int)(digits >= 16 ? -1 : digits), pt, Nt, ps, pn,
-                                            Ns, m.data(), SCTL_AMD_HOST_PTRS, nullptr));
+      SCTL_AMD_CHECK(sctl_amd_kernel_matrix_ctx(uKernel::KERNEL_ID, ctx, (int)(digits >= 16 ? -1 : digits), pt, Nt,
+                                                ps, pn, Ns, m.data(), SCTL_AMD_HOST_PTRS, nullptr));
       Real* out = &M[0][0];
       for (Long i = 0; i < n; i++) out[i] = (Real)m[i];
     }
   }
 
  private:
+  // The kernels read their context as *(const Real*)ctx_ptr (kernel_functions.hpp:485-489);
+  // the library takes it as a double.
+  template <class Real> static const double* ctx_value(const void* ctx, double* tmp) {
+    if (!ctx) return nullptr;
+    *tmp = (double)*static_cast<const Real*>(ctx);
+    return tmp;
+  }
+
   template <class VectorT> static const double* as_double(const VectorT& v, std::vector<double>& tmp) {
     using T = typename std::remove_cv<typename std::remove_reference<decltype(v[0])>::type>::type;
     if (std::is_same<T, double>::value) return v.Dim() ? reinterpret_cast<const double*>(&v[0]) : nullptr;
@@ -229,7 +248,7 @@ template <class uKernel> class GenericKernel : public uKernel {
 
   template <class Real, class VectorT>
   static void EvalImpl(VectorT& v_trg, const VectorT& r_trg, const VectorT& r_src, const VectorT& n_src,
-                       const VectorT& v_src, Integer digits) {
+                       const VectorT& v_src, Integer digits, const void* ctx_in) {
     const Long Ns = r_src.Dim() / DIM;
     const Long Nt = r_trg.Dim() / DIM;
     SCTL_AMD_ASSERT(r_trg.Dim() == Nt * DIM);
@@ -248,13 +267,15 @@ template <class uKernel> class GenericKernel : public uKernel {
     const double* ps = as_double(r_src, xs);
     const double* pn = N_DIM ? as_double(n_src, xn) : nullptr;
     const double* pv = as_double(v_src, vs);
+    double mu;
+    const double* ctx = ctx_value<Real>(ctx_in, &mu);
     if (is_double) {
-      SCTL_AMD_CHECK(sctl_amd_eval(uKernel::KERNEL_ID, precision, (int)digits, pt, Nt, ps, pn, pv, Ns,
-                                   reinterpret_cast<double*>(&v_trg[0]), SCTL_AMD_HOST_PTRS, nullptr));
+      SCTL_AMD_CHECK(sctl_amd_eval_ctx(uKernel::KERNEL_ID, ctx, precision, (int)digits, pt, Nt, ps, pn, pv, Ns,
+                                       reinterpret_cast<double*>(&v_trg[0]), SCTL_AMD_HOST_PTRS, nullptr));
     } else {
       vt.assign(Nt * KDIM1, 0.0);
-      SCTL_AMD_CHECK(sctl_amd_eval(uKernel::KERNEL_ID, precision, (int)digits, pt, Nt, ps, pn, pv, Ns, vt.data(),
-                                   SCTL_AMD_HOST_PTRS, nullptr));
+      SCTL_AMD_CHECK(sctl_amd_eval_ctx(uKernel::KERNEL_ID, ctx, precision, (int)digits, pt, Nt, ps, pn, pv, Ns,
+                                       vt.data(), SCTL_AMD_HOST_PTRS, nullptr));
       for (Long i = 0; i < Nt * KDIM1; i++) v_trg[i] += (Real)vt[i];
     }
   }
@@ -275,6 +296,10 @@ using Stokes3D_FSxU = GenericKernel<kernel_impl::Stokes3D_FSxU>;
 using Stokes3D_FxUP = GenericKernel<kernel_impl::Stokes3D_FxUP>;
 using BiotSavart3D_FxU = GenericKernel<kernel_impl::BiotSavart3D_FxU>;
 using BiotSavart3D_FxdU = GenericKernel<kernel_impl::BiotSavart3D_FxdU>;
+using Helmholtz3D_FxU = GenericKernel<kernel_impl::Helmholtz3D_FxU>;
+using Helmholtz3D_DxU = GenericKernel<kernel_impl::Helmholtz3D_DxU>;
+using Helmholtz3D_FxdU = GenericKernel<kernel_impl::Helmholtz3D_FxdU>;
+using HelmholtzDiff3D_FxdU = GenericKernel<kernel_impl::HelmholtzDiff3D_FxdU>;
 
 }  // namespace sctl_amd
 

@@ -27,9 +27,13 @@ LAPLACE3D_FXU, LAPLACE3D_DXU, STOKES3D_FXU, STOKES3D_DXU = 0, 1, 2, 3
 LAPLACE3D_FXDU, LAPLACE3D_DXDU, LAPLACE3D_FXD2U = 4, 5, 6
 STOKES3D_FXT, STOKES3D_FSXU, STOKES3D_FXUP = 7, 8, 9
 BIOTSAVART3D_FXU, BIOTSAVART3D_FXDU = 10, 11
+HELMHOLTZ3D_FXU, HELMHOLTZ3D_DXU, HELMHOLTZ3D_FXDU, HELMHOLTZDIFF3D_FXDU = 12, 13, 14, 15
 KERNEL_NAMES = {0: "Laplace3D_FxU", 1: "Laplace3D_DxU", 2: "Stokes3D_FxU", 3: "Stokes3D_DxU",
                 4: "Laplace3D_FxdU", 5: "Laplace3D_DxdU", 6: "Laplace3D_Fxd2U", 7: "Stokes3D_FxT",
-                8: "Stokes3D_FSxU", 9: "Stokes3D_FxUP", 10: "BiotSavart3D_FxU", 11: "BiotSavart3D_FxdU"}
+                8: "Stokes3D_FSxU", 9: "Stokes3D_FxUP", 10: "BiotSavart3D_FxU", 11: "BiotSavart3D_FxdU",
+                12: "Helmholtz3D_FxU", 13: "Helmholtz3D_DxU", 14: "Helmholtz3D_FxdU", 15: "HelmholtzDiff3D_FxdU"}
+# Kernels that take a context value (the wavenumber), as GenericKernel::SetCtxPtr.
+CTX_KERNELS = frozenset(range(12, 16))
 NUM_KERNELS = len(KERNEL_NAMES)
 FP64, MIXED, AUTO = 0, 1, 2
 HOST_PTRS, MIXED_NEWTON = 0x1, 0x2
@@ -49,7 +53,8 @@ SYMBOLS = [
     "sctl_amd_plan_set_shape", "sctl_amd_tune_db_path", "sctl_amd_tune_eval", "sctl_amd_plan_tune",
     "sctl_amd_near_blocks", "sctl_amd_plan_near_layout", "sctl_amd_plan_near_matrix", "sctl_amd_plan_near_gemv",
     "sctl_amd_near_accumulate", "sctl_amd_dense_shape_get", "sctl_amd_dense_shape_set", "sctl_amd_near_list_build",
-    "sctl_amd_near_list_size", "sctl_amd_near_list_get", "sctl_amd_near_list_destroy",
+    "sctl_amd_near_list_size", "sctl_amd_near_list_get", "sctl_amd_near_list_destroy", "sctl_amd_eval_ctx",
+    "sctl_amd_kernel_matrix_ctx", "sctl_amd_plan_set_ctx",
 ]
 
 
@@ -83,7 +88,10 @@ def lib():
     L.sctl_amd_init.argtypes = [i32]
     L.sctl_amd_kernel_info.argtypes = [i32] + [ctypes.POINTER(i32)] * 5 + [_dp, ctypes.POINTER(ctypes.c_char_p)]
     L.sctl_amd_eval.argtypes = [i32, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp]
+    L.sctl_amd_eval_ctx.argtypes = [i32, _dp, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp]
     L.sctl_amd_kernel_matrix.argtypes = [i32, i32, _vp, i64, _vp, _vp, i64, _vp, u32, _vp]
+    L.sctl_amd_kernel_matrix_ctx.argtypes = [i32, _dp, i32, _vp, i64, _vp, _vp, i64, _vp, u32, _vp]
+    L.sctl_amd_plan_set_ctx.argtypes = [_vp, _dp]
     L.sctl_amd_plan_create.argtypes = [ctypes.POINTER(_vp), i32, i32, i32, i64, _i64p, _i64p, _i64p, _i64p, i64, i64]
     L.sctl_amd_plan_set_geometry.argtypes = [_vp, _vp, _vp, _vp, u32, _vp]
     L.sctl_amd_plan_apply.argtypes = [_vp, _vp, _vp, u32, _vp]
@@ -204,6 +212,11 @@ def _host(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _ctx(ctx):
+    """Pointer to one double (the wavenumber of the Helmholtz kernels), or NULL."""
+    return None if ctx is None else ctypes.pointer(ctypes.c_double(float(ctx)))
+
+
 def _ptr(a):
     if a is None:
         return None
@@ -213,12 +226,13 @@ def _ptr(a):
 
 
 # ----------------------------------------------------------------- Eval API
-def eval(kernel, r_trg, r_src, n_src, v_src, v_trg=None, precision=FP64, digits=-1, flags=0, stream=None):
+def eval(kernel, r_trg, r_src, n_src, v_src, v_trg=None, precision=FP64, digits=-1, flags=0, stream=None, ctx=None):
     """v_trg += Eval(...).
 
     With numpy arrays the call stages through the device and returns the
     (new, or updated in place) host v_trg.  With DeviceArray arguments the call
-    is asynchronous on `stream` and returns v_trg."""
+    is asynchronous on `stream` and returns v_trg.  ctx: the wavenumber of the
+    Helmholtz kernels."""
     _, K0, K1, ND, _, _, _ = kernel_info(kernel)
     dev = isinstance(r_trg, DeviceArray)
     if not dev:
@@ -232,12 +246,12 @@ def eval(kernel, r_trg, r_src, n_src, v_src, v_trg=None, precision=FP64, digits=
         nt, ns = r_trg.nbytes // 24, r_src.nbytes // 24
         if not ND:
             n_src = None
-    check(lib().sctl_amd_eval(kernel, precision, digits, _ptr(r_trg), nt, _ptr(r_src), _ptr(n_src), _ptr(v_src), ns,
-                              _ptr(v_trg), flags, stream))
+    check(lib().sctl_amd_eval_ctx(kernel, _ctx(ctx), precision, digits, _ptr(r_trg), nt, _ptr(r_src), _ptr(n_src),
+                                  _ptr(v_src), ns, _ptr(v_trg), flags, stream))
     return v_trg
 
 
-def kernel_matrix(kernel, r_trg, r_src, n_src, digits=-1, M=None, flags=0, stream=None):
+def kernel_matrix(kernel, r_trg, r_src, n_src, digits=-1, M=None, flags=0, stream=None, ctx=None):
     _, K0, K1, ND, _, _, _ = kernel_info(kernel)
     dev = isinstance(r_trg, DeviceArray)
     if not dev:
@@ -251,18 +265,26 @@ def kernel_matrix(kernel, r_trg, r_src, n_src, digits=-1, M=None, flags=0, strea
         nt, ns = r_trg.nbytes // 24, r_src.nbytes // 24
         if not ND:
             n_src = None
-    check(lib().sctl_amd_kernel_matrix(kernel, digits, _ptr(r_trg), nt, _ptr(r_src), _ptr(n_src), ns, _ptr(M),
-                                       flags, stream))
+    check(lib().sctl_amd_kernel_matrix_ctx(kernel, _ctx(ctx), digits, _ptr(r_trg), nt, _ptr(r_src), _ptr(n_src), ns,
+                                           _ptr(M), flags, stream))
     return M
 
 
 class GenericKernel:
-    """Python mirror of sctl::GenericKernel<uKernel> for the four kernels."""
+    """Python mirror of sctl::GenericKernel<uKernel> (SetCtxPtr takes the
+    wavenumber itself)."""
 
     def __init__(self, kernel):
         (self._coord, self._k0, self._k1, self._nd, self._flops, self._scale,
          self._name) = kernel_info(kernel)
         self.kernel = kernel
+        self._ctx = None
+
+    def SetCtxPtr(self, ctx):
+        self._ctx = ctx
+
+    def GetCtxPtr(self):
+        return self._ctx
 
     def CoordDim(self):
         return self._coord
@@ -303,17 +325,18 @@ class GenericKernel:
             v = np.zeros(nt * self._k1)
         else:
             v = v.copy()
-        return eval(self.kernel, r_trg, r_src, n_src, v_src, v, precision=precision, digits=digits)
+        return eval(self.kernel, r_trg, r_src, n_src, v_src, v, precision=precision, digits=digits, ctx=self._ctx)
 
     def KernelMatrix(self, r_trg, r_src, n_src, digits=-1):
-        return kernel_matrix(self.kernel, r_trg, r_src, n_src, digits=digits)
+        return kernel_matrix(self.kernel, r_trg, r_src, n_src, digits=digits, ctx=self._ctx)
 
 
 # ------------------------------------------------------------------ batched
 class Plan:
     """Batched (target-set, source-set) blocks evaluated in one persistent launch."""
 
-    def __init__(self, kernel, trg_dsp, trg_cnt, src_dsp, src_cnt, nt_total, ns_total, precision=FP64, digits=-1):
+    def __init__(self, kernel, trg_dsp, trg_cnt, src_dsp, src_cnt, nt_total, ns_total, precision=FP64, digits=-1,
+                 ctx=None):
         a = [np.ascontiguousarray(x, dtype=np.int64) for x in (trg_dsp, trg_cnt, src_dsp, src_cnt)]
         self._keep = a
         self.kernel = kernel
@@ -323,6 +346,12 @@ class Plan:
         check(lib().sctl_amd_plan_create(ctypes.byref(h), kernel, precision, digits, self.nblocks, P(a[0]), P(a[1]),
                                          P(a[2]), P(a[3]), int(nt_total), int(ns_total)))
         self.handle = h
+        if ctx is not None:
+            self.set_ctx(ctx)
+
+    def set_ctx(self, ctx):
+        """The wavenumber of the Helmholtz kernels (None clears it)."""
+        check(lib().sctl_amd_plan_set_ctx(self.handle, _ctx(ctx)))
 
     def set_geometry(self, r_trg, r_src, n_src=None, stream=None):
         flags = 0
@@ -544,6 +573,7 @@ class NearField:
                             the element's nodes, so F is indexed the same way
       matrix_free           per element: no K_near matrix (MatrixFree())
       trg_normal_dot        contract the output with the target normal
+      ctx                   the wavenumber of the Helmholtz kernels
 
     Methods:
       k_near_subtract(K_near)           KNearSubtract (:1165-1240)
@@ -555,7 +585,7 @@ class NearField:
 
     def __init__(self, kernel, near_elem_cnt, near_elem_dsp, elem_nds_cnt_far, elem_nds_dsp_far, Xtrg_near,
                  X_far, Xn_far=None, wts_far=None, Xn_trg_near=None, matrix_free=None, trg_normal_dot=False,
-                 nthreads=None, blocks=None):
+                 nthreads=None, blocks=None, ctx=None):
         _, K0, K1, ND, _, _, _ = kernel_info(kernel)
         self.kernel, self.K0, self.K1 = kernel, K0, K1
         self.dof = K1 // 3 if trg_normal_dot else K1
@@ -573,7 +603,7 @@ class NearField:
         nt_total = int(dsp[-1] + cnt[-1]) if nelem else 0
         ns_total = int(np.max(ndf + ncf)) if nelem else 0
         self.nt_total, self.ns_total = nt_total, ns_total
-        self.plan = Plan(kernel, dsp[be] + bt, bc, ndf[be], ncf[be], nt_total, ns_total)
+        self.plan = Plan(kernel, dsp[be] + bt, bc, ndf[be], ncf[be], nt_total, ns_total, ctx=ctx)
         self.plan.set_geometry(Xtrg_near, X_far, Xn_far if ND else None)
         need = self.plan.near_layout(off, trg_normal_dot)
         assert need <= self.k_len

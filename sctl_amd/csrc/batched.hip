@@ -72,7 +72,8 @@ __global__ void __launch_bounds__(BWG)
 k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ counter,
              const int64_t* __restrict__ trg_dsp, const int64_t* __restrict__ src_dsp,
              const int32_t* __restrict__ src_cnt, const double* __restrict__ xt, const double* __restrict__ xs,
-             const double* __restrict__ xn, const double* __restrict__ fs, double* __restrict__ out, double scale) {
+             const double* __restrict__ xn, const double* __restrict__ fs, double* __restrict__ out, double scale,
+             double mu) {
   using K = Ker<KER>;
   constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, R = rec_f64<KER>(), STAGE = 64 * SPL;
   __shared__ double lds_all[(BWG / 64) * STAGE * R];
@@ -138,7 +139,7 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
         for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
 #pragma unroll
         for (int j = 0; j < TPL; j++)
-          interact<KER, NEWTON, false>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f);
+          interact<KER, NEWTON, false>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f, mu);
       }
       bool bad = false;
 #pragma unroll
@@ -157,7 +158,7 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
           for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
 #pragma unroll
           for (int j = 0; j < TPL; j++)
-            interact<KER, NEWTON, true>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f);
+            interact<KER, NEWTON, true>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f, mu);
         }
       }
     }
@@ -187,7 +188,8 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
                const int64_t* __restrict__ trg_dsp, const int64_t* __restrict__ src_dsp,
                const int32_t* __restrict__ src_cnt, const double* __restrict__ xt, const double* __restrict__ xs,
                const double* __restrict__ xn, const double* __restrict__ fs, double* __restrict__ out,
-               double scale) {
+               double scale, double ctx) {
+  const float mu = (float)ctx;
   using K = Ker<KER>;
   constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, STAGE = 64 * SPL;
   constexpr int AW = ((ND + K0) + 3) & ~3;  // float words of normal + density
@@ -259,7 +261,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
 #pragma unroll
         for (int j = 0; j < TPL; j++)
           interact<KER, NEWTON, false>(p[j], (float)(tx[j] - pr[0]), (float)(ty[j] - pr[1]), (float)(tz[j] - pr[2]),
-                                       n, f);
+                                       n, f, mu);
       }
       bool bad = false;
 #pragma unroll
@@ -280,7 +282,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
 #pragma unroll
           for (int j = 0; j < TPL; j++)
             interact<KER, NEWTON, true>(p[j], (float)(tx[j] - pr[0]), (float)(ty[j] - pr[1]), (float)(tz[j] - pr[2]),
-                                        n, f);
+                                        n, f, mu);
         }
       }
 #pragma unroll
@@ -310,7 +312,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
 }
 
 using BlocksFn = void (*)(const Item*, int, int*, const int64_t*, const int64_t*, const int32_t*, const double*,
-                          const double*, const double*, const double*, double*, double);
+                          const double*, const double*, const double*, double*, double, double);
 
 // Targets per lane: 1 or 2 (2 only where max_tpl allows it).
 template <int KER, int NEWTON, int SPL> BlocksFn f64_blocks_t(int tpl) {
@@ -449,7 +451,7 @@ int plan_launch(sctl_amd_plan_s* p, const double* v_src, double* v_trg, hipStrea
   if (cs == hipStreamCaptureStatusNone) profile_count(p->kernel, p->interactions);  // graphs count per launch
   ProfileScope prof(s);
   hipLaunchKernelGGL(plan_fn(p, p->shape), dim3(grid), dim3(BWG), p->dyn_lds, s, p->d_items, p->nitems, p->d_counter,
-                     p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, v_src, v_trg, scale);
+                     p->d_trg_dsp, p->d_src_dsp, p->d_src_cnt, p->d_xt, p->d_xs, p->d_xn, v_src, v_trg, scale, p->ctx);
   SCTL_HIP_TRY(hipGetLastError());
   return SCTL_AMD_SUCCESS;
 }
@@ -572,9 +574,19 @@ int sctl_amd_plan_set_geometry(sctl_amd_plan p, const double* r_trg, const doubl
   return SCTL_AMD_SUCCESS;
 }
 
+int sctl_amd_plan_set_ctx(sctl_amd_plan p, const double* ctx) {
+  SCTL_CHECK_ARG(p, "plan is NULL");
+  SCTL_CHECK_ARG(!ctx || std::isfinite(*ctx), "the context value is not finite");
+  p->has_ctx = ctx != nullptr;
+  p->ctx = ctx ? *ctx : 0.0;
+  return SCTL_AMD_SUCCESS;
+}
+
 int sctl_amd_plan_apply(sctl_amd_plan p, const double* v_src, double* v_trg, uint32_t flags, void* stream) {
   SCTL_CHECK_ARG(p, "plan is NULL");
   SCTL_CHECK_ARG(p->has_geometry, "plan has no geometry (call sctl_amd_plan_set_geometry first)");
+  SCTL_CHECK_ARG(!needs_ctx(p->kernel) || p->has_ctx,
+                 "the kernel needs a wavenumber (call sctl_amd_plan_set_ctx first)");
   SCTL_CHECK_ARG((flags & ~SCTL_AMD_HOST_PTRS) == 0, "unknown flag bits");
   SCTL_CHECK_ARG(p->ns_total == 0 || v_src, "v_src is NULL");
   SCTL_CHECK_ARG(p->nt_total == 0 || v_trg, "v_trg is NULL");

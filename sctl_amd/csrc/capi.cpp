@@ -21,12 +21,14 @@ struct KernelMeta {
 };
 // GenericKernel dims (generic-kernel.txx:48-62) and FLOPS()/Name()
 // (kernel_functions.hpp:33-39,62-67,82-88,126-131,181-186,221-227,265-271,
-// 326-332,367-373,393-399,417-423,452-458).
+// 326-332,367-373,393-399,417-423,452-458,492-498,530-536,574-580,628-634).
 constexpr KernelMeta META[NUM_KERNELS] = {
     {1, 1, 0, 6, "Laplace3D-FxU"},      {1, 1, 3, 14, "Laplace3D-DxU"},    {3, 3, 0, 23, "Stokes3D-FxU"},
     {3, 3, 3, 26, "Stokes3D-DxU"},      {1, 3, 0, 11, "Laplace3D-FxdU"},   {1, 3, 3, 22, "Laplace3D-DxdU"},
     {1, 9, 0, 25, "Laplace3D-Fxd2U"},   {3, 9, 0, 39, "Stokes3D-FxT"},     {4, 3, 0, 26, "Stokes3D-FSxU"},
     {3, 4, 0, 26, "Stokes3D-FxUP"},     {3, 3, 0, 15, "BiotSavart3D-FxU"}, {3, 9, 0, 51, "BiotSavart3D-FxdU"},
+    {2, 2, 0, 20, "Helmholtz3D-FxU"},   {2, 2, 3, 29, "Helmholtz3D-DxU"},  {2, 6, 0, 35, "Helmholtz3D-FxdU"},
+    {2, 6, 0, 41, "HelmholtzDiff3D-FxdU"},
 };
 static_assert(Ker<LAP_FXD2U>::K1 == 9 && Ker<STK_FSXU>::K0 == 4 && Ker<LAP_DXDU>::ND == 3, "tables disagree");
 
@@ -55,24 +57,30 @@ int kernel_dims(int ker, int* K0, int* K1, int* ND) {
 }
 
 // uKerScaleFactor<double>() (kernel_functions.hpp:40,69,89,133,188,228,272,
-// 333,374,400,424,459).
+// 333,374,400,424,459,499,537,581,635).
 double kernel_scale(int ker) {
   switch (ker) {
     case LAP_FXU:
     case LAP_DXU:
     case LAP_DXDU:
     case LAP_FXD2U:
-    case BS_FXU: return 1 / (4 * const_pi());
+    case BS_FXU:
+    case HLM_FXU:
+    case HLM_FXDU:
+    case HLMD_FXDU: return 1 / (4 * const_pi());
     case STK_FXU:
     case STK_FSXU:
     case STK_FXUP: return 1 / (8 * const_pi());
     case STK_DXU: return 3 / (4 * const_pi());
     case LAP_FXDU:
-    case BS_FXDU: return -1 / (4 * const_pi());
+    case BS_FXDU:
+    case HLM_DXU: return -1 / (4 * const_pi());
     case STK_FXT: return -3 / (4 * const_pi());
   }
   return 0;
 }
+
+bool kernel_needs_ctx(int ker) { return needs_ctx(ker); }
 
 int kernel_max_tpl(int ker) {
   if (ker < 0 || ker >= NUM_KERNELS) return 1;
@@ -150,6 +158,16 @@ int check_common(int kernel, const double* r_trg, int64_t nt, const double* r_sr
   return SCTL_AMD_SUCCESS;
 }
 
+// The Helmholtz kernels read their wavenumber through ctx (a pointer to one
+// double, as the reference's ctx_ptr points to one Real); the others ignore it.
+int check_ctx(int kernel, const double* ctx) {
+  if (!needs_ctx(kernel)) return SCTL_AMD_SUCCESS;
+  SCTL_CHECK_ARG(ctx, std::string(META[kernel].name) + " needs its wavenumber (ctx is NULL)");
+  SCTL_CHECK_ARG(std::isfinite(*ctx), "the wavenumber is not finite");
+  return SCTL_AMD_SUCCESS;
+}
+double ctx_value(int kernel, const double* ctx) { return needs_ctx(kernel) && ctx ? *ctx : 0.0; }
+
 }  // namespace
 }  // namespace sctl_amd
 
@@ -193,7 +211,15 @@ int sctl_amd_kernel_info(int kernel, int* coord_dim, int* src_dim, int* trg_dim,
 int sctl_amd_eval(int kernel, int precision, int digits, const double* r_trg, int64_t nt, const double* r_src,
                   const double* n_src, const double* v_src, int64_t ns, double* v_trg, uint32_t flags,
                   void* stream) {
+  return sctl_amd_eval_ctx(kernel, nullptr, precision, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg, flags,
+                           stream);
+}
+
+int sctl_amd_eval_ctx(int kernel, const double* ctx, int precision, int digits, const double* r_trg, int64_t nt,
+                      const double* r_src, const double* n_src, const double* v_src, int64_t ns, double* v_trg,
+                      uint32_t flags, void* stream) {
   SCTL_TRY(check_common(kernel, r_trg, nt, r_src, n_src, ns, digits));
+  SCTL_TRY(check_ctx(kernel, ctx));
   SCTL_CHECK_ARG(ns == 0 || v_src, "v_src is NULL");
   SCTL_CHECK_ARG(nt == 0 || v_trg, "v_trg is NULL");
   SCTL_CHECK_ARG((flags & ~(SCTL_AMD_HOST_PTRS | SCTL_AMD_MIXED_NEWTON)) == 0, "unknown flag bits");
@@ -221,9 +247,9 @@ int sctl_amd_eval(int kernel, int precision, int digits, const double* r_trg, in
   const int newton = precision == SCTL_AMD_FP64 ? newton_steps_f64(digits) : ((flags & SCTL_AMD_MIXED_NEWTON) ? 1 : 0);
   const DenseShape shape = pick_dense_shape(kernel, precision, newton, nt, ns);
   if (precision == SCTL_AMD_FP64) {
-    SCTL_TRY(launch_eval_f64(kernel, newton, rt, nt, rs, nsrc, vs, ns, vt, shape, s));
+    SCTL_TRY(launch_eval_f64(kernel, newton, rt, nt, rs, nsrc, vs, ns, vt, shape, ctx_value(kernel, ctx), s));
   } else {
-    SCTL_TRY(launch_eval_mixed(kernel, newton, rt, nt, rs, nsrc, vs, ns, vt, shape, s));
+    SCTL_TRY(launch_eval_mixed(kernel, newton, rt, nt, rs, nsrc, vs, ns, vt, shape, ctx_value(kernel, ctx), s));
   }
   if (host) {
     SCTL_HIP_TRY(hipMemcpyAsync(v_trg, vt, sizeof(double) * nt * m.K1, hipMemcpyDeviceToHost, s));
@@ -235,7 +261,14 @@ int sctl_amd_eval(int kernel, int precision, int digits, const double* r_trg, in
 
 int sctl_amd_kernel_matrix(int kernel, int digits, const double* r_trg, int64_t nt, const double* r_src,
                            const double* n_src, int64_t ns, double* M, uint32_t flags, void* stream) {
+  return sctl_amd_kernel_matrix_ctx(kernel, nullptr, digits, r_trg, nt, r_src, n_src, ns, M, flags, stream);
+}
+
+int sctl_amd_kernel_matrix_ctx(int kernel, const double* ctx, int digits, const double* r_trg, int64_t nt,
+                               const double* r_src, const double* n_src, int64_t ns, double* M, uint32_t flags,
+                               void* stream) {
   SCTL_TRY(check_common(kernel, r_trg, nt, r_src, n_src, ns, digits));
+  SCTL_TRY(check_ctx(kernel, ctx));
   SCTL_CHECK_ARG(nt == 0 || ns == 0 || M, "M is NULL");
   SCTL_CHECK_ARG((flags & ~SCTL_AMD_HOST_PTRS) == 0, "unknown flag bits");
   DeviceInfo dev;
@@ -255,7 +288,8 @@ int sctl_amd_kernel_matrix(int kernel, int digits, const double* r_trg, int64_t 
     SCTL_TRY(b_M.alloc(sizeof(double) * nM, s));
     Md = b_M.as<double>();
   }
-  SCTL_TRY(launch_kernel_matrix(kernel, newton_steps_f64(digits), rt, nt, rs, nsrc, ns, Md, s));
+  SCTL_TRY(launch_kernel_matrix(kernel, newton_steps_f64(digits), rt, nt, rs, nsrc, ns, Md, ctx_value(kernel, ctx),
+                                s));
   if (host) {
     SCTL_HIP_TRY(hipMemcpyAsync(M, Md, sizeof(double) * nM, hipMemcpyDeviceToHost, s));
     b_rt.release(); b_rs.release(); b_ns.release(); b_M.release();

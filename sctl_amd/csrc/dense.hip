@@ -37,7 +37,7 @@ template <int KER, int TPL, int NEWTON>
 __global__ void __launch_bounds__(WG)
 k_eval_f64(const double* __restrict__ xt, int64_t nt, const double* __restrict__ xs, const double* __restrict__ xn,
            const double* __restrict__ fs, int64_t ns, int64_t chunk, double* __restrict__ out, int direct,
-           double scale) {
+           double scale, double mu) {
   using K = Ker<KER>;
   constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, R = rec_f64<KER>();
   __shared__ double lds[TILE * R];
@@ -97,7 +97,7 @@ k_eval_f64(const double* __restrict__ xt, int64_t nt, const double* __restrict__
 #pragma unroll
       for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
 #pragma unroll
-      for (int j = 0; j < TPL; j++) interact<KER, NEWTON, false>(acc[j], tx[j] - sx, ty[j] - sy, tz[j] - sz, n, f);
+      for (int j = 0; j < TPL; j++) interact<KER, NEWTON, false>(acc[j], tx[j] - sx, ty[j] - sy, tz[j] - sz, n, f, mu);
     }
 
     bool bad = false;
@@ -117,7 +117,7 @@ k_eval_f64(const double* __restrict__ xt, int64_t nt, const double* __restrict__
         for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
 #pragma unroll
         for (int j = 0; j < TPL; j++)
-          interact<KER, NEWTON, true>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f);
+          interact<KER, NEWTON, true>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f, mu);
       }
     }
   }
@@ -151,7 +151,7 @@ k_reduce_splits(const double* __restrict__ part, int64_t n, int nsplit, double* 
 template <int KER, int NEWTON>
 __global__ void __launch_bounds__(WG)
 k_kernel_matrix(const double* __restrict__ xt, int64_t nt, const double* __restrict__ xs,
-                const double* __restrict__ xn, int64_t ns, double* __restrict__ M, double scale) {
+                const double* __restrict__ xn, int64_t ns, double* __restrict__ M, double scale, double mu) {
   using K = Ker<KER>;
   constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND;
   const int64_t t = (int64_t)blockIdx.x * WG + threadIdx.x;
@@ -163,7 +163,7 @@ k_kernel_matrix(const double* __restrict__ xt, int64_t nt, const double* __restr
 #pragma unroll
     for (int k = 0; k < ND; k++) n[k] = xn[s * 3 + k];
     double u[K0][K1];
-    kmatrix<KER, NEWTON>(u, tx - xs[s * 3 + 0], ty - xs[s * 3 + 1], tz - xs[s * 3 + 2], n);
+    kmatrix<KER, NEWTON>(u, tx - xs[s * 3 + 0], ty - xs[s * 3 + 1], tz - xs[s * 3 + 2], n, mu);
 #pragma unroll
     for (int k0 = 0; k0 < K0; k0++)
 #pragma unroll
@@ -175,8 +175,8 @@ k_kernel_matrix(const double* __restrict__ xt, int64_t nt, const double* __restr
 namespace {
 
 using EvalF64Fn = void (*)(const double*, int64_t, const double*, const double*, const double*, int64_t, int64_t,
-                           double*, int, double);
-using KMatFn = void (*)(const double*, int64_t, const double*, const double*, int64_t, double*, double);
+                           double*, int, double, double);
+using KMatFn = void (*)(const double*, int64_t, const double*, const double*, int64_t, double*, double, double);
 
 constexpr int TPLS[3] = {1, 2, 4};
 int tpl_index(int tpl) { return tpl >= 4 ? 2 : (tpl >= 2 ? 1 : 0); }
@@ -222,7 +222,8 @@ void split_sources(int64_t target_blocks, int64_t ntiles, int num_cus, int64_t* 
 }  // namespace
 
 int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src, const double* n_src,
-                    const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape, hipStream_t stream) {
+                    const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape, double ctx,
+                    hipStream_t stream) {
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
   DeviceInfo dev;
   SCTL_TRY(device_info(&dev));
@@ -245,7 +246,7 @@ int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const 
   {
     ProfileScope prof(stream);
     hipLaunchKernelGGL(fn, grid, dim3(WG), dyn, stream, r_trg, nt, r_src, n_src, v_src, ns, tps * TILE,
-                       nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale);
+                       nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale, ctx);
   }
   SCTL_HIP_TRY(hipGetLastError());
   if (nsplit > 1) {
@@ -258,7 +259,7 @@ int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const 
 }
 
 int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
-                         const double* n_src, int64_t ns, double* M, hipStream_t stream) {
+                         const double* n_src, int64_t ns, double* M, double ctx, hipStream_t stream) {
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
   const int64_t tb = (nt + WG - 1) / WG;
   SCTL_CHECK_ARG(tb <= 0x7fffffffLL, "too many targets for one launch");
@@ -267,7 +268,7 @@ int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, c
   profile_count(ker, nt * ns);
   ProfileScope prof(stream);
   hipLaunchKernelGGL(kmat_fn(ker, newton), dim3((unsigned)tb, (unsigned)gy), dim3(WG), 0, stream, r_trg, nt, r_src,
-                     n_src, ns, M, scale);
+                     n_src, ns, M, scale, ctx);
   SCTL_HIP_TRY(hipGetLastError());
   return SCTL_AMD_SUCCESS;
 }

@@ -92,15 +92,16 @@ inline bool mfma_supported(int ker) { return ker == 0 || ker == 1; }
 int occupancy_lds(const void* kernel, int occ, size_t static_lds, size_t* dyn_bytes);
 
 // Dense Eval launchers (device pointers, stream-ordered).  v_trg is
-// accumulated: v_trg[t*K1+k] += scale * sum.
+// accumulated: v_trg[t*K1+k] += scale * sum.  ctx: the wavenumber of the
+// Helmholtz kernels (ignored by the others).
 int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
                     const double* n_src, const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape,
-                    hipStream_t stream);
+                    double ctx, hipStream_t stream);
 int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
                       const double* n_src, const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape,
-                      hipStream_t stream);
+                      double ctx, hipStream_t stream);
 int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
-                         const double* n_src, int64_t ns, double* M, hipStream_t stream);
+                         const double* n_src, int64_t ns, double* M, double ctx, hipStream_t stream);
 
 // Morton ordering of points (30-bit keys over the points' bounding box);
 // writes a permutation of [0, n).  Implemented in sort.hip.
@@ -139,6 +140,8 @@ int plan_sizes(sctl_amd_plan p, int64_t* ns, int64_t* nt, int* K0, int* K1, int6
 int plan_kernel(sctl_amd_plan p);
 
 int kernel_dims(int ker, int* K0, int* K1, int* ND);
+// True for the kernels that need a context value (the Helmholtz wavenumber).
+bool kernel_needs_ctx(int ker);
 double kernel_scale(int ker);
 // Largest targets-per-lane count the dense and batched kernels take for a
 // kernel (larger requests run with this many).

@@ -5,7 +5,10 @@
 // :125-178 Stokes3D_FxU, :180-218 Stokes3D_DxU, :220-263 Stokes3D_FxT,
 // :264-324 Stokes3D_FSxU, :325-365 Stokes3D_FxUP, :366-390 Laplace3D_Fxd2U,
 // :392-414 Laplace3D_DxdU, :416-450 BiotSavart3D_FxU, :451-489
-// BiotSavart3D_FxdU).  Eval uses a fused apply form (K * f accumulated without
+// BiotSavart3D_FxdU, :491-527 Helmholtz3D_FxU, :529-571 Helmholtz3D_DxU,
+// :573-620 Helmholtz3D_FxdU, :627-659 HelmholtzDiff3D_FxdU).  The Helmholtz
+// kernels take the wavenumber mu (the value the reference reads through
+// ctx_ptr) as an argument; their complex values are (re, im) pairs.  Eval uses a fused apply form (K * f accumulated without
 // forming K; the reference's own for the kernels that have one, e.g.
 // kernel_functions.hpp:165-176,204-217, and an algebraically equal one written
 // here for Laplace DxdU / Fxd2U and BiotSavart FxdU, whose reference Eval
@@ -37,7 +40,8 @@ namespace sctl_amd {
 enum KernelId {
   LAP_FXU = 0, LAP_DXU = 1, STK_FXU = 2, STK_DXU = 3,
   LAP_FXDU = 4, LAP_DXDU = 5, LAP_FXD2U = 6, STK_FXT = 7, STK_FSXU = 8, STK_FXUP = 9, BS_FXU = 10, BS_FXDU = 11,
-  NUM_KERNELS = 12
+  HLM_FXU = 12, HLM_DXU = 13, HLM_FXDU = 14, HLMD_FXDU = 15,
+  NUM_KERNELS = 16
 };
 
 // K0 densities per source, K1 outputs per target, ND normal words per source
@@ -56,7 +60,13 @@ template <> struct Ker<STK_FSXU> { static constexpr int K0 = 4, K1 = 3, ND = 0, 
 template <> struct Ker<STK_FXUP> { static constexpr int K0 = 3, K1 = 4, ND = 0, POW = 0; };
 template <> struct Ker<BS_FXU> { static constexpr int K0 = 3, K1 = 3, ND = 0, POW = 3; };
 template <> struct Ker<BS_FXDU> { static constexpr int K0 = 3, K1 = 9, ND = 0, POW = 0; };
+template <> struct Ker<HLM_FXU> { static constexpr int K0 = 2, K1 = 2, ND = 0, POW = 0; };
+template <> struct Ker<HLM_DXU> { static constexpr int K0 = 2, K1 = 2, ND = 3, POW = 0; };
+template <> struct Ker<HLM_FXDU> { static constexpr int K0 = 2, K1 = 6, ND = 0, POW = 0; };
+template <> struct Ker<HLMD_FXDU> { static constexpr int K0 = 2, K1 = 6, ND = 0, POW = 0; };
 constexpr int KMAX0 = 4, KMAX1 = 9;
+// Kernels that need the wavenumber (the reference's ctx_ptr).
+constexpr bool needs_ctx(int ker) { return ker >= HLM_FXU && ker <= HLMD_FXDU; }
 
 // Calls f(std::integral_constant<int, KER>()) for a kernel id (host side
 // dispatch to the kernel templates; ids are checked by the C-ABI first).
@@ -73,7 +83,11 @@ template <class F> inline auto with_kernel(int ker, F&& f) {
     case STK_FSXU: return f(std::integral_constant<int, STK_FSXU>());
     case STK_FXUP: return f(std::integral_constant<int, STK_FXUP>());
     case BS_FXU: return f(std::integral_constant<int, BS_FXU>());
-    default: return f(std::integral_constant<int, BS_FXDU>());
+    case BS_FXDU: return f(std::integral_constant<int, BS_FXDU>());
+    case HLM_FXU: return f(std::integral_constant<int, HLM_FXU>());
+    case HLM_DXU: return f(std::integral_constant<int, HLM_DXU>());
+    case HLM_FXDU: return f(std::integral_constant<int, HLM_FXDU>());
+    default: return f(std::integral_constant<int, HLMD_FXDU>());
   }
 }
 
@@ -91,6 +105,38 @@ __device__ __forceinline__ double hw_rsq(double x) { return __builtin_amdgcn_rsq
 __device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// sin and cos of one argument (the device math library's sincos).
+__device__ __forceinline__ void dev_sincos(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ void dev_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
+
+// Helmholtz: (G0, G1) = the complex factor of the kernel for z = 1/r
+// (already scaled), as the reference forms it (kernel_functions.hpp:505-510,
+// 544-549, 587-592, 641-654):
+//   FxU         e^{i mu r} / r
+//   DxU, FxdU   (-mu sin - cos / r, mu cos - sin / r) / r^2   (DxU: times r.n)
+//   Diff FxdU   the same with cos - 1 = -2 sin^2(mu r / 2) for cos
+template <int KER, class T>
+__device__ __forceinline__ void helmholtz_factor(T r2, T z, T mu, T* G0, T* G1) {
+  if constexpr (KER == HLMD_FXDU) {
+    T sh, ch;
+    dev_sincos(r2 * z * mu * (T)0.5, &sh, &ch);
+    const T sn = (T)2 * sh * ch, cm1 = (T)-2 * sh * sh;
+    const T z2 = z * z;
+    *G0 = fmad(-mu, sn, -cm1 * z) * z2;
+    *G1 = fmad(mu, (T)1 + cm1, -sn * z) * z2;
+  } else {
+    T sn, cs;
+    dev_sincos(r2 * z * mu, &sn, &cs);
+    if constexpr (KER == HLM_FXU) {
+      *G0 = cs * z;
+      *G1 = sn * z;
+    } else {
+      const T z2 = z * z;
+      *G0 = fmad(-mu, sn, -cs * z) * z2;
+      *G1 = fmad(mu, cs, -sn * z) * z2;
+    }
+  }
+}
 
 // Deferred-scaling Newton constants (intrin-wrapper.hpp:540-559).
 __host__ __device__ constexpr double newton_c1(int j) { return j == 0 ? -3.0 : (j == 1 ? -12.0 : -768.0); }
@@ -119,9 +165,10 @@ inline double fold_factor(int ker, int newton) {
 }
 
 // acc[K1] += K(r, n) * f  (fused apply, unscaled).  r = x_t - x_s.
-// n: source normal (ignored when ND == 0); f: K0 densities.
+// n: source normal (ignored when ND == 0); f: K0 densities; mu: the
+// wavenumber (Helmholtz kernels only).
 template <int KER, int NEWTON, bool MASK, class T>
-__device__ __forceinline__ void interact(T* acc, T rx, T ry, T rz, const T* n, const T* f) {
+__device__ __forceinline__ void interact(T* acc, T rx, T ry, T rz, const T* n, const T* f, T mu = (T)0) {
   const T r2 = fmad(rz, rz, fmad(ry, ry, rx * rx));
   T z = rsq_deferred<NEWTON>(r2);
   if constexpr (MASK) z = (r2 > (T)0) ? z : (T)0;
@@ -148,6 +195,7 @@ __device__ __forceinline__ void interact(T* acc, T rx, T ry, T rz, const T* n, c
     acc[2] = fmad(rz, q, acc[2]);
   } else {
     static_assert(KER >= LAP_FXDU && KER < NUM_KERNELS, "unknown kernel");
+    (void)mu;
     const T r[3] = {rx, ry, rz};
     if constexpr (Ker<KER>::POW == 0 && NEWTON > 0) z = z * (T)newton_c2(NEWTON);
     if constexpr (KER == LAP_FXDU) {  // f r / r^3
@@ -194,7 +242,7 @@ __device__ __forceinline__ void interact(T* acc, T rx, T ry, T rz, const T* n, c
       acc[0] = fmad(fmad(f[1], rz, -ry * f[2]), z3, acc[0]);
       acc[1] = fmad(fmad(f[2], rx, -rz * f[0]), z3, acc[1]);
       acc[2] = fmad(fmad(f[0], ry, -rx * f[1]), z3, acc[2]);
-    } else {  // BS_FXDU, output i*3+j: 3 (f x r)_i r_j / r^5 - (f x e_j)_i / r^3
+    } else if constexpr (KER == BS_FXDU) {  // output i*3+j: 3 (f x r)_i r_j / r^5 - (f x e_j)_i / r^3
       const T z2 = z * z;
       const T z3 = z2 * z;
       const T t = (T)3 * z3 * z2;
@@ -211,6 +259,25 @@ __device__ __forceinline__ void interact(T* acc, T rx, T ry, T rz, const T* n, c
       acc[5] = fmad(z3, f[0], acc[5]);   // -(f x e_2)_1 =  f0
       acc[6] = fmad(z3, f[1], acc[6]);   // -(f x e_0)_2 =  f1
       acc[7] = fmad(-z3, f[0], acc[7]);  // -(f x e_1)_2 = -f0
+    } else {  // Helmholtz: complex factor times the complex density (f0 + i f1)
+      T G0, G1;
+      helmholtz_factor<KER>(r2, z, mu, &G0, &G1);
+      if constexpr (KER == HLM_DXU) {
+        const T rn = fmad(rz, n[2], fmad(ry, n[1], rx * n[0]));
+        G0 = G0 * rn;
+        G1 = G1 * rn;
+      }
+      const T fG0 = fmad(f[0], G0, -f[1] * G1), fG1 = fmad(f[0], G1, f[1] * G0);
+      if constexpr (KER == HLM_FXU || KER == HLM_DXU) {
+        acc[0] += fG0;
+        acc[1] += fG1;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          acc[i * 2 + 0] = fmad(fG0, r[i], acc[i * 2 + 0]);
+          acc[i * 2 + 1] = fmad(fG1, r[i], acc[i * 2 + 1]);
+        }
+      }
     }
   }
 }
@@ -220,7 +287,7 @@ __device__ __forceinline__ void interact(T* acc, T rx, T ry, T rz, const T* n, c
 // :377-389, :403-413, :427-434, :462-486.  Used by KernelMatrix.
 template <int KER, int NEWTON>
 __device__ __forceinline__ void kmatrix(double (&u)[Ker<KER>::K0][Ker<KER>::K1], double rx, double ry, double rz,
-                                        const double* n) {
+                                        const double* n, double mu = 0.0) {
   const double r2 = fmad(rz, rz, fmad(ry, ry, rx * rx));
   double z = rsq_deferred<NEWTON>(r2) * newton_c2(NEWTON);
   z = (r2 > 0.0) ? z : 0.0;
@@ -283,6 +350,28 @@ __device__ __forceinline__ void kmatrix(double (&u)[Ker<KER>::K0][Ker<KER>::K1],
     u[2][0] = -r[1] * z3;
     u[2][1] = r[0] * z3;
     u[2][2] = 0.0;
+  } else if constexpr (KER >= HLM_FXU) {
+    double G0, G1;
+    helmholtz_factor<KER>(r2, z, mu, &G0, &G1);
+    if constexpr (KER == HLM_DXU) {
+      const double rn = fmad(rz, n[2], fmad(ry, n[1], rx * n[0]));
+      G0 *= rn;
+      G1 *= rn;
+    }
+    if constexpr (KER == HLM_FXU || KER == HLM_DXU) {
+      u[0][0] = G0;
+      u[0][1] = G1;
+      u[1][0] = -G1;
+      u[1][1] = G0;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        u[0][i * 2 + 0] = G0 * r[i];
+        u[0][i * 2 + 1] = G1 * r[i];
+        u[1][i * 2 + 0] = -G1 * r[i];
+        u[1][i * 2 + 1] = G0 * r[i];
+      }
+    }
   } else {  // BS_FXDU: source component k, column i*3+j = d/dr_j of (e_k x r)_i / r^3
     const double t5 = 3.0 * z5;
 #pragma unroll

@@ -90,7 +90,7 @@ __device__ __forceinline__ void fill_record(float* r, const double* d, const dou
 
 // Far-field interaction.  th: t' ; a: -2 t' ; T: |t'|^2 ; r: the record.
 template <int KER, int NEWTON>
-__device__ __forceinline__ void mix_far(float* p, const float* th, const float* a, float T, const float* r) {
+__device__ __forceinline__ void mix_far(float* p, const float* th, const float* a, float T, const float* r, float mu) {
   if constexpr (KER == LAP_FXU) {
     const float r2 = fmad(a[2], r[2], fmad(a[1], r[1], fmad(a[0], r[0], T + r[3])));
     p[0] = fmad(rsq_deferred<NEWTON>(r2), r[4], p[0]);
@@ -101,13 +101,13 @@ __device__ __forceinline__ void mix_far(float* p, const float* th, const float* 
     p[0] = fmad(rnq, z * z * z, p[0]);
   } else {
     using L = MixLayout<KER>;
-    interact<KER, NEWTON, false>(p, th[0] - r[0], th[1] - r[1], th[2] - r[2], r + L::NOFF, r + L::FOFF);
+    interact<KER, NEWTON, false>(p, th[0] - r[0], th[1] - r[1], th[2] - r[2], r + L::NOFF, r + L::FOFF, mu);
   }
 }
 
 // Near-field interaction: two-part differencing, r^2 == 0 masked.
 template <int KER, int NEWTON>
-__device__ __forceinline__ void mix_near(float* p, const float* th, const float* tl, const float* r) {
+__device__ __forceinline__ void mix_near(float* p, const float* th, const float* tl, const float* r, float mu) {
   constexpr int LO = MixLayout<KER>::LO;
   const float dx = (th[0] - r[0]) + (tl[0] - r[LO + 0]);
   const float dy = (th[1] - r[1]) + (tl[1] - r[LO + 1]);
@@ -120,7 +120,7 @@ __device__ __forceinline__ void mix_near(float* p, const float* th, const float*
     interact<LAP_DXU, NEWTON, true>(p, dx, dy, dz, nq, one);
   } else {
     using L = MixLayout<KER>;
-    interact<KER, NEWTON, true>(p, dx, dy, dz, r + L::NOFF, r + L::FOFF);
+    interact<KER, NEWTON, true>(p, dx, dy, dz, r + L::NOFF, r + L::FOFF, mu);
   }
 }
 
@@ -192,10 +192,11 @@ template <int KER, int TPL, int NEWTON>
 __global__ void __launch_bounds__(WG)
 k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restrict__ tperm,
              const float4* __restrict__ packed, const double4* __restrict__ meta, int64_t ntiles,
-             int64_t tiles_per_split, double* __restrict__ out, int direct, double scale) {
+             int64_t tiles_per_split, double* __restrict__ out, int direct, double scale, double ctx) {
   using K = Ker<KER>;
   constexpr int K1 = K::K1, Q = MixLayout<KER>::Q, R = 4 * Q;
   __shared__ float4 lds[TILE * Q];
+  const float mu = (float)ctx;
 
   const int tid = threadIdx.x;
   const int64_t tile_begin = (int64_t)blockIdx.y * tiles_per_split;
@@ -245,7 +246,7 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
       for (int i = 0; i < TILE; i++) {
         const float* rec = recs + i * R;
 #pragma unroll
-        for (int j = 0; j < TPL; j++) mix_far<KER, NEWTON>(p[j], th[j], a[j], T[j], rec);
+        for (int j = 0; j < TPL; j++) mix_far<KER, NEWTON>(p[j], th[j], a[j], T[j], rec, mu);
       }
       bool bad = false;
 #pragma unroll
@@ -268,7 +269,7 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
       for (int i = 0; i < TILE; i++) {
         const float* rec = recs + i * R;
 #pragma unroll
-        for (int j = 0; j < TPL; j++) mix_near<KER, NEWTON>(p[j], th[j], tl[j], rec);
+        for (int j = 0; j < TPL; j++) mix_near<KER, NEWTON>(p[j], th[j], tl[j], rec, mu);
       }
     }
 #pragma unroll
@@ -319,7 +320,7 @@ template <int KER, int TPL, int NEWTON>
 __global__ void __launch_bounds__(WG)
 k_eval_mixed_mfma(const double* __restrict__ xt, int64_t nt, const uint32_t* __restrict__ tperm,
                   const float4* __restrict__ packed, const double4* __restrict__ meta, int64_t ntiles,
-                  int64_t tiles_per_split, double* __restrict__ out, int direct, double scale) {
+                  int64_t tiles_per_split, double* __restrict__ out, int direct, double scale, double ctx) {
   static_assert(KER == LAP_FXU || KER == LAP_DXU, "matrix-core far path: Laplace kernels only");
   constexpr int Q = MixLayout<KER>::Q, R = 4 * Q, NR = 4 * TPL;
   static_assert(sizeof(float4) * TILE * Q >= sizeof(double) * 16 * NR * (WG / 64), "LDS too small for the fold");
@@ -439,7 +440,7 @@ k_eval_mixed_mfma(const double* __restrict__ xt, int64_t nt, const uint32_t* __r
       for (int i = 0; i < TILE; i++) {
         const float* rec = recs + i * R;
 #pragma unroll
-        for (int j = 0; j < TPL; j++) mix_near<KER, NEWTON>(pn[j], th[j], tl[j], rec);
+        for (int j = 0; j < TPL; j++) mix_near<KER, NEWTON>(pn[j], th[j], tl[j], rec, 0.0f);
       }
 #pragma unroll
       for (int j = 0; j < TPL; j++) {  // Kahan: acc += pn
@@ -496,7 +497,7 @@ k_reduce_splits_perm(const double* __restrict__ part, int64_t nt, int K1, int ns
 }
 
 using EvalMixFn = void (*)(const double*, int64_t, const uint32_t*, const float4*, const double4*, int64_t, int64_t,
-                           double*, int, double);
+                           double*, int, double, double);
 using PackFn = void (*)(const double*, const double*, const double*, int64_t, const uint32_t*, float4*, double4*);
 
 constexpr int TPLS[3] = {1, 2, 4};
@@ -538,7 +539,7 @@ int max_tpl_index(int ker) {
 
 int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
                       const double* n_src, const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape,
-                      hipStream_t stream) {
+                      double ctx, hipStream_t stream) {
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
   DeviceInfo dev;
   SCTL_TRY(device_info(&dev));
@@ -583,7 +584,7 @@ int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, cons
     ProfileScope prof(stream);
     hipLaunchKernelGGL(fn, dim3((unsigned)tb, (unsigned)nsplit), dim3(WG), dyn, stream, r_trg, nt,
                        tperm.as<uint32_t>(), packed.as<float4>(), meta.as<double4>(), ntiles, tps,
-                       nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale);
+                       nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale, ctx);
   }
   SCTL_HIP_TRY(hipGetLastError());
   if (nsplit > 1) {

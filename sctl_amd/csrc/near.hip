@@ -75,7 +75,7 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
               const int64_t* __restrict__ src_dsp, const int32_t* __restrict__ src_cnt,
               const int64_t* __restrict__ mat_off, const double* __restrict__ xt, const double* __restrict__ n_trg,
               const double* __restrict__ xs, const double* __restrict__ xn, const double* __restrict__ wts,
-              double* __restrict__ K, double scale) {
+              double* __restrict__ K, double scale, double mu) {
   using Kt = Ker<KER>;
   constexpr int K0 = Kt::K0, K1 = Kt::K1, ND = Kt::ND;
   constexpr int DOF = NDOT ? K1 / 3 : K1;
@@ -108,7 +108,7 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
       double* Kc = Ks + (int64_t)s * K0;
       for (int t = 0; t < nts; t++) {
         double u[K0][K1];
-        kmatrix<KER, NEAR_NEWTON>(u, readlane_d(mx, t) - sx, readlane_d(my, t) - sy, readlane_d(mz, t) - sz, n);
+        kmatrix<KER, NEAR_NEWTON>(u, readlane_d(mx, t) - sx, readlane_d(my, t) - sy, readlane_d(mz, t) - sz, n, mu);
         if (!active) continue;
         if constexpr (NDOT) {
           const double n0 = readlane_d(mn[0], t), n1 = readlane_d(mn[1], t), n2 = readlane_d(mn[2], t);
@@ -222,7 +222,7 @@ k_near_accumulate(int64_t ntrg, const int64_t* __restrict__ cnt, const int64_t* 
 
 using NearMatFn = void (*)(const int32_t*, const int64_t*, int64_t, const int64_t*, const int32_t*, const int64_t*,
                            const int32_t*, const int64_t*, const double*, const double*, const double*,
-                           const double*, const double*, double*, double);
+                           const double*, const double*, double*, double, double);
 
 template <int KER, bool NDOT> NearMatFn near_mat_fn(bool sub) {
   return sub ? k_near_matrix<KER, NDOT, true> : k_near_matrix<KER, NDOT, false>;
@@ -286,7 +286,7 @@ int near_matrix_launch(sctl_amd_plan_s* p, bool sub, const double* wts, const do
   hipLaunchKernelGGL(near_mat_fn(p->kernel, p->near_ndot, sub), dim3(tile_grid(p->mat_tiles.ntiles)), dim3(NWG), 0,
                      s, p->mat_tiles.d_tile_blk, p->mat_tiles.d_tile_dsp, p->mat_tiles.ntiles, p->d_trg_dsp,
                      p->d_trg_cnt, p->d_src_dsp, p->d_src_cnt, p->d_near_off, p->d_xt, n_trg, p->d_xs, p->d_xn, wts,
-                     K, kernel_scale(p->kernel));
+                     K, kernel_scale(p->kernel), p->ctx);
   SCTL_HIP_TRY(hipGetLastError());
   return SCTL_AMD_SUCCESS;
 }
@@ -417,6 +417,8 @@ int sctl_amd_plan_near_matrix(sctl_amd_plan p, int mode, const double* wts, cons
   SCTL_CHECK_ARG(p, "plan is NULL");
   SCTL_CHECK_ARG(p->has_geometry, "plan has no geometry (call sctl_amd_plan_set_geometry first)");
   SCTL_CHECK_ARG(p->has_near, "plan has no near-field layout (call sctl_amd_plan_near_layout first)");
+  SCTL_CHECK_ARG(!needs_ctx(p->kernel) || p->has_ctx,
+                 "the kernel needs a wavenumber (call sctl_amd_plan_set_ctx first)");
   SCTL_CHECK_ARG(mode == SCTL_AMD_NEAR_SUBTRACT || mode == SCTL_AMD_NEAR_STORE, "unknown near-matrix mode");
   SCTL_CHECK_ARG((flags & ~SCTL_AMD_HOST_PTRS) == 0, "unknown flag bits");
   SCTL_CHECK_ARG(p->near_len == 0 || K, "K is NULL");

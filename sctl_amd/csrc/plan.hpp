@@ -48,6 +48,8 @@ struct sctl_amd_plan_s {
   int grid = 0;               // persistent grid (workgroups)
   size_t dyn_lds = 0;         // dynamic LDS that enforces shape.occ
   bool has_geometry = false;
+  double ctx = 0.0;     // wavenumber of the Helmholtz kernels (sctl_amd_plan_set_ctx)
+  bool has_ctx = false;
   sctl_amd::Item* d_items = nullptr;
   int64_t *d_trg_dsp = nullptr, *d_src_dsp = nullptr;
   int32_t *d_src_cnt = nullptr, *d_trg_cnt = nullptr;

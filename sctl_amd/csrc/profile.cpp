@@ -10,9 +10,9 @@
 namespace sctl_amd {
 
 namespace {
-// FLOPS() of kernel_functions.hpp:37,66,86,130,185,225,269,330,371,397,421,456
-// (kernel ids 0..11).
-constexpr int64_t REF_FLOPS[] = {6, 14, 23, 26, 11, 22, 25, 39, 26, 26, 15, 51};
+// FLOPS() of kernel_functions.hpp:37,66,86,130,185,225,269,330,371,397,421,456,
+// 496,534,578,632 (kernel ids 0..15).
+constexpr int64_t REF_FLOPS[] = {6, 14, 23, 26, 11, 22, 25, 39, 26, 26, 15, 51, 20, 29, 35, 41};
 constexpr int N_REF_FLOPS = (int)(sizeof(REF_FLOPS) / sizeof(REF_FLOPS[0]));
 std::atomic<int64_t> g_inter{0}, g_flops{0};
 std::atomic<bool> g_timing{false};

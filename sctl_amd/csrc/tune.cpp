@@ -274,6 +274,8 @@ int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_tr
   const int newton = precision == SCTL_AMD_FP64 ? newton_steps_f64(digits) : ((flags & SCTL_AMD_MIXED_NEWTON) ? 1 : 0);
   double best = 1e300;
   DenseShape bs;
+  // The Helmholtz kernels are timed at wavenumber 1 (the shape does not depend on it).
+  const double mu = kernel_needs_ctx(kernel) ? 1.0 : 0.0;
   const bool with_mfma = precision == SCTL_AMD_MIXED && mfma_supported(kernel);
   for (int mfma = 0; mfma <= (with_mfma ? 1 : 0); mfma++) {
     for (int tpl : {1, 2, 4}) {
@@ -287,8 +289,8 @@ int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_tr
         double ms = 0;
         auto run = [&]() {
           return precision == SCTL_AMD_FP64
-                     ? launch_eval_f64(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, s)
-                     : launch_eval_mixed(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, s);
+                     ? launch_eval_f64(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, mu, s)
+                     : launch_eval_mixed(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, mu, s);
         };
         SCTL_TRY(time_ms(run, s, 2, &ms));
         if (ms < best) {

@@ -33,7 +33,7 @@ template <class T> static bool approx_eq(T a, T b, T tol) {
   return d <= tol || d <= tol * m;
 }
 
-template <class K> static void eval_vs_matrix(Long K0, Long K1, Long ND) {
+template <class K> static void eval_vs_matrix(Long K0, Long K1, Long ND, double mu = 0.0) {
   const Long Ns = 7, Nt = 5;
   Vector<double> Xs(Ns * 3), Xn(Ns * ND), Vs(Ns * K0), Xt(Nt * 3);
   for (Long i = 0; i < Xs.Dim(); i++) Xs[i] = 0.1 * (i + 1) - 0.35;
@@ -45,6 +45,7 @@ template <class K> static void eval_vs_matrix(Long K0, Long K1, Long ND) {
     Xn[i * 3 + 2] = 0;
   }
   K ker;
+  if (mu != 0.0) ker.SetCtxPtr(&mu);
   Vector<double> Vt;
   ker.template Eval<double, false>(Vt, Xt, Xs, Xn, Vs);
   Matrix<double> M;
@@ -141,6 +142,19 @@ int main() {
   eval_vs_matrix<sctl_amd::Stokes3D_FxUP>(3, 4, 0);
   eval_vs_matrix<sctl_amd::BiotSavart3D_FxU>(3, 3, 0);
   eval_vs_matrix<sctl_amd::BiotSavart3D_FxdU>(3, 9, 0);
+  eval_vs_matrix<sctl_amd::Helmholtz3D_FxU>(2, 2, 0, 2.5);
+  eval_vs_matrix<sctl_amd::Helmholtz3D_DxU>(2, 2, 3, 2.5);
+  eval_vs_matrix<sctl_amd::Helmholtz3D_FxdU>(2, 6, 0, 2.5);
+  eval_vs_matrix<sctl_amd::HelmholtzDiff3D_FxdU>(2, 6, 0, 2.5);
+  {  // Helmholtz3D_FxU of one unit source: e^{i mu r} / (4 pi r); through the static Eval and `self`
+    sctl_amd::Helmholtz3D_FxU K;
+    double mu = 1.3;
+    K.SetCtxPtr(&mu);
+    Vector<double> Xs({0, 0, 0}), Xt({2, 0, 0}), Xn, v_s({1.0, 0.0}), v_t;
+    sctl_amd::Helmholtz3D_FxU::Eval<double, false, Vector<double>>(v_t, Xt, Xs, Xn, v_s, -1, (const char*)&K);
+    CHECK(v_t.Dim() == 2 && approx_eq(v_t[0], std::cos(2 * mu) / (8 * pi), 1e-13) &&
+          approx_eq(v_t[1], std::sin(2 * mu) / (8 * pi), 1e-13));
+  }
 
   std::printf("\n%s (%d check(s) failed)\n", failures == 0 ? "PASS" : "FAIL", failures);
   return failures == 0 ? 0 : 1;

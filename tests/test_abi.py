@@ -45,7 +45,9 @@ def test_kernel_info_matches_reference_metadata():
             4: (3, 1, 3, 0, 11, "Laplace3D-FxdU"), 5: (3, 1, 3, 3, 22, "Laplace3D-DxdU"),
             6: (3, 1, 9, 0, 25, "Laplace3D-Fxd2U"), 7: (3, 3, 9, 0, 39, "Stokes3D-FxT"),
             8: (3, 4, 3, 0, 26, "Stokes3D-FSxU"), 9: (3, 3, 4, 0, 26, "Stokes3D-FxUP"),
-            10: (3, 3, 3, 0, 15, "BiotSavart3D-FxU"), 11: (3, 3, 9, 0, 51, "BiotSavart3D-FxdU")}
+            10: (3, 3, 3, 0, 15, "BiotSavart3D-FxU"), 11: (3, 3, 9, 0, 51, "BiotSavart3D-FxdU"),
+            12: (3, 2, 2, 0, 20, "Helmholtz3D-FxU"), 13: (3, 2, 2, 3, 29, "Helmholtz3D-DxU"),
+            14: (3, 2, 6, 0, 35, "Helmholtz3D-FxdU"), 15: (3, 2, 6, 0, 41, "HelmholtzDiff3D-FxdU")}
     import oracle
     assert len(want) == S.NUM_KERNELS
     for k, w in want.items():
@@ -63,7 +65,7 @@ def test_error_codes_before_any_device_work():
     x = np.zeros(3)
     v = np.zeros(3)
     p = x.ctypes.data
-    assert _eval_rc(12, 0, -1, p, 1, p, None, p, 1, p, 1, None) == 2  # unknown kernel
+    assert _eval_rc(16, 0, -1, p, 1, p, None, p, 1, p, 1, None) == 2  # unknown kernel
     assert b"unknown kernel" in L.sctl_amd_last_error()
     assert _eval_rc(0, 7, -1, p, 1, p, None, p, 1, p, 1, None) == 1  # unknown precision
     assert _eval_rc(0, 0, -2, p, 1, p, None, p, 1, p, 1, None) == 1  # digits < -1
@@ -72,10 +74,14 @@ def test_error_codes_before_any_device_work():
     assert b"n_src" in L.sctl_amd_last_error()
     assert _eval_rc(0, 0, -1, p, -1, p, None, p, 1, p, 1, None) == 1  # negative size
     assert _eval_rc(0, 0, -1, p, 1, p, None, p, 1, p, 0x80, None) == 1  # unknown flag
-    assert L.sctl_amd_kernel_matrix(12, -1, p, 1, p, None, 1, p, 1, None) == 2
+    assert L.sctl_amd_kernel_matrix(16, -1, p, 1, p, None, 1, p, 1, None) == 2
+    assert _eval_rc(12, 0, -1, p, 1, p, None, p, 1, p, 1, None) == 1  # Helmholtz without its wavenumber
+    assert b"wavenumber" in L.sctl_amd_last_error()
+    inf = ctypes.pointer(ctypes.c_double(float("inf")))
+    assert L.sctl_amd_eval_ctx(12, inf, 0, -1, p, 1, p, None, p, 1, p, 1, None) == 1  # non-finite wavenumber
     assert L.sctl_amd_memcpy(p, p, 8, 3, None) == 1
     with pytest.raises(S.SctlAmdError):
-        S.kernel_info(12)
+        S.kernel_info(16)
     with pytest.raises(S.SctlAmdError):
         S.kernel_info(-1)
 
@@ -90,7 +96,7 @@ def test_plan_rejects_bad_blocks():
     assert b"overlap" in L.sctl_amd_last_error()
     td, tc = a(0, 10), a(10, 20)  # second block runs past nt_total
     assert L.sctl_amd_plan_create(ctypes.byref(h), 0, 0, -1, 2, P(td), P(tc), P(sd), P(sc), 20, 4) == 1
-    assert L.sctl_amd_plan_create(ctypes.byref(h), 12, 0, -1, 2, P(td), P(tc), P(sd), P(sc), 40, 4) == 2
+    assert L.sctl_amd_plan_create(ctypes.byref(h), 16, 0, -1, 2, P(td), P(tc), P(sd), P(sc), 40, 4) == 2
     assert L.sctl_amd_plan_apply(None, None, None, 0, None) == 1
 
 

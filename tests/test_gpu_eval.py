@@ -188,6 +188,89 @@ def test_kernel_matrix_matches_reference(gpu, k):
     assert nerr(M.T @ v, gpu.eval(k, xt, xs, n, v)) < FP64_TOL
 
 
+HELMHOLTZ = [12, 13, 14, 15]
+H_IDS = ["HelmholtzFxU", "HelmholtzDxU", "HelmholtzFxdU", "HelmholtzDiffFxdU"]
+
+
+@pytest.mark.parametrize("k", HELMHOLTZ, ids=H_IDS)
+def test_helmholtz_matches_reference_golden(gpu, k):
+    """Eval and KernelMatrix against the reference's own outputs at four
+    wavenumbers (0, 0.7, 3, 25), all branches."""
+    G = pins.golden_helmholtz()
+    _, K0, K1, ND, _ = oracle.dims(k)
+    for j, mu in enumerate(G["mu"]):
+        p = "k%d_m%d_" % (k, j)
+        Xt, Xs, Xn, Vs = pins.small_inputs(K0, ND)
+        assert nerr(gpu.eval(k, Xt, Xs, Xn, Vs, ctx=mu), G[p + "small_eval_scalar"]) < FP64_TOL
+        assert nerr(gpu.kernel_matrix(k, Xt, Xs, Xn, ctx=mu), G[p + "small_kmat_scalar"]) < FP64_TOL
+        xt, xs, n, v = (G[p + "rand_" + a] for a in ("xt", "xs", "n", "v"))
+        out = gpu.eval(k, xt, xs, n, v, ctx=mu)
+        assert np.isfinite(out).all()
+        assert nerr(out, G[p + "rand_eval_scalar"]) < FP64_TOL, (mu, nerr(out, G[p + "rand_eval_scalar"]))
+        vt = gpu.eval(k, G[p + "veclen_xt"], xs, n, v, v_trg=G[p + "veclen_vt0"].copy(), ctx=mu)
+        assert nerr(vt, G[p + "veclen_eval_scalar"]) < FP64_TOL
+        xt11, xs13, n13 = (G[p + "kmat_" + a] for a in ("xt", "xs", "n"))
+        for tag, a, b, c in (("kmat", xt11, xs13, n13), ("kmat1t", xt11[:3], xs13, n13),
+                             ("kmat1s", xt11, xs13[:3], n13[:3])):
+            ref = G[p + tag + "_scalar"]
+            M = gpu.kernel_matrix(k, a, b, c, ctx=mu)
+            assert M.shape == ref.shape and nerr(M, ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("k", HELMHOLTZ, ids=H_IDS)
+@pytest.mark.parametrize("mu", [0.5, 4.0, 40.0])
+@pytest.mark.parametrize("nt,ns,coincide", [(63, 65, 10), (1000, 3000, 40), (20000, 700, 0)])
+def test_helmholtz_fp64_matches_oracle(gpu, k, mu, nt, ns, coincide):
+    rng = np.random.default_rng(nt + 3 * ns)
+    xt, xs, n, v = rand_case(rng, k, nt, ns, coincide)
+    ref = oracle.eval(k, xt, xs, n, v, ctx=mu)
+    out = gpu.eval(k, xt, xs, n, v, ctx=mu)
+    assert np.isfinite(out).all()
+    assert nerr(out, ref) < FP64_TOL
+    M = gpu.kernel_matrix(k, xt[:3 * 50], xs[:3 * 40], n[:3 * 40], ctx=mu)
+    assert nerr(M, oracle.kernel_matrix(k, xt[:3 * 50], xs[:3 * 40], n[:3 * 40], ctx=mu)) < FP64_TOL
+
+
+@pytest.mark.parametrize("k", HELMHOLTZ, ids=H_IDS)
+@pytest.mark.parametrize("mu,tol", [(1.0, MIXED_TOL), (6.0, 1e-5)])
+@pytest.mark.parametrize("nt,ns,coincide", [(300, 257, 30), (20000, 20000, 0)])
+def test_helmholtz_mixed_matches_oracle(gpu, k, mu, tol, nt, ns, coincide):
+    """Mixed precision: the FP32 phase mu*r carries about 2^-23 relative
+    error, so the bound grows with mu times the domain size (unit cube):
+    1e-6 at mu = 1, 1e-5 at mu = 6."""
+    rng = np.random.default_rng(nt + ns + 1)
+    xt, xs, n, v = rand_case(rng, k, nt, ns, coincide)
+    ref = oracle.eval(k, xt, xs, n, v, ctx=mu)
+    out = gpu.eval(k, xt, xs, n, v, precision=gpu.MIXED, ctx=mu)
+    assert np.isfinite(out).all()
+    e_inf, e_rms = W.rel_errors(out, ref)
+    assert e_inf < tol and e_rms < tol, (e_inf, e_rms)
+
+
+@pytest.mark.parametrize("k", HELMHOLTZ, ids=H_IDS)
+@pytest.mark.parametrize("digits", [3, 11])
+def test_helmholtz_digits(gpu, k, digits):
+    rng = np.random.default_rng(digits + k)
+    xt, xs, n, v = rand_case(rng, k, 500, 700)
+    ref = oracle.eval(k, xt, xs, n, v, ctx=1.0)
+    out = gpu.eval(k, xt, xs, n, v, digits=digits, ctx=1.0)
+    assert nerr(out, ref) < 10.0 ** (-digits)
+
+
+def test_helmholtz_needs_its_wavenumber(gpu):
+    rng = np.random.default_rng(0)
+    xt, xs, n, v = rand_case(rng, 12, 10, 10)
+    with pytest.raises(gpu.SctlAmdError):
+        gpu.eval(12, xt, xs, n, v)
+    with pytest.raises(gpu.SctlAmdError):
+        gpu.kernel_matrix(12, xt, xs, n)
+    K = gpu.GenericKernel(gpu.HELMHOLTZ3D_FXU)
+    K.SetCtxPtr(2.0)
+    assert nerr(K.Eval(None, xt, xs, None, v), oracle.eval(12, xt, xs, None, v, ctx=2.0)) < FP64_TOL
+    # the other kernels ignore a context
+    assert nerr(gpu.eval(0, xt, xs, None, v[:10], ctx=5.0), oracle.eval(0, xt, xs, None, v[:10])) < FP64_TOL
+
+
 @pytest.mark.parametrize("k", [0, 1], ids=["LaplaceFxU", "LaplaceDxU"])
 def test_full_size_sphere_shell(gpu, k):
     """Config 2 at full size (2^20 sources on the sphere, 2^20 targets in the

@@ -19,10 +19,16 @@ pytestmark = pytest.mark.gpu
 
 KERNEL_CASES = [(0, False), (1, False), (2, False), (3, False), (2, True), (3, True),
                 (4, False), (5, False), (5, True), (6, False), (6, True), (7, False), (7, True), (8, False),
-                (9, False), (10, False), (11, True)]
+                (9, False), (10, False), (11, True), (12, False), (13, False), (14, False), (15, True)]
 IDS = ["LaplaceFxU", "LaplaceDxU", "StokesFxU", "StokesDxU", "StokesFxU-ndot", "StokesDxU-ndot",
        "LaplaceFxdU", "LaplaceDxdU", "LaplaceDxdU-ndot", "LaplaceFxd2U", "LaplaceFxd2U-ndot", "StokesFxT",
-       "StokesFxT-ndot", "StokesFSxU", "StokesFxUP", "BiotSavartFxU", "BiotSavartFxdU-ndot"]
+       "StokesFxT-ndot", "StokesFSxU", "StokesFxUP", "BiotSavartFxU", "BiotSavartFxdU-ndot", "HelmholtzFxU",
+       "HelmholtzDxU", "HelmholtzFxdU", "HelmholtzDiffFxdU-ndot"]
+MU = 3.5  # wavenumber of the Helmholtz kernels
+
+
+def ctx_of(kernel):
+    return MU if kernel >= 12 else None
 
 
 def _case(kernel, ndot, seed=1, n_u=6, n_v=5, n_trg=200, matrix_free=None, nthreads=5):
@@ -37,13 +43,13 @@ def _case(kernel, ndot, seed=1, n_u=6, n_v=5, n_trg=200, matrix_free=None, nthre
 def _near_field(S, kernel, ndot, c, blocks, matrix_free=None):
     return S.NearField(kernel, c["near_elem_cnt"], c["near_elem_dsp"], c["elem_nds_cnt_far"], c["elem_nds_dsp_far"],
                        c["Xtrg_near"], c["X_far"], c["Xn_far"], c["wts_far"], c["Xn_trg_near"],
-                       matrix_free=matrix_free, trg_normal_dot=ndot, blocks=blocks)
+                       matrix_free=matrix_free, trg_normal_dot=ndot, blocks=blocks, ctx=ctx_of(kernel))
 
 
 def _oracle_subtract(orc, kernel, ndot, c, blocks, kc, kd, K, matrix_free=None):
     return orc.near_subtract(kernel, ndot, blocks, c["near_elem_dsp"], c["elem_nds_cnt"], c["elem_nds_cnt_far"],
                              c["elem_nds_dsp_far"], kc, kd, c["Xtrg_near"], c["Xn_trg_near"], c["X_far"],
-                             c["Xn_far"], c["wts_far"], K, matrix_free=matrix_free)
+                             c["Xn_far"], c["wts_far"], K, matrix_free=matrix_free, ctx=ctx_of(kernel))
 
 
 @pytest.mark.parametrize("kernel,ndot", KERNEL_CASES, ids=IDS)

@@ -11,6 +11,13 @@ from sctl_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
+MU = 2.5  # wavenumber of the Helmholtz kernels
+
+
+def ctx_of(k):
+    return MU if k >= 12 else None
+
+
 def per_block_oracle(k, B, v_src, v_trg0=None):
     _, K0, K1, ND, _ = oracle.dims(k)
     out = np.zeros(B["r_trg"].size // 3 * K1) if v_trg0 is None else v_trg0.copy()
@@ -21,7 +28,7 @@ def per_block_oracle(k, B, v_src, v_trg0=None):
         out[t0 * K1:(t0 + tc) * K1] = oracle.eval(
             k, B["r_trg"][3 * t0:3 * (t0 + tc)], B["r_src"][3 * s0:3 * (s0 + sc)],
             B["n_src"][3 * s0:3 * (s0 + sc)] if ND else None, v_src[s0 * K0:(s0 + sc) * K0],
-            v_trg=out[t0 * K1:(t0 + tc) * K1])
+            v_trg=out[t0 * K1:(t0 + tc) * K1], ctx=ctx_of(k))
     return out
 
 
@@ -30,9 +37,10 @@ def small_torus(k, nu=8, nv=8, seed=0):
     return W.torus_blocks(nu, nv, seed=seed, k0=K0, ns_range=(64, 1024), nt_range=(1, 300))
 
 
-KERNELS = list(range(12))
+KERNELS = list(range(16))
 IDS = ["LaplaceFxU", "LaplaceDxU", "StokesFxU", "StokesDxU", "LaplaceFxdU", "LaplaceDxdU", "LaplaceFxd2U",
-       "StokesFxT", "StokesFSxU", "StokesFxUP", "BiotSavartFxU", "BiotSavartFxdU"]
+       "StokesFxT", "StokesFSxU", "StokesFxUP", "BiotSavartFxU", "BiotSavartFxdU", "HelmholtzFxU", "HelmholtzDxU",
+       "HelmholtzFxdU", "HelmholtzDiffFxdU"]
 
 
 @pytest.mark.parametrize("k", KERNELS, ids=IDS)
@@ -41,7 +49,7 @@ def test_plan_matches_per_block_oracle(gpu, k, prec, tol):
     B = small_torus(k)
     _, K0, K1, ND, _ = oracle.dims(k)
     nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
-    P = gpu.Plan(k, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=prec)
+    P = gpu.Plan(k, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=prec, ctx=ctx_of(k))
     assert P.interactions() == W.interactions(B)
     P.set_geometry(B["r_trg"], B["r_src"], B["n_src"] if ND else None)
     v0 = np.random.default_rng(2).standard_normal(nt * K1)
@@ -66,10 +74,10 @@ def test_plan_ragged_and_empty_blocks(gpu, prec, tol):
     src_dsp = np.array([0, 100, 200, 0, 900, 1000, 1500], dtype=np.int64)
     src_cnt = np.array([100, 1, 700, 10, 33, 1000, 500], dtype=np.int64)
     B = dict(r_trg=xt, r_src=xs, n_src=n, trg_dsp=trg_dsp, trg_cnt=trg_cnt, src_dsp=src_dsp, src_cnt=src_cnt)
-    for k in (1, 2, 5, 8, 11):
+    for k in (1, 2, 5, 8, 11, 13):
         _, K0, K1, ND, _ = oracle.dims(k)
         v = rng.uniform(-1, 1, ns * K0)
-        P = gpu.Plan(k, trg_dsp, trg_cnt, src_dsp, src_cnt, nt, ns, precision=prec)
+        P = gpu.Plan(k, trg_dsp, trg_cnt, src_dsp, src_cnt, nt, ns, precision=prec, ctx=ctx_of(k))
         P.set_geometry(xt, xs, n if ND else None)
         out = P.apply(v, np.zeros(nt * K1))
         ref = per_block_oracle(k, B, v)
@@ -101,3 +109,15 @@ def test_plan_device_pointers_and_torus_config(gpu):
         ref = oracle.eval(1, B["r_trg"][3 * t0:3 * (t0 + tc)], B["r_src"][3 * s0:3 * (s0 + sc)],
                           B["n_src"][3 * s0:3 * (s0 + sc)], B["v_src"][s0:s0 + sc])
         assert W.rel_errors(out[0][t0:t0 + tc], ref)[0] < 1e-13
+
+
+def test_plan_helmholtz_needs_its_wavenumber(gpu):
+    B = small_torus(12, nu=2, nv=2)
+    nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
+    P = gpu.Plan(12, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns)
+    P.set_geometry(B["r_trg"], B["r_src"])
+    with pytest.raises(gpu.SctlAmdError):
+        P.apply(B["v_src"], np.zeros(nt * 2))
+    P.set_ctx(MU)
+    out = P.apply(B["v_src"], np.zeros(nt * 2))
+    assert W.rel_errors(out, per_block_oracle(12, B, B["v_src"]))[0] < 1e-13
