@@ -109,31 +109,49 @@ __device__ __forceinline__ float fmad(float a, float b, float c) { return __buil
 __device__ __forceinline__ void dev_sincos(double x, double* s, double* c) { sincos(x, s, c); }
 __device__ __forceinline__ void dev_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
 
+// x cos x - sin x without cancellation for |x| < 1/2 (its Taylor series
+// -x^3/3 + x^5/30 - x^7/840 + x^9/45360, next term < 2^-28 relative there),
+// directly otherwise.  For FP32, where the direct form loses about
+// 3 / x^2 ulp as x -> 0.
+__device__ __forceinline__ float x_cos_minus_sin(float x, float sn, float cs) {
+  const float x2 = x * x;
+  const float series = x * x2 * fmad(x2, fmad(x2, fmad(x2, 1.0f / 45360, -1.0f / 840), 1.0f / 30), -1.0f / 3);
+  return fabsf(x) < 0.5f ? series : fmad(x, cs, -sn);
+}
+
 // Helmholtz: (G0, G1) = the complex factor of the kernel for z = 1/r
 // (already scaled), as the reference forms it (kernel_functions.hpp:505-510,
 // 544-549, 587-592, 641-654):
 //   FxU         e^{i mu r} / r
 //   DxU, FxdU   (-mu sin - cos / r, mu cos - sin / r) / r^2   (DxU: times r.n)
 //   Diff FxdU   the same with cos - 1 = -2 sin^2(mu r / 2) for cos
+// In FP32 (the mixed path) the imaginary part mu cos - sin / r, which
+// cancels to -mu^3 r^2 / 3 as mu r -> 0, is formed as z (x cos x - sin x)
+// with x = mu r (x_cos_minus_sin); FP64 keeps the reference's form.
 template <int KER, class T>
 __device__ __forceinline__ void helmholtz_factor(T r2, T z, T mu, T* G0, T* G1) {
+  constexpr bool F32 = std::is_same<T, float>::value;
   if constexpr (KER == HLMD_FXDU) {
+    const T xh = r2 * z * mu * (T)0.5;
     T sh, ch;
-    dev_sincos(r2 * z * mu * (T)0.5, &sh, &ch);
+    dev_sincos(xh, &sh, &ch);
     const T sn = (T)2 * sh * ch, cm1 = (T)-2 * sh * sh;
     const T z2 = z * z;
     *G0 = fmad(-mu, sn, -cm1 * z) * z2;
-    *G1 = fmad(mu, (T)1 + cm1, -sn * z) * z2;
+    if constexpr (F32) *G1 = x_cos_minus_sin((T)2 * xh, sn, (T)1 + cm1) * (z2 * z);
+    else *G1 = fmad(mu, (T)1 + cm1, -sn * z) * z2;
   } else {
+    const T x = r2 * z * mu;
     T sn, cs;
-    dev_sincos(r2 * z * mu, &sn, &cs);
+    dev_sincos(x, &sn, &cs);
     if constexpr (KER == HLM_FXU) {
       *G0 = cs * z;
       *G1 = sn * z;
     } else {
       const T z2 = z * z;
       *G0 = fmad(-mu, sn, -cs * z) * z2;
-      *G1 = fmad(mu, cs, -sn * z) * z2;
+      if constexpr (F32) *G1 = x_cos_minus_sin(x, sn, cs) * (z2 * z);
+      else *G1 = fmad(mu, cs, -sn * z) * z2;
     }
   }
 }
