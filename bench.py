@@ -128,7 +128,7 @@ def time_once(fn, stream, reps=1):
     return e0.elapsed_ms(e1) / reps
 
 
-def dense_pair(kernel, xt, xs, n, f, stream, reps, digits=-1):
+def dense_pair(kernel, xt, xs, n, f, stream, reps, digits=-1, ctx=None):
     """FP64 and mixed Eval of one dense configuration: rates and mixed-vs-FP64 errors."""
     _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
     nt, ns = xt.size // 3, xs.size // 3
@@ -138,11 +138,11 @@ def dense_pair(kernel, xt, xs, n, f, stream, reps, digits=-1):
     outs = {}
     for name, prec in (("fp64", S.FP64), ("mixed", S.MIXED)):
         o = S.DeviceArray.from_host(np.zeros(nt * K1))
-        S.eval(kernel, d[0], d[1], dn, d[3], o, precision=prec, digits=digits, stream=stream)  # warm-up
+        S.eval(kernel, d[0], d[1], dn, d[3], o, precision=prec, digits=digits, stream=stream, ctx=ctx)  # warm-up
         S.synchronize(stream)
         outs[name] = o.download()
         ms = time_once(lambda: S.eval(kernel, d[0], d[1], dn, d[3], o, precision=prec, digits=digits,
-                                      stream=stream), stream, reps)
+                                      stream=stream, ctx=ctx), stream, reps)
         res[name] = {"ms": round(ms, 3), "G_int_per_s": round(nt * ns / ms / 1e6, 2)}
     e_inf, e_rms = W.rel_errors(outs["mixed"], outs["fp64"])
     res["mixed_vs_fp64"] = {"max_rel": float("%.3g" % e_inf), "rms_rel": float("%.3g" % e_rms)}
@@ -374,17 +374,24 @@ def other_configs(stream, quick=False):
     return out
 
 
+HELMHOLTZ_MU = 3.0  # wavenumber of the Helmholtz lines (about half a wavelength across the unit cube)
+
+
 def other_kernels(stream, reps):
-    """FP64 and mixed Eval of the eight other real-valued micro-kernels on a
-    2^17-target x 2^17-source uniform cube: rates, the reference's FLOPS()
-    count per interaction times the rate, and mixed-vs-FP64 errors."""
+    """FP64 and mixed Eval of the twelve other micro-kernels on a
+    2^17-target x 2^17-source uniform cube (Helmholtz at wavenumber
+    HELMHOLTZ_MU): rates, the reference's FLOPS() count per interaction times
+    the rate, and mixed-vs-FP64 errors."""
     N = 1 << 17
     out = {}
     for k in range(4, S.NUM_KERNELS):
         _, K0, K1, ND, flops, _, name = S.kernel_info(k)
         xt, xs, f = W.cube(N, N, seed=k, k0=K0)
         nrm = W.normal(k, 3, N * 3) if ND else None
-        r, _ = dense_pair(k, xt, xs, nrm, f, stream, reps=reps)
+        ctx = HELMHOLTZ_MU if k in S.CTX_KERNELS else None
+        r, _ = dense_pair(k, xt, xs, nrm, f, stream, reps=reps, ctx=ctx)
+        if ctx is not None:
+            r["wavenumber"] = ctx
         for prec in ("fp64", "mixed"):
             r[prec]["ref_TFLOP_per_s"] = round(r[prec]["G_int_per_s"] * flops / 1e3, 2)
         r["ref_flops_per_interaction"] = flops

@@ -105,9 +105,75 @@ __device__ __forceinline__ double hw_rsq(double x) { return __builtin_amdgcn_rsq
 __device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-// sin and cos of one argument (the device math library's sincos).
-__device__ __forceinline__ void dev_sincos(double x, double* s, double* c) { sincos(x, s, c); }
-__device__ __forceinline__ void dev_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
+// sin and cos of one argument, for the Helmholtz phase mu*r.  Cody and
+// Waite's reduction x = k*pi/2 + r (W. J. Cody, W. Waite, Software Manual for
+// the Elementary Functions, 1980), with pi/2 split into three FP64 (two FP32)
+// parts, each step one FMA, then the Taylor series of sin and cos on
+// |r| <= pi/4 (truncated below 2^-55 resp. 2^-28 relative) and the quadrant
+// k mod 4.  Arguments beyond 2^20 (FP64) / 2^16 (FP32), where the split
+// would lose accuracy, take the device math library's sincos instead.
+template <class T> struct SinCosConst;
+template <> struct SinCosConst<double> {
+  static constexpr double TWO_OVER_PI = 0.6366197723675814;
+  static constexpr double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
+  static constexpr double LIMIT = 1048576.0;
+};
+template <> struct SinCosConst<float> {
+  static constexpr float TWO_OVER_PI = 0.63661975f;
+  static constexpr float P1 = 1.57079637f, P2 = -4.37113883e-08f;
+  static constexpr float LIMIT = 65536.0f;
+};
+
+__host__ __device__ __forceinline__ double round_even(double x) { return __builtin_rint(x); }
+__host__ __device__ __forceinline__ float round_even(float x) { return __builtin_rintf(x); }
+
+template <class T> __host__ __device__ __forceinline__ void sincos_reduced(T x, T* s, T* c) {
+  using C = SinCosConst<T>;
+  const T k = round_even(x * C::TWO_OVER_PI);
+  T r = __builtin_fma(-k, C::P1, x);
+  r = __builtin_fma(-k, C::P2, r);
+  if constexpr (std::is_same<T, double>::value) r = __builtin_fma(-k, C::P3, r);
+  const T r2 = r * r;
+  T ps, pc;  // sin r = r + r^3 ps(r^2), cos r = 1 + r^2 pc(r^2)
+  if constexpr (std::is_same<T, double>::value) {
+    ps = __builtin_fma(r2, 1.0 / 355687428096000.0, -1.0 / 1307674368000.0);
+    ps = __builtin_fma(r2, ps, 1.0 / 6227020800.0);
+    ps = __builtin_fma(r2, ps, -1.0 / 39916800.0);
+    ps = __builtin_fma(r2, ps, 1.0 / 362880.0);
+    ps = __builtin_fma(r2, ps, -1.0 / 5040.0);
+    ps = __builtin_fma(r2, ps, 1.0 / 120.0);
+    ps = __builtin_fma(r2, ps, -1.0 / 6.0);
+    pc = __builtin_fma(r2, 1.0 / 20922789888000.0, -1.0 / 87178291200.0);
+    pc = __builtin_fma(r2, pc, 1.0 / 479001600.0);
+    pc = __builtin_fma(r2, pc, -1.0 / 3628800.0);
+    pc = __builtin_fma(r2, pc, 1.0 / 40320.0);
+    pc = __builtin_fma(r2, pc, -1.0 / 720.0);
+    pc = __builtin_fma(r2, pc, 1.0 / 24.0);
+    pc = __builtin_fma(r2, pc, -0.5);
+  } else {
+    ps = __builtin_fmaf(r2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    ps = __builtin_fmaf(r2, ps, 1.0f / 120.0f);
+    ps = __builtin_fmaf(r2, ps, -1.0f / 6.0f);
+    pc = __builtin_fmaf(r2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+    pc = __builtin_fmaf(r2, pc, -1.0f / 720.0f);
+    pc = __builtin_fmaf(r2, pc, 1.0f / 24.0f);
+    pc = __builtin_fmaf(r2, pc, -0.5f);
+  }
+  const T sr = __builtin_fma(r * r2, ps, r), cr = __builtin_fma(r2, pc, (T)1);
+  const int q = (int)k & 3;  // quadrant (k mod 4, also for negative k)
+  const T sq = (q & 1) ? cr : sr, cq = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -sq : sq;
+  *c = ((q + 1) & 2) ? -cq : cq;
+}
+
+__device__ __forceinline__ void dev_sincos(double x, double* s, double* c) {
+  if (__builtin_fabs(x) > SinCosConst<double>::LIMIT) sincos(x, s, c);
+  else sincos_reduced(x, s, c);
+}
+__device__ __forceinline__ void dev_sincos(float x, float* s, float* c) {
+  if (__builtin_fabsf(x) > SinCosConst<float>::LIMIT) sincosf(x, s, c);
+  else sincos_reduced(x, s, c);
+}
 
 // x cos x - sin x without cancellation for |x| < 1/2 (its Taylor series
 // -x^3/3 + x^5/30 - x^7/840 + x^9/45360, next term < 2^-28 relative there),
