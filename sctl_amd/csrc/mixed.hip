@@ -566,10 +566,16 @@ int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, cons
   SCTL_HIP_TRY(hipGetLastError());
 
   // Split the sources over gridDim.y when the target blocks cannot fill the
-  // device (>= 8 workgroups per CU, each split >= 4 tiles).
+  // device (>= 8 workgroups per CU, each split >= 4 tiles), and so that no
+  // split exceeds SPLIT_TILES tiles: workgroups are dispatched in order of
+  // their linear index (blockIdx.x fastest), so the ones resident at a time
+  // share one split's records (2 MiB for Laplace FxU) in their XCD's 4 MiB L2
+  // instead of drifting apart over the whole source array.
+  constexpr int64_t SPLIT_TILES = 256;
   const int64_t want = 8LL * std::max(dev.num_cus, 1);
   int64_t nsplit = tb < want ? (want + tb - 1) / tb : 1;
   nsplit = std::min<int64_t>(std::min<int64_t>(nsplit, std::max<int64_t>(1, ntiles / 4)), 65535);
+  nsplit = std::min<int64_t>(std::max<int64_t>(nsplit, (ntiles + SPLIT_TILES - 1) / SPLIT_TILES), 65535);
   const int64_t tps = (ntiles + nsplit - 1) / nsplit;
   nsplit = (ntiles + tps - 1) / tps;
 
