@@ -207,12 +207,17 @@ KMatFn kmat_fn(int ker, int newton) {
 }
 
 // Splits the source range over gridDim.y when the target blocks alone cannot
-// fill the device: aim for >= 8 workgroups per CU, each split >= 4 tiles.
+// fill the device (aim for >= 8 workgroups per CU, each split >= 4 tiles),
+// and so that no split exceeds SPLIT_TILES tiles (2 MiB of Laplace sources):
+// the workgroups resident at a time (dispatched in linear-index order) then
+// share one split in L2 rather than each streaming the whole source array.
 void split_sources(int64_t target_blocks, int64_t ntiles, int num_cus, int64_t* nsplit, int64_t* tiles_per_split) {
+  constexpr int64_t SPLIT_TILES = 256;
   const int64_t want = 8LL * std::max(num_cus, 1);
   int64_t ns_ = 1;
   if (target_blocks < want) ns_ = (want + target_blocks - 1) / target_blocks;
   ns_ = std::min<int64_t>(ns_, std::max<int64_t>(1, ntiles / 4));
+  ns_ = std::max<int64_t>(ns_, (ntiles + SPLIT_TILES - 1) / SPLIT_TILES);
   ns_ = std::min<int64_t>(ns_, 65535);
   int64_t tps = (ntiles + ns_ - 1) / ns_;
   *tiles_per_split = tps;
