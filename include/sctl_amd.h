@@ -79,6 +79,7 @@ enum {
 /* flags */
 #define SCTL_AMD_HOST_PTRS 0x1u     /* array arguments are host memory; call is synchronous */
 #define SCTL_AMD_MIXED_NEWTON 0x2u  /* MIXED: one FP32 Newton step after v_rsq_f32 */
+#define SCTL_AMD_TUNE_MATRIX_CORES 0x4u /* sctl_amd_tune_eval: also time the matrix-core far path */
 
 /* ------------------------------------------------------------------ setup */
 
@@ -315,9 +316,10 @@ int sctl_amd_gen_normal(uint64_t seed, uint64_t stream_id, int64_t n, double* ou
  * sctl_amd_eval / sctl_amd_plan_create with a matching key use the result. */
 int sctl_amd_tune_db_path(const char* path);
 
-/* Dense Eval: targets per lane {1,2,4} x workgroups-per-CU cap {none,3,2},
- * and for the mixed Laplace kernels the far path on the VALU or on the matrix
- * cores.  Device pointers only; v_scratch (nt*K1) receives garbage. */
+/* Dense Eval: targets per lane {1,2,4} x workgroups-per-CU cap {none,3,2} on
+ * the VALU; with SCTL_AMD_TUNE_MATRIX_CORES in flags, for the mixed Laplace
+ * kernels also r^2 of the far tiles on the matrix cores.  Device pointers
+ * only; v_scratch (nt*K1) receives garbage. */
 int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_trg, int64_t nt, const double* r_src,
                        const double* n_src, const double* v_src, int64_t ns, double* v_scratch, uint32_t flags,
                        void* stream, int* best_tpl, int* best_occ, double* best_ms, int* best_matrix_cores);

@@ -36,7 +36,7 @@ KERNEL_NAMES = {0: "Laplace3D_FxU", 1: "Laplace3D_DxU", 2: "Stokes3D_FxU", 3: "S
 CTX_KERNELS = frozenset(range(12, 16))
 NUM_KERNELS = len(KERNEL_NAMES)
 FP64, MIXED, AUTO = 0, 1, 2
-HOST_PTRS, MIXED_NEWTON = 0x1, 0x2
+HOST_PTRS, MIXED_NEWTON, TUNE_MATRIX_CORES = 0x1, 0x2, 0x4
 NEAR_SUBTRACT, NEAR_STORE = 0, 1
 
 STATUS = {0: "SUCCESS", 1: "INVALID_VALUE", 2: "UNKNOWN_KERNEL", 3: "UNSUPPORTED", 4: "NO_DEVICE",

@@ -164,14 +164,6 @@ DenseShape pick_dense_shape(int ker, int precision, int newton, int64_t nt, int6
     s.mfma = v.size() >= 4 ? (int)v[3] : 0;
     return s;
   }
-  // Mixed Laplace kernels on large target sets: the matrix-core far path with
-  // one target per lane (the fastest shape measured on 2^20 x 2^20).
-  if (precision == SCTL_AMD_MIXED && mfma_supported(ker) && nt >= (1 << 16)) {
-    s.tpl = 1;
-    s.occ = 0;
-    s.mfma = 1;
-    return s;
-  }
   // FP64 on 2^20 or more targets: four targets per lane (the tuner's choice on
   // 2^20 x 2^20).  Otherwise enough target blocks to fill the device with two
   // targets per lane, else one target per lane and the source range split
@@ -263,7 +255,7 @@ int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_tr
     return SCTL_AMD_ERROR_UNKNOWN_KERNEL;
   }
   SCTL_CHECK_ARG(!(flags & SCTL_AMD_HOST_PTRS), "the tuner takes device pointers");
-  SCTL_CHECK_ARG((flags & ~SCTL_AMD_MIXED_NEWTON) == 0, "unknown flag bits");
+  SCTL_CHECK_ARG((flags & ~(SCTL_AMD_MIXED_NEWTON | SCTL_AMD_TUNE_MATRIX_CORES)) == 0, "unknown flag bits");
   SCTL_CHECK_ARG(nt > 0 && ns > 0 && r_trg && r_src && v_src && v_scratch && (!ND || n_src), "bad arguments");
   SCTL_CHECK_ARG(digits >= -1, "digits must be -1 or >= 0");
   SCTL_CHECK_ARG(precision >= SCTL_AMD_FP64 && precision <= SCTL_AMD_AUTO, "unknown precision");
@@ -276,7 +268,8 @@ int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_tr
   DenseShape bs;
   // The Helmholtz kernels are timed at wavenumber 1 (the shape does not depend on it).
   const double mu = kernel_needs_ctx(kernel) ? 1.0 : 0.0;
-  const bool with_mfma = precision == SCTL_AMD_MIXED && mfma_supported(kernel);
+  const bool with_mfma =
+      precision == SCTL_AMD_MIXED && mfma_supported(kernel) && (flags & SCTL_AMD_TUNE_MATRIX_CORES);
   for (int mfma = 0; mfma <= (with_mfma ? 1 : 0); mfma++) {
     for (int tpl : {1, 2, 4}) {
       if (mfma && tpl == 4) continue;  // 16 row tiles per wave do not fit the register file

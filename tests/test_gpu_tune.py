@@ -90,8 +90,12 @@ def test_tune_eval_records_choice(gpu, tmp_path):
         d = [gpu.DeviceArray.from_host(a) for a in (xt, xs, f)]
         scratch = gpu.DeviceArray(40000)
         tpl, cap, ms, mc = gpu.tune_eval(0, d[0], d[1], None, d[2], scratch, precision=1)
-        assert tpl in (1, 2, 4) and cap in (0, 3, 2) and mc in (0, 1) and ms > 0
+        assert tpl in (1, 2, 4) and cap in (0, 3, 2) and mc == 0 and ms > 0  # VALU shapes only by default
         assert gpu.dense_shape(0, 30000, 40000, precision=1) == (tpl, cap, mc)  # cube(n_src, n_trg)
+        tpl, cap, ms, mc = gpu.tune_eval(0, d[0], d[1], None, d[2], scratch, precision=1,
+                                         flags=gpu.TUNE_MATRIX_CORES)
+        assert tpl in (1, 2, 4) and cap in (0, 3, 2) and mc in (0, 1) and ms > 0
+        assert gpu.dense_shape(0, 30000, 40000, precision=1) == (tpl, cap, mc)
         assert "dense k0 p1 n0" in open(tmp_path / "t.db").read()
         ref = oracle.eval(0, xt[:3000], xs, None, f)
         out = gpu.eval(0, xt, xs, None, f, precision=1)
