@@ -17,13 +17,13 @@
 // (broadcast from LDS).
 //
 // Reciprocal square root.  The hardware estimate (v_rsq_f64 / v_rsq_f32) is
-// refined by N deferred-scaling Newton steps, the scheme of
-// intrin-wrapper.hpp:540-559: z <- z * (x*z*z + c1_j) with c1_j =
-// -3*2^(3^j-1), and the constant (-1/2)^((3^(N-1)-1)*3/2+1) applied once.
-// Here that constant is folded into the per-target scale wherever the kernel
-// value is a single power of 1/r (Laplace FxU: r^-1, Laplace DxU and FxdU,
-// BiotSavart FxU: r^-3, Stokes DxU and FxT: r^-5), so a Newton step costs
-// three VALU operations; the other kernels scale 1/r once per pair.
+// refined by one deferred-scaling Newton step, the scheme of
+// intrin-wrapper.hpp:540-559 (z <- z * (x*z*z - 3), the constant -1/2 applied
+// once), for about 14 digits, or by one third-order step for full FP64
+// precision (rsq_deferred).  The Newton constant is folded into the
+// per-target scale wherever the kernel value is a single power of 1/r
+// (Laplace FxU: r^-1, Laplace DxU and FxdU, BiotSavart FxU: r^-3, Stokes DxU
+// and FxT: r^-5); the other kernels scale 1/r once per pair.
 //
 // Masking.  The reference masks r2 > 0 so that coincident points contribute
 // exactly zero (kernel_functions.hpp:45, vec-test.hpp:449-463).  The fast
@@ -222,17 +222,27 @@ __device__ __forceinline__ void helmholtz_factor(T r2, T z, T mu, T* G0, T* G1) 
   }
 }
 
-// Deferred-scaling Newton constants (intrin-wrapper.hpp:540-559).
+// Deferred-scaling Newton constants (intrin-wrapper.hpp:540-559).  NEWTON = 2
+// (full FP64 precision) is not two deferred steps here but one third-order
+// step, which needs no final constant (newton_c2(2) = 1; see rsq_deferred).
 __host__ __device__ constexpr double newton_c1(int j) { return j == 0 ? -3.0 : (j == 1 ? -12.0 : -768.0); }
-__host__ __device__ constexpr double newton_c2(int n) { return n == 0 ? 1.0 : (n == 1 ? -0.5 : (n == 2 ? 1.0 / 16 : -1.0 / 8192)); }
+__host__ __device__ constexpr double newton_c2(int n) { return n == 1 ? -0.5 : 1.0; }
 __host__ __device__ constexpr double ipow(double x, int p) { return p == 0 ? 1.0 : x * ipow(x, p - 1); }
 
-// Unscaled rsqrt estimate after NEWTON deferred steps: equals
-// newton_c2(NEWTON)^-1 / sqrt(x) up to rounding.
+// Unscaled rsqrt: equals newton_c2(NEWTON)^-1 / sqrt(x) up to rounding.
+//   NEWTON 0: the hardware estimate (v_rsq_f64 / v_rsq_f32, ~2^-23 relative);
+//   NEWTON 1: one deferred Newton step z (x z^2 - 3) (= -2 / sqrt(x), ~2^-47);
+//   NEWTON 2: the third-order step z (1 + e/2 + 3 e^2 / 8), e = 1 - x z^2,
+//             the first terms of (1 - e)^(-1/2) (Householder's method of
+//             order 3): error ~ (5/16) e^3 ~ 2^-70, i.e. rounding only, in 5
+//             operations instead of the 6 of two deferred Newton steps.
 template <int NEWTON, class T> __device__ __forceinline__ T rsq_deferred(T x) {
   T z = hw_rsq(x);
-  if constexpr (NEWTON >= 1) z = z * fmad(x, z * z, (T)newton_c1(0));
-  if constexpr (NEWTON >= 2) z = z * fmad(x, z * z, (T)newton_c1(1));
+  if constexpr (NEWTON == 1) z = z * fmad(x, z * z, (T)newton_c1(0));
+  if constexpr (NEWTON >= 2) {
+    const T e = fmad(-(x * z), z, (T)1);
+    z = fmad(z * e, fmad(e, (T)0.375, (T)0.5), z);
+  }
   return z;
 }
 
