@@ -289,3 +289,25 @@ def test_full_size_sphere_shell(gpu, k):
     ref = oracle.eval(k, xt[: 3 * sub], xs, n, f)
     assert nerr(outs[0][:sub], ref) < FP64_TOL
     assert nerr(outs[0] + outs[1], outs[2]) < 1e-12
+
+
+@pytest.mark.parametrize("k", [0, 3, 7, 12], ids=["LaplaceFxU", "StokesDxU", "StokesFxT", "HelmholtzFxU"])
+@pytest.mark.parametrize("offset,scale", [(1e6, 1.0), (-3e3, 1e-4), (0.0, 1e5), (1e9, 1e3)],
+                         ids=["far-offset", "small-scale", "large-scale", "offset-and-scale"])
+def test_mixed_translation_and_scale(gpu, k, offset, scale):
+    """Mixed precision differences positions in FP64 against tile centres, so
+    its accuracy does not depend on where the points sit or (within the FP32
+    exponent range of r^-5) on their scale: the same configuration translated
+    and scaled against the oracle.  Densities and the Helmholtz wavenumber are
+    scaled with it so that the kernel values stay comparable."""
+    rng = np.random.default_rng(31)
+    xt, xs, n, v = rand_case(rng, k, 3000, 2500, coincide=20)
+    xt, xs = xt * scale + offset, xs * scale + offset
+    mu = 2.0 / scale if k >= 12 else None
+    ref = oracle.eval(k, xt, xs, n, v, ctx=mu)
+    out = gpu.eval(k, xt, xs, n, v, precision=gpu.MIXED, ctx=mu)
+    assert np.isfinite(out).all()
+    e_inf, e_rms = W.rel_errors(out, ref)
+    assert e_inf < MIXED_TOL and e_rms < MIXED_TOL, (e_inf, e_rms)
+    f64 = gpu.eval(k, xt, xs, n, v, ctx=mu)
+    assert nerr(f64, ref) < FP64_TOL
