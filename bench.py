@@ -150,6 +150,27 @@ def dense_pair(kernel, xt, xs, n, f, stream, reps, digits=-1, ctx=None):
     return res, outs
 
 
+def dense_plan_rates(kernel, xt, xs, n, f, stream, reps):
+    """Repeated applies of a dense plan (geometry resident, set-up once):
+    rates per precision and the setup time."""
+    _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
+    nt, ns = xt.size // 3, xs.size // 3
+    d = [S.DeviceArray.from_host(a) for a in (xt, xs, n if ND else np.zeros(3), f)]
+    res = {}
+    for name, prec in (("fp64", S.FP64), ("mixed", S.MIXED)):
+        t = time.perf_counter()
+        P = S.DensePlan(kernel, d[0], d[1], d[2] if ND else None, precision=prec, stream=stream)
+        setup_ms = (time.perf_counter() - t) * 1e3
+        o = S.DeviceArray.from_host(np.zeros(nt * K1))
+        P.apply(d[3], o, stream=stream)
+        S.synchronize(stream)
+        ms = time_once(lambda: P.apply(d[3], o, stream=stream), stream, reps)
+        res[name] = {"ms_per_apply": round(ms, 4), "G_int_per_s": round(nt * ns / ms / 1e6, 2),
+                     "setup_ms": round(setup_ms, 3)}
+    res["speedup"] = round(res["fp64"]["ms_per_apply"] / res["mixed"]["ms_per_apply"], 3)
+    return res
+
+
 def plan_pair(kernel, B, stream, reps):
     _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
     nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
@@ -337,6 +358,7 @@ def other_configs(stream, quick=False):
     ref = oracle.eval(S.LAPLACE3D_FXU, xt, xs, None, f)
     r["fp64_vs_oracle"] = dict(zip(("max_rel", "rms_rel"), (float("%.3g" % e) for e in W.rel_errors(o["fp64"], ref))))
     r["mixed_vs_oracle"] = dict(zip(("max_rel", "rms_rel"), (float("%.3g" % e) for e in W.rel_errors(o["mixed"], ref))))
+    r["dense_plan"] = dense_plan_rates(S.LAPLACE3D_FXU, xt, xs, None, f, stream, reps=20)
     out["c1_laplace_fxu_cube_20k"] = r
     N = 1 << 20
     log("config 2 (DxU leg): Laplace DxU, 2^20 sphere sources x 2^20 shell targets")

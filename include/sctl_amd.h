@@ -140,6 +140,34 @@ int sctl_amd_kernel_matrix_ctx(int kernel, const double* ctx, int digits, const 
                                const double* r_src, const double* n_src, int64_t ns, double* M, uint32_t flags,
                                void* stream);
 
+/* -------------------------------------------------------------- dense plans
+ * One (target set, source set) Eval whose geometry stays on the device across
+ * applications: the far field that BoundaryIntegralOp::ComputeFarField
+ * (boundary_integral.txx:1248-1309) and ParticleFMM::EvalDirect
+ * (fmm-wrapper.txx:551-623) sum once per GMRES iteration with new densities.
+ * Creation copies r_trg (nt x 3), r_src and n_src (ns x 3; n_src NULL iff the
+ * kernel has no normal) to plan-owned device memory (HOST_PTRS: from host
+ * memory), resolves precision and launch shape as sctl_amd_eval does and, for
+ * mixed precision, orders targets and sources once; ctx is the Helmholtz
+ * wavenumber (read now).  flags: HOST_PTRS, MIXED_NEWTON. */
+typedef struct sctl_amd_dense_plan_s* sctl_amd_dense_plan;
+
+int sctl_amd_dense_plan_create(sctl_amd_dense_plan* plan, int kernel, int precision, int digits, uint32_t flags,
+                               const double* r_trg, int64_t nt, const double* r_src, const double* n_src, int64_t ns,
+                               const double* ctx, void* stream);
+
+/* v_trg (nt*K1) += Eval(densities v_src (ns*K0)), the same values as
+ * sctl_amd_eval with the plan's shape.  With device pointers it allocates
+ * nothing and issues no memset (capturable in a hipGraph). */
+int sctl_amd_dense_plan_apply(sctl_amd_dense_plan plan, const double* v_src, double* v_trg, uint32_t flags,
+                              void* stream);
+
+/* The resolved precision and launch shape (any output may be NULL). */
+int sctl_amd_dense_plan_shape(sctl_amd_dense_plan plan, int* precision, int* targets_per_lane, int* wg_per_cu_cap,
+                              int* matrix_cores);
+
+int sctl_amd_dense_plan_destroy(sctl_amd_dense_plan plan);
+
 /* ------------------------------------------------------------ batched blocks
  * A plan holds nblocks (target-set, source-set) blocks over one target array
  * and one source array -- the near-field blocks (elem, t0, cnt) that
@@ -299,6 +327,9 @@ typedef struct sctl_amd_graph_s* sctl_amd_graph;
 
 int sctl_amd_plan_graph_create(sctl_amd_graph* graph, sctl_amd_plan plan, int nsteps, uint64_t seed,
                                uint64_t stream_id, double* v_src, double* v_trg);
+/* The same for a dense plan: v_src (ns*K0), v_trg (nt*K1). */
+int sctl_amd_dense_plan_graph_create(sctl_amd_graph* graph, sctl_amd_dense_plan plan, int nsteps, uint64_t seed,
+                                     uint64_t stream_id, double* v_src, double* v_trg);
 int sctl_amd_graph_launch(sctl_amd_graph graph, void* stream);
 int sctl_amd_graph_interactions(sctl_amd_graph graph, int64_t* interactions);
 int sctl_amd_graph_destroy(sctl_amd_graph graph);

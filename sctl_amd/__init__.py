@@ -54,7 +54,8 @@ SYMBOLS = [
     "sctl_amd_near_blocks", "sctl_amd_plan_near_layout", "sctl_amd_plan_near_matrix", "sctl_amd_plan_near_gemv",
     "sctl_amd_near_accumulate", "sctl_amd_dense_shape_get", "sctl_amd_dense_shape_set", "sctl_amd_near_list_build",
     "sctl_amd_near_list_size", "sctl_amd_near_list_get", "sctl_amd_near_list_destroy", "sctl_amd_eval_ctx",
-    "sctl_amd_kernel_matrix_ctx", "sctl_amd_plan_set_ctx",
+    "sctl_amd_kernel_matrix_ctx", "sctl_amd_plan_set_ctx", "sctl_amd_dense_plan_create", "sctl_amd_dense_plan_apply",
+    "sctl_amd_dense_plan_shape", "sctl_amd_dense_plan_destroy", "sctl_amd_dense_plan_graph_create",
 ]
 
 
@@ -92,6 +93,12 @@ def lib():
     L.sctl_amd_kernel_matrix.argtypes = [i32, i32, _vp, i64, _vp, _vp, i64, _vp, u32, _vp]
     L.sctl_amd_kernel_matrix_ctx.argtypes = [i32, _dp, i32, _vp, i64, _vp, _vp, i64, _vp, u32, _vp]
     L.sctl_amd_plan_set_ctx.argtypes = [_vp, _dp]
+    L.sctl_amd_dense_plan_create.argtypes = [ctypes.POINTER(_vp), i32, i32, i32, u32, _vp, i64, _vp, _vp, i64, _dp, _vp]
+    L.sctl_amd_dense_plan_apply.argtypes = [_vp, _vp, _vp, u32, _vp]
+    L.sctl_amd_dense_plan_shape.argtypes = [_vp] + [ctypes.POINTER(i32)] * 4
+    L.sctl_amd_dense_plan_destroy.argtypes = [_vp]
+    L.sctl_amd_dense_plan_graph_create.argtypes = [ctypes.POINTER(_vp), _vp, i32, ctypes.c_uint64, ctypes.c_uint64,
+                                                   _vp, _vp]
     L.sctl_amd_plan_create.argtypes = [ctypes.POINTER(_vp), i32, i32, i32, i64, _i64p, _i64p, _i64p, _i64p, i64, i64]
     L.sctl_amd_plan_set_geometry.argtypes = [_vp, _vp, _vp, _vp, u32, _vp]
     L.sctl_amd_plan_apply.argtypes = [_vp, _vp, _vp, u32, _vp]
@@ -439,6 +446,62 @@ class Plan:
             pass
 
 
+class DensePlan:
+    """One (targets, sources) Eval with resident geometry: create once, apply
+    with new densities (the far field of a GMRES loop)."""
+
+    def __init__(self, kernel, r_trg, r_src, n_src=None, precision=FP64, digits=-1, flags=0, ctx=None, stream=None):
+        _, K0, K1, ND, _, _, _ = kernel_info(kernel)
+        self.kernel, self.K0, self.K1 = kernel, K0, K1
+        if isinstance(r_trg, DeviceArray):
+            nt, ns = r_trg.nbytes // 24, r_src.nbytes // 24
+        else:
+            r_trg, r_src = _host(r_trg), _host(r_src)
+            n_src = _host(n_src) if ND else None
+            nt, ns = r_trg.size // 3, r_src.size // 3
+            flags |= HOST_PTRS
+        if not ND:
+            n_src = None
+        self.nt, self.ns = nt, ns
+        h = _vp()
+        check(lib().sctl_amd_dense_plan_create(ctypes.byref(h), kernel, precision, digits, flags, _ptr(r_trg), nt,
+                                               _ptr(r_src), _ptr(n_src), ns, _ctx(ctx), stream))
+        self.handle = h
+
+    def apply(self, v_src, v_trg=None, stream=None):
+        """v_trg += Eval(v_src); host arrays are staged (and v_trg returned),
+        DeviceArrays run asynchronously on `stream`."""
+        flags = 0
+        if not isinstance(v_src, DeviceArray):
+            v_src = _host(v_src)
+            if v_trg is None:
+                v_trg = np.zeros(self.nt * self.K1)
+            flags = HOST_PTRS
+        check(lib().sctl_amd_dense_plan_apply(self.handle, _ptr(v_src), _ptr(v_trg), flags, stream))
+        return v_trg
+
+    def shape(self):
+        """(precision, targets_per_lane, wg_per_cu_cap, matrix_cores)."""
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().sctl_amd_dense_plan_shape(self.handle, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def graph(self, nsteps, seed, stream_id, v_src, v_trg):
+        """nsteps applies with densities regenerated on the device, as one hipGraph."""
+        return Graph(self, nsteps, seed, stream_id, v_src, v_trg)
+
+    def destroy(self):
+        if self.handle:
+            lib().sctl_amd_dense_plan_destroy(self.handle)
+            self.handle = _vp()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class Graph:
     """nsteps applications of a plan captured as one hipGraph: for step s the
     device buffer v_src is refilled with N(0,1) draws of (seed + s, stream_id),
@@ -446,8 +509,9 @@ class Graph:
 
     def __init__(self, plan, nsteps, seed, stream_id, v_src, v_trg):
         h = _vp()
-        check(lib().sctl_amd_plan_graph_create(ctypes.byref(h), plan.handle, nsteps, seed, stream_id, v_src.ptr,
-                                               v_trg.ptr))
+        create = lib().sctl_amd_dense_plan_graph_create if isinstance(plan, DensePlan) else \
+            lib().sctl_amd_plan_graph_create
+        check(create(ctypes.byref(h), plan.handle, nsteps, seed, stream_id, v_src.ptr, v_trg.ptr))
         self.handle = h
         self._keep = (plan, v_src, v_trg)
 

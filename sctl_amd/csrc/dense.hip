@@ -226,9 +226,25 @@ void split_sources(int64_t target_blocks, int64_t ntiles, int num_cus, int64_t* 
 
 }  // namespace
 
-int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src, const double* n_src,
-                    const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape, double ctx,
-                    hipStream_t stream) {
+// Bytes of the per-split partial sums an FP64 Eval of this size and shape needs.
+int f64_part_bytes(int ker, int64_t nt, int64_t ns, const DenseShape& shape, size_t* bytes) {
+  *bytes = 0;
+  if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
+  DeviceInfo dev;
+  SCTL_TRY(device_info(&dev));
+  int K0, K1, ND;
+  kernel_dims(ker, &K0, &K1, &ND);
+  const int tpl = TPLS[std::min(tpl_index(shape.tpl), max_tpl_index(ker))];
+  const int64_t tb = (nt + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
+  int64_t nsplit, tps;
+  split_sources(tb, (ns + TILE - 1) / TILE, dev.num_cus, &nsplit, &tps);
+  *bytes = nsplit > 1 ? sizeof(double) * nsplit * nt * K1 : 0;
+  return SCTL_AMD_SUCCESS;
+}
+
+int f64_apply(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src, const double* n_src,
+              const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape, double ctx, double* part,
+              hipStream_t stream) {
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
   DeviceInfo dev;
   SCTL_TRY(device_info(&dev));
@@ -240,27 +256,37 @@ int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const 
   const int64_t ntiles = (ns + TILE - 1) / TILE;
   int64_t nsplit, tps;
   split_sources(tb, ntiles, dev.num_cus, &nsplit, &tps);
+  SCTL_CHECK_ARG(nsplit == 1 || part, "no buffer for the split partial sums");
   const double scale = kernel_scale(ker) * fold_factor(ker, newton);
   const EvalF64Fn fn = f64_fn(ker, newton, ti);
   size_t dyn = 0;
   SCTL_TRY(occupancy_lds((const void*)fn, shape.occ, sizeof(double) * TILE * rec_words_f64(ker), &dyn));
-  Scratch part;
-  if (nsplit > 1) SCTL_TRY(part.alloc(sizeof(double) * nsplit * nt * K1, stream));
   dim3 grid((unsigned)tb, (unsigned)nsplit);
-  profile_count(ker, nt * ns);
+  if (!stream_capturing(stream)) profile_count(ker, nt * ns);
   {
     ProfileScope prof(stream);
     hipLaunchKernelGGL(fn, grid, dim3(WG), dyn, stream, r_trg, nt, r_src, n_src, v_src, ns, tps * TILE,
-                       nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale, ctx);
+                       nsplit > 1 ? part : v_trg, nsplit > 1 ? 0 : 1, scale, ctx);
   }
   SCTL_HIP_TRY(hipGetLastError());
   if (nsplit > 1) {
     const int64_t n = nt * K1;
-    hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part.as<double>(),
-                       n, (int)nsplit, v_trg, scale);
+    hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part, n,
+                       (int)nsplit, v_trg, scale);
     SCTL_HIP_TRY(hipGetLastError());
   }
   return SCTL_AMD_SUCCESS;
+}
+
+int launch_eval_f64(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src, const double* n_src,
+                    const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape, double ctx,
+                    hipStream_t stream) {
+  if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
+  size_t bytes = 0;
+  SCTL_TRY(f64_part_bytes(ker, nt, ns, shape, &bytes));
+  Scratch part;
+  SCTL_TRY(part.alloc(bytes, stream));
+  return f64_apply(ker, newton, r_trg, nt, r_src, n_src, v_src, ns, v_trg, shape, ctx, part.as<double>(), stream);
 }
 
 int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,

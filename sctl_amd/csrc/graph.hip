@@ -1,4 +1,4 @@
-// Repeated applications of a plan captured as one hipGraph, and the device
+// Repeated applications of a plan (batched or dense) captured as one hipGraph, and the device
 // side of the counter-based random number generator.
 //
 // A Krylov solve applies the same boundary-integral operator once per
@@ -82,23 +82,21 @@ int sctl_amd_gen_normal(uint64_t seed, uint64_t stream_id, int64_t n, double* ou
   return launch_gen_normal(seed, stream_id, n, out, stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread);
 }
 
-int sctl_amd_plan_graph_create(sctl_amd_graph* out, sctl_amd_plan plan, int nsteps, uint64_t seed,
-                               uint64_t stream_id, double* v_src, double* v_trg) {
-  SCTL_CHECK_ARG(out, "graph is NULL");
-  *out = nullptr;
-  SCTL_CHECK_ARG(plan, "plan is NULL");
-  SCTL_CHECK_ARG(nsteps >= 1, "nsteps must be >= 1");
-  int64_t ns = 0, nt = 0, inter = 0;
-  int k0 = 0, k1 = 0;
-  SCTL_TRY(plan_sizes(plan, &ns, &nt, &k0, &k1, &inter));
-  SCTL_CHECK_ARG(ns == 0 || v_src, "v_src is NULL");
-  SCTL_CHECK_ARG(nt == 0 || v_trg, "v_trg is NULL");
+}  // extern "C"
+
+namespace {
+
+// Captures nsteps x (regenerate v_src, apply) on a private stream and
+// instantiates the graph.
+template <class Apply>
+int capture_applies(sctl_amd_graph* out, int kernel, int64_t n_dens, int64_t interactions, int nsteps, uint64_t seed,
+                    uint64_t stream_id, double* v_src, Apply apply) {
   hipStream_t cap;
   SCTL_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   auto* g = new sctl_amd_graph_s();
   g->nsteps = nsteps;
-  g->kernel = plan_kernel(plan);
-  g->interactions = inter * nsteps;
+  g->kernel = kernel;
+  g->interactions = interactions * nsteps;
   int rc = SCTL_AMD_SUCCESS;
   hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) {
@@ -106,8 +104,8 @@ int sctl_amd_plan_graph_create(sctl_amd_graph* out, sctl_amd_plan plan, int nste
     rc = SCTL_AMD_ERROR_HIP;
   }
   for (int step = 0; step < nsteps && rc == SCTL_AMD_SUCCESS; step++) {
-    rc = launch_gen_normal(seed + (uint64_t)step, stream_id, ns * k0, v_src, cap);
-    if (rc == SCTL_AMD_SUCCESS) rc = sctl_amd_plan_apply(plan, v_src, v_trg, 0, cap);
+    rc = launch_gen_normal(seed + (uint64_t)step, stream_id, n_dens, v_src, cap);
+    if (rc == SCTL_AMD_SUCCESS) rc = apply(cap);
   }
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(cap, &graph);
@@ -130,6 +128,40 @@ int sctl_amd_plan_graph_create(sctl_amd_graph* out, sctl_amd_plan plan, int nste
   }
   *out = g;
   return SCTL_AMD_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sctl_amd_plan_graph_create(sctl_amd_graph* out, sctl_amd_plan plan, int nsteps, uint64_t seed,
+                               uint64_t stream_id, double* v_src, double* v_trg) {
+  SCTL_CHECK_ARG(out, "graph is NULL");
+  *out = nullptr;
+  SCTL_CHECK_ARG(plan, "plan is NULL");
+  SCTL_CHECK_ARG(nsteps >= 1, "nsteps must be >= 1");
+  int64_t ns = 0, nt = 0, inter = 0;
+  int k0 = 0, k1 = 0;
+  SCTL_TRY(plan_sizes(plan, &ns, &nt, &k0, &k1, &inter));
+  SCTL_CHECK_ARG(ns == 0 || v_src, "v_src is NULL");
+  SCTL_CHECK_ARG(nt == 0 || v_trg, "v_trg is NULL");
+  return capture_applies(out, plan_kernel(plan), ns * k0, inter, nsteps, seed, stream_id, v_src,
+                         [&](hipStream_t cap) { return sctl_amd_plan_apply(plan, v_src, v_trg, 0, cap); });
+}
+
+int sctl_amd_dense_plan_graph_create(sctl_amd_graph* out, sctl_amd_dense_plan plan, int nsteps, uint64_t seed,
+                                     uint64_t stream_id, double* v_src, double* v_trg) {
+  SCTL_CHECK_ARG(out, "graph is NULL");
+  *out = nullptr;
+  SCTL_CHECK_ARG(plan, "plan is NULL");
+  SCTL_CHECK_ARG(nsteps >= 1, "nsteps must be >= 1");
+  int64_t ns = 0, nt = 0;
+  int k0 = 0, k1 = 0, kernel = 0;
+  SCTL_TRY(dense_plan_sizes(plan, &ns, &nt, &k0, &k1, &kernel));
+  SCTL_CHECK_ARG(ns == 0 || v_src, "v_src is NULL");
+  SCTL_CHECK_ARG(nt == 0 || v_trg, "v_trg is NULL");
+  return capture_applies(out, kernel, ns * k0, ns * nt, nsteps, seed, stream_id, v_src,
+                         [&](hipStream_t cap) { return dense_plan_apply_device(plan, v_src, v_trg, cap); });
 }
 
 int sctl_amd_graph_launch(sctl_amd_graph g, void* stream) {

@@ -103,6 +103,29 @@ int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, cons
 int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
                          const double* n_src, int64_t ns, double* M, double ctx, hipStream_t stream);
 
+// The dense Eval launchers in two parts, for callers that keep the geometry
+// (dense plans): a one-time part (sizes; for mixed precision the Morton
+// orders) and an apply that allocates nothing and issues no memset.
+int f64_part_bytes(int ker, int64_t nt, int64_t ns, const DenseShape& shape, size_t* bytes);
+int f64_apply(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src, const double* n_src,
+              const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape, double ctx, double* part,
+              hipStream_t stream);
+struct MixedGeometry {
+  bool mfma = false;
+  int ti = 0, Q = 0;
+  int64_t tb = 0, ntiles = 0, nsplit = 1, tps = 0;
+  size_t packed_bytes = 0, meta_bytes = 0, part_bytes = 0;
+};
+int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, MixedGeometry* g);
+int mixed_order(const double* r_trg, int64_t nt, const double* r_src, int64_t ns, uint32_t* tperm, uint32_t* sperm,
+                hipStream_t stream);
+int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& shape, const double* r_trg, int64_t nt,
+                const uint32_t* tperm, const double* r_src, const double* n_src, const double* v_src, int64_t ns,
+                const uint32_t* sperm, float4* packed, double4* meta, double* part, double* v_trg, double ctx,
+                hipStream_t stream);
+// True while `s` is being captured into a graph (no host-side counting then).
+bool stream_capturing(hipStream_t s);
+
 // Morton ordering of points (30-bit keys over the points' bounding box);
 // writes a permutation of [0, n).  Implemented in sort.hip.
 int morton_order(const double* pts, int64_t n, uint32_t* perm, hipStream_t stream);
@@ -134,6 +157,10 @@ class ProfileScope {
   hipEvent_t a_ = nullptr;
   hipStream_t s_ = nullptr;
 };
+
+// Dense plans (dense_plan.cpp): sizes and the allocation-free apply.
+int dense_plan_sizes(sctl_amd_dense_plan p, int64_t* ns, int64_t* nt, int* K0, int* K1, int* kernel);
+int dense_plan_apply_device(sctl_amd_dense_plan p, const double* v_src, double* v_trg, hipStream_t s);
 
 // Plan sizes (batched.hip): source/target totals, K0/K1, interactions per apply.
 int plan_sizes(sctl_amd_plan p, int64_t* ns, int64_t* nt, int* K0, int* K1, int64_t* interactions);

@@ -537,69 +537,93 @@ int max_tpl_index(int ker) {
 
 }  // namespace
 
+// Launch geometry of one mixed Eval: target blocks and the source splits.
+// Split the sources over gridDim.y when the target blocks cannot fill the
+// device (>= 8 workgroups per CU, each split >= 4 tiles), and so that no
+// split exceeds SPLIT_TILES tiles: workgroups are dispatched in order of
+// their linear index (blockIdx.x fastest), so the ones resident at a time
+// share one split's records (2 MiB for Laplace FxU) in their XCD's 4 MiB L2
+// instead of drifting apart over the whole source array.
+int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, MixedGeometry* g) {
+  DeviceInfo dev;
+  SCTL_TRY(device_info(&dev));
+  SCTL_CHECK_ARG(ns <= 0xffffffffLL && nt <= 0xffffffffLL, "mixed precision supports at most 2^32-1 points");
+  g->mfma = shape.mfma && mfma_supported(ker);
+  g->ti = g->mfma ? tpl_index(shape.tpl) : std::min(tpl_index(shape.tpl), max_tpl_index(ker));
+  const int tpl = TPLS[g->ti];
+  g->tb = (nt + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
+  SCTL_CHECK_ARG(g->tb <= 0x7fffffffLL, "too many targets for one launch");
+  g->ntiles = (ns + TILE - 1) / TILE;
+  SCTL_CHECK_ARG(g->ntiles <= 0x7fffffffLL, "too many sources for one launch");
+  g->Q = layout_q(ker);
+  constexpr int64_t SPLIT_TILES = 256;
+  const int64_t want = 8LL * std::max(dev.num_cus, 1);
+  int64_t nsplit = g->tb < want ? (want + g->tb - 1) / g->tb : 1;
+  nsplit = std::min<int64_t>(std::min<int64_t>(nsplit, std::max<int64_t>(1, g->ntiles / 4)), 65535);
+  nsplit = std::min<int64_t>(std::max<int64_t>(nsplit, (g->ntiles + SPLIT_TILES - 1) / SPLIT_TILES), 65535);
+  g->tps = (g->ntiles + nsplit - 1) / nsplit;
+  g->nsplit = (g->ntiles + g->tps - 1) / g->tps;
+  int K0, K1, ND;
+  kernel_dims(ker, &K0, &K1, &ND);
+  g->packed_bytes = sizeof(float4) * g->Q * TILE * g->ntiles;
+  g->meta_bytes = sizeof(double4) * g->ntiles;
+  g->part_bytes = g->nsplit > 1 ? sizeof(double) * g->nsplit * nt * K1 : 0;
+  return SCTL_AMD_SUCCESS;
+}
+
+int mixed_order(const double* r_trg, int64_t nt, const double* r_src, int64_t ns, uint32_t* tperm, uint32_t* sperm,
+                hipStream_t stream) {
+  SCTL_TRY(morton_order(r_src, ns, sperm, stream));
+  SCTL_TRY(morton_order(r_trg, nt, tperm, stream));
+  return SCTL_AMD_SUCCESS;
+}
+
+int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& shape, const double* r_trg, int64_t nt,
+                const uint32_t* tperm, const double* r_src, const double* n_src, const double* v_src, int64_t ns,
+                const uint32_t* sperm, float4* packed, double4* meta, double* part, double* v_trg, double ctx,
+                hipStream_t stream) {
+  if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
+  int K0, K1, ND;
+  kernel_dims(ker, &K0, &K1, &ND);
+  hipLaunchKernelGGL(pack_fn(ker), dim3((unsigned)g.ntiles), dim3(WG), 0, stream, r_src, n_src, v_src, ns, sperm,
+                     packed, meta);
+  SCTL_HIP_TRY(hipGetLastError());
+  const double scale = kernel_scale(ker) * fold_factor(ker, newton ? 1 : 0);
+  const EvalMixFn fn = mix_fn(ker, newton, g.ti, g.mfma);
+  size_t dyn = 0;
+  SCTL_TRY(occupancy_lds((const void*)fn, shape.occ, sizeof(float4) * TILE * g.Q, &dyn));
+  if (!stream_capturing(stream)) profile_count(ker, nt * ns);
+  {
+    ProfileScope prof(stream);
+    hipLaunchKernelGGL(fn, dim3((unsigned)g.tb, (unsigned)g.nsplit), dim3(WG), dyn, stream, r_trg, nt, tperm, packed,
+                       meta, g.ntiles, g.tps, g.nsplit > 1 ? part : v_trg, g.nsplit > 1 ? 0 : 1, scale, ctx);
+  }
+  SCTL_HIP_TRY(hipGetLastError());
+  if (g.nsplit > 1) {
+    const int64_t n = nt * K1;
+    hipLaunchKernelGGL(k_reduce_splits_perm, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part, nt, K1,
+                       (int)g.nsplit, tperm, v_trg, scale);
+    SCTL_HIP_TRY(hipGetLastError());
+  }
+  return SCTL_AMD_SUCCESS;
+}
+
 int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
                       const double* n_src, const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape,
                       double ctx, hipStream_t stream) {
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
-  DeviceInfo dev;
-  SCTL_TRY(device_info(&dev));
-  int K0, K1, ND;
-  kernel_dims(ker, &K0, &K1, &ND);
-  SCTL_CHECK_ARG(ns <= 0xffffffffLL && nt <= 0xffffffffLL, "mixed precision supports at most 2^32-1 points");
-  const bool mfma = shape.mfma && mfma_supported(ker);
-  const int ti = mfma ? tpl_index(shape.tpl) : std::min(tpl_index(shape.tpl), max_tpl_index(ker)), tpl = TPLS[ti];
-  const int64_t tb = (nt + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
-  SCTL_CHECK_ARG(tb <= 0x7fffffffLL, "too many targets for one launch");
-  const int64_t ntiles = (ns + TILE - 1) / TILE;
-  SCTL_CHECK_ARG(ntiles <= 0x7fffffffLL, "too many sources for one launch");
-  const int Q = layout_q(ker);
-
-  Scratch sperm, tperm, packed, meta;
+  MixedGeometry g;
+  SCTL_TRY(mixed_geometry(ker, nt, ns, shape, &g));
+  Scratch sperm, tperm, packed, meta, part;
   SCTL_TRY(sperm.alloc(sizeof(uint32_t) * ns, stream));
   SCTL_TRY(tperm.alloc(sizeof(uint32_t) * nt, stream));
-  SCTL_TRY(morton_order(r_src, ns, sperm.as<uint32_t>(), stream));
-  SCTL_TRY(morton_order(r_trg, nt, tperm.as<uint32_t>(), stream));
-  SCTL_TRY(packed.alloc(sizeof(float4) * Q * TILE * ntiles, stream));
-  SCTL_TRY(meta.alloc(sizeof(double4) * ntiles, stream));
-  hipLaunchKernelGGL(pack_fn(ker), dim3((unsigned)ntiles), dim3(WG), 0, stream, r_src, n_src, v_src, ns,
-                     sperm.as<uint32_t>(), packed.as<float4>(), meta.as<double4>());
-  SCTL_HIP_TRY(hipGetLastError());
-
-  // Split the sources over gridDim.y when the target blocks cannot fill the
-  // device (>= 8 workgroups per CU, each split >= 4 tiles), and so that no
-  // split exceeds SPLIT_TILES tiles: workgroups are dispatched in order of
-  // their linear index (blockIdx.x fastest), so the ones resident at a time
-  // share one split's records (2 MiB for Laplace FxU) in their XCD's 4 MiB L2
-  // instead of drifting apart over the whole source array.
-  constexpr int64_t SPLIT_TILES = 256;
-  const int64_t want = 8LL * std::max(dev.num_cus, 1);
-  int64_t nsplit = tb < want ? (want + tb - 1) / tb : 1;
-  nsplit = std::min<int64_t>(std::min<int64_t>(nsplit, std::max<int64_t>(1, ntiles / 4)), 65535);
-  nsplit = std::min<int64_t>(std::max<int64_t>(nsplit, (ntiles + SPLIT_TILES - 1) / SPLIT_TILES), 65535);
-  const int64_t tps = (ntiles + nsplit - 1) / nsplit;
-  nsplit = (ntiles + tps - 1) / tps;
-
-  const double scale = kernel_scale(ker) * fold_factor(ker, newton ? 1 : 0);
-  const EvalMixFn fn = mix_fn(ker, newton, ti, mfma);
-  size_t dyn = 0;
-  SCTL_TRY(occupancy_lds((const void*)fn, shape.occ, sizeof(float4) * TILE * Q, &dyn));
-  Scratch part;
-  if (nsplit > 1) SCTL_TRY(part.alloc(sizeof(double) * nsplit * nt * K1, stream));
-  profile_count(ker, nt * ns);
-  {
-    ProfileScope prof(stream);
-    hipLaunchKernelGGL(fn, dim3((unsigned)tb, (unsigned)nsplit), dim3(WG), dyn, stream, r_trg, nt,
-                       tperm.as<uint32_t>(), packed.as<float4>(), meta.as<double4>(), ntiles, tps,
-                       nsplit > 1 ? part.as<double>() : v_trg, nsplit > 1 ? 0 : 1, scale, ctx);
-  }
-  SCTL_HIP_TRY(hipGetLastError());
-  if (nsplit > 1) {
-    const int64_t n = nt * K1;
-    hipLaunchKernelGGL(k_reduce_splits_perm, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream,
-                       part.as<double>(), nt, K1, (int)nsplit, tperm.as<uint32_t>(), v_trg, scale);
-    SCTL_HIP_TRY(hipGetLastError());
-  }
-  return SCTL_AMD_SUCCESS;
+  SCTL_TRY(mixed_order(r_trg, nt, r_src, ns, tperm.as<uint32_t>(), sperm.as<uint32_t>(), stream));
+  SCTL_TRY(packed.alloc(g.packed_bytes, stream));
+  SCTL_TRY(meta.alloc(g.meta_bytes, stream));
+  SCTL_TRY(part.alloc(g.part_bytes, stream));
+  return mixed_apply(ker, newton, g, shape, r_trg, nt, tperm.as<uint32_t>(), r_src, n_src, v_src, ns,
+                     sperm.as<uint32_t>(), packed.as<float4>(), meta.as<double4>(), part.as<double>(), v_trg, ctx,
+                     stream);
 }
 
 }  // namespace sctl_amd

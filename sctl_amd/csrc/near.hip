@@ -268,15 +268,11 @@ int tile_grid(int64_t ntiles) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)dev.num_cus * 64));
 }
 
-bool capturing(hipStream_t s) {
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
-}
 
 int near_matrix_launch(sctl_amd_plan_s* p, bool sub, const double* wts, const double* n_trg, double* K,
                        hipStream_t s) {
   if (p->mat_tiles.ntiles == 0) return SCTL_AMD_SUCCESS;
-  if (!capturing(s)) {
+  if (!stream_capturing(s)) {
     int64_t inter = 0;
     for (int64_t b = 0; b < p->nblocks; b++)
       if (p->near_off[b] >= 0) inter += p->trg_cnt[b] * p->src_cnt[b];

@@ -22,6 +22,11 @@ int64_t g_launches = 0;
 double g_ms = 0;
 }  // namespace
 
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 void profile_count(int ker, int64_t interactions) {
   g_inter += interactions;
   if (ker >= 0 && ker < N_REF_FLOPS) g_flops += interactions * REF_FLOPS[ker];
