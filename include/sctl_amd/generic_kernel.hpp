@@ -33,6 +33,10 @@
 // the mixed-precision form (FP64 differencing, FP32 pairwise, Kahan FP64
 // accumulation).  Real = float is evaluated by the mixed-precision form and
 // rounded to float.
+//
+// Profiling: Eval and KernelMatrix add Ns*Nt*FLOPS() to the reference
+// profiler's FLOP counter (generic-kernel.txx:210,224) when sctl/profile.hpp
+// (e.g. through <sctl.hpp>) is included before this header.
 #ifndef SCTL_AMD_GENERIC_KERNEL_HPP_
 #define SCTL_AMD_GENERIC_KERNEL_HPP_
 
@@ -64,6 +68,17 @@ namespace detail {
 
 inline void check(int status, const char* file, int line) {
   if (status != SCTL_AMD_SUCCESS) fail(sctl_amd_last_error(), file, line);
+}
+
+// Ns*Nt*FLOPS() into the reference profiler's FLOP counter, as its Eval and
+// KernelMatrix do (generic-kernel.txx:210,224), when that profiler is part of
+// the translation unit (sctl/profile.hpp included before this header).
+inline void count_flops(int64_t n) {
+#if defined(_SCTL_PROFILE_HPP_)
+  sctl::Profile::IncrementCounter(sctl::ProfileCounter::FLOP, (sctl::Long)n);
+#else
+  (void)n;
+#endif
 }
 #define SCTL_AMD_CHECK(expr) ::sctl_amd::detail::check((expr), __FILE__, __LINE__)
 
@@ -207,6 +222,7 @@ template <class uKernel> class GenericKernel : public uKernel {
       M.ReInit(Ns * KDIM0, Nt * KDIM1);
       M.SetZero();
     }
+    detail::count_flops(Ns * Nt * uKernel::FLOPS());
     if (!Ns || !Nt) return;
     SCTL_AMD_ASSERT(!N_DIM || Xn.Dim() >= Ns * N_DIM);
     std::vector<double> xt, xs, xn, m;
@@ -259,6 +275,7 @@ template <class uKernel> class GenericKernel : public uKernel {
       v_trg.ReInit(Nt * KDIM1);
       v_trg.SetZero();
     }
+    detail::count_flops(Ns * Nt * uKernel::FLOPS());
     if (!Ns || !Nt) return;
     const bool is_double = std::is_same<Real, double>::value;
     const int precision = is_double ? SCTL_AMD_AUTO : SCTL_AMD_MIXED;

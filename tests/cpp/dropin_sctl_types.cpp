@@ -3,6 +3,7 @@
 // (fmm-wrapper.txx:153-154).  Built against the reference headers where they
 // are present; never linked or run.
 #include "sctl/common.hpp"
+#include "sctl/profile.hpp"  // the drop-in then feeds the profiler's FLOP counter
 #include "sctl/iterator.hpp"
 #include "sctl/iterator.txx"
 #include "sctl/vector.hpp"
