@@ -62,6 +62,22 @@ __device__ __forceinline__ double shfl_xor_d(double v, int m) {
   return __shfl(v, lane ^ m, 64);
 }
 
+// Calls body(i) for the sources i = g, g + G, ... < STAGE of lane group g,
+// with the group count G a compile-time constant (the item's lane-group
+// count is wave-uniform): the unrolled iterations then address LDS with
+// immediate offsets from one base.
+template <int G, int STAGE, class Body> __device__ __forceinline__ void group_sources(int g, Body&& body) {
+#pragma unroll 4
+  for (int k = 0; k < STAGE / G; k++) body(g + k * G);
+}
+template <int STAGE, class Body> __device__ __forceinline__ void group_sources(int lg, int g, Body&& body) {
+  switch (lg) {
+    case 0: group_sources<1, STAGE>(g, body); break;
+    case 1: group_sources<2, STAGE>(g, body); break;
+    default: group_sources<4, STAGE>(g, body); break;
+  }
+}
+
 __global__ void k_reset_counter(int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
@@ -129,8 +145,7 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
       for (int j = 0; j < TPL; j++)
 #pragma unroll
         for (int k = 0; k < K1; k++) saved[j][k] = acc[j][k];
-#pragma unroll 2
-      for (int i = g; i < STAGE; i += G) {
+      group_sources<STAGE>(lg, g, [&](int i) {
         const double* rec = lds + i * R;
         double n[3] = {0, 0, 0}, f[K0];
 #pragma unroll
@@ -140,7 +155,7 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
 #pragma unroll
         for (int j = 0; j < TPL; j++)
           interact<KER, NEWTON, false>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f, mu);
-      }
+      });
       bool bad = false;
 #pragma unroll
       for (int j = 0; j < TPL; j++)
@@ -249,8 +264,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
       for (int j = 0; j < TPL; j++)
 #pragma unroll
         for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
-#pragma unroll 2
-      for (int i = g; i < STAGE; i += G) {
+      group_sources<STAGE>(lg, g, [&](int i) {
         const double* pr = pos + i * 4;
         const float* ar = aux + i * AW;
         float n[3] = {0, 0, 0}, f[K0];
@@ -262,7 +276,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
         for (int j = 0; j < TPL; j++)
           interact<KER, NEWTON, false>(p[j], (float)(tx[j] - pr[0]), (float)(ty[j] - pr[1]), (float)(tz[j] - pr[2]),
                                        n, f, mu);
-      }
+      });
       bool bad = false;
 #pragma unroll
       for (int j = 0; j < TPL; j++)
