@@ -16,9 +16,13 @@
 //  * The block's sources are staged through a wave-private LDS region,
 //    64*SPL records at a time (no workgroup barrier: waves run independently).
 //  * Mixed precision: each pair is differenced in FP64 and the difference
-//    rounded to FP32, the rest of the interaction is FP32; each stage's FP32
-//    partial is added into FP64 sums with Kahan compensation.  FP64: plain
-//    FP64 records, as in the dense path.
+//    rounded to FP32 (the Laplace single and double layers round r^2 and
+//    f n.r, formed in FP64, instead), the rest of the interaction is FP32;
+//    each stage's FP32 partial is added into FP64 sums with Kahan
+//    compensation.  FP64: plain FP64 records, as in the dense path.
+//  * A stage's source loop stops at the first multiple of 4 sources per
+//    lane group past its real sources, so a block's last, partial stage
+//    costs what it holds.
 //
 // The knobs (SPL: LDS stage size, TPL: targets per lane, a cap on resident
 // workgroups per CU) are chosen per plan by the tuner (tune.cpp) or defaults.
@@ -43,7 +47,14 @@ namespace sctl_amd {
 namespace {
 
 constexpr int BWG = 256;  // 4 waves per workgroup, each wave independent
-constexpr double PADD = 1e100;
+
+// Padding of a partial source stage: the slots past the block's last source
+// repeat the stage's first source with zero density and normal.  Every
+// kernel is linear in the density, so such a slot adds exactly 0 wherever
+// the first source's own pairs are finite; a far-away padding point would
+// instead overflow the FP32 differences of the mixed path (0 * inf = NaN,
+// sending the stage down the masked retry) and send Helmholtz phases down
+// the large-argument sincos branch.
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -62,19 +73,25 @@ __device__ __forceinline__ double shfl_xor_d(double v, int m) {
   return __shfl(v, lane ^ m, 64);
 }
 
-// Calls body(i) for the sources i = g, g + G, ... < STAGE of lane group g,
-// with the group count G a compile-time constant (the item's lane-group
-// count is wave-uniform): the unrolled iterations then address LDS with
-// immediate offsets from one base.
-template <int G, int STAGE, class Body> __device__ __forceinline__ void group_sources(int g, Body&& body) {
-#pragma unroll 4
-  for (int k = 0; k < STAGE / G; k++) body(g + k * G);
+// Calls body(i) for the sources i = g, g + G, ... of lane group g, in steps
+// of 4 per group, up to the first multiple of 4G at or past `len` (the
+// stage's real sources; the staged padding covers the rest, and STAGE is a
+// multiple of 4G, so every i < STAGE).  The group count G is a compile-time
+// constant (the item's lane-group count is wave-uniform): the 4 unrolled
+// calls address LDS with immediate offsets from one base.
+template <int G, int STAGE, class Body> __device__ __forceinline__ void group_sources(int g, int len, Body&& body) {
+  static_assert(STAGE % (4 * G) == 0, "the stage must hold whole steps");
+  const int steps = (len + 4 * G - 1) / (4 * G);
+  for (int k = 0; k < steps; k++) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) body(g + (4 * k + u) * G);
+  }
 }
-template <int STAGE, class Body> __device__ __forceinline__ void group_sources(int lg, int g, Body&& body) {
+template <int STAGE, class Body> __device__ __forceinline__ void group_sources(int lg, int g, int len, Body&& body) {
   switch (lg) {
-    case 0: group_sources<1, STAGE>(g, body); break;
-    case 1: group_sources<2, STAGE>(g, body); break;
-    default: group_sources<4, STAGE>(g, body); break;
+    case 0: group_sources<1, STAGE>(g, len, body); break;
+    case 1: group_sources<2, STAGE>(g, len, body); break;
+    default: group_sources<4, STAGE>(g, len, body); break;
   }
 }
 
@@ -124,20 +141,17 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
       for (int q = 0; q < SPL; q++) {
         const int i = lane + 64 * q;
         double* rec = lds + i * R;
-        if (i < len) {
-          const int64_t s = sb + s0 + i;
-          rec[0] = xs[s * 3 + 0];
-          rec[1] = xs[s * 3 + 1];
-          rec[2] = xs[s * 3 + 2];
+        const bool real = i < len;
+        const int64_t s = sb + s0 + (real ? i : 0);
+        rec[0] = xs[s * 3 + 0];
+        rec[1] = xs[s * 3 + 1];
+        rec[2] = xs[s * 3 + 2];
 #pragma unroll
-          for (int k = 0; k < ND; k++) rec[3 + k] = xn[s * 3 + k];
+        for (int k = 0; k < ND; k++) rec[3 + k] = real ? xn[s * 3 + k] : 0.0;
 #pragma unroll
-          for (int k = 0; k < K0; k++) rec[3 + ND + k] = fs[s * K0 + k];
-        } else {
-          rec[0] = rec[1] = rec[2] = PADD;
+        for (int k = 0; k < K0; k++) rec[3 + ND + k] = real ? fs[s * K0 + k] : 0.0;
 #pragma unroll
-          for (int k = 3; k < R; k++) rec[k] = 0.0;
-        }
+        for (int k = 3 + ND + K0; k < R; k++) rec[k] = 0.0;
       }
       wave_sync();
       double saved[TPL][K1];
@@ -145,7 +159,7 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
       for (int j = 0; j < TPL; j++)
 #pragma unroll
         for (int k = 0; k < K1; k++) saved[j][k] = acc[j][k];
-      group_sources<STAGE>(lg, g, [&](int i) {
+      group_sources<STAGE>(lg, g, len, [&](int i) {
         const double* rec = lds + i * R;
         double n[3] = {0, 0, 0}, f[K0];
 #pragma unroll
@@ -197,6 +211,31 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
 // interaction is FP32 with v_rsq_f32.  The stage's FP32 partial is added into
 // FP64 sums with Kahan compensation; a non-finite partial (r == 0 in the
 // stage) is redone with the r^2 > 0 mask.
+//
+// The Laplace single and double layers need only r^2 and f n.r of each pair,
+// and FP64 FMAs issue at the FP32 rate on CDNA4, so they form both in FP64
+// from the FP64 differences and round each once (f n is staged in FP64):
+// 6 (FxU) or 9 (DxU) FP64 operations and 1 or 2 conversions instead of 3
+// conversions and 3 or 6 FP32 operations, and r^2 correctly rounded.
+template <int KER> constexpr bool lap_r2d() { return KER == LAP_FXU || KER == LAP_DXU; }
+
+// One pair of the Laplace FP64-r^2 form: pr = {s, f n} (DxU) or {s, -} (FxU),
+// fs the FP32 density (FxU).  z carries the same power as in interact(), so
+// the deferred-Newton constant folds the same way.
+template <int KER, int NEWTON, bool MASK>
+__device__ __forceinline__ void lap_pair_r2d(float& p, double tx, double ty, double tz, const double* pr, float fs) {
+  const double dx = tx - pr[0], dy = ty - pr[1], dz = tz - pr[2];
+  const float r2 = (float)fmad(dz, dz, fmad(dy, dy, dx * dx));
+  float z = rsq_deferred<NEWTON>(r2);
+  if constexpr (MASK) z = r2 > 0.0f ? z : 0.0f;
+  if constexpr (KER == LAP_FXU) {
+    p = fmad(z, fs, p);
+  } else {
+    const float nd = (float)fmad(dz, pr[5], fmad(dy, pr[4], dx * pr[3]));
+    p = fmad(nd * z, z * z, p);
+  }
+}
+
 template <int KER, int NEWTON, int SPL, int TPL>
 __global__ void __launch_bounds__(BWG)
 k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ counter,
@@ -207,11 +246,13 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
   const float mu = (float)ctx;
   using K = Ker<KER>;
   constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, STAGE = 64 * SPL;
-  constexpr int AW = ((ND + K0) + 3) & ~3;  // float words of normal + density
-  __shared__ double pos_all[(BWG / 64) * STAGE * 4];
+  constexpr bool R2D = lap_r2d<KER>();
+  constexpr int PW = KER == LAP_DXU ? 6 : 4;                       // doubles per source: s (, f n)
+  constexpr int AW = KER == LAP_DXU ? 1 : ((ND + K0) + 3) & ~3;  // float words of normal + density
+  __shared__ double pos_all[(BWG / 64) * STAGE * PW];
   __shared__ float aux_all[(BWG / 64) * STAGE * AW];
   const int lane = threadIdx.x & 63;
-  double* pos = pos_all + (threadIdx.x >> 6) * (STAGE * 4);
+  double* pos = pos_all + (threadIdx.x >> 6) * (STAGE * PW);
   float* aux = aux_all + (threadIdx.x >> 6) * (STAGE * AW);
 
   for (int it = next_item(counter, lane); it < nitems; it = next_item(counter, lane)) {
@@ -241,21 +282,24 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
 #pragma unroll
       for (int q = 0; q < SPL; q++) {
         const int i = lane + 64 * q;
-        double* pr = pos + i * 4;
+        double* pr = pos + i * PW;
         float* ar = aux + i * AW;
-        if (i < len) {
-          const int64_t s = sb + s0 + i;
-          pr[0] = xs[s * 3 + 0];
-          pr[1] = xs[s * 3 + 1];
-          pr[2] = xs[s * 3 + 2];
+        const bool real = i < len;
+        const int64_t s = sb + s0 + (real ? i : 0);
+        pr[0] = xs[s * 3 + 0];
+        pr[1] = xs[s * 3 + 1];
+        pr[2] = xs[s * 3 + 2];
+        if constexpr (KER == LAP_DXU) {
+          const double f = real ? fs[s] : 0.0;
 #pragma unroll
-          for (int k = 0; k < ND; k++) ar[k] = (float)xn[s * 3 + k];
-#pragma unroll
-          for (int k = 0; k < K0; k++) ar[ND + k] = (float)fs[s * K0 + k];
+          for (int k = 0; k < 3; k++) pr[3 + k] = f * xn[s * 3 + k];
         } else {
-          pr[0] = pr[1] = pr[2] = PADD;
 #pragma unroll
-          for (int k = 0; k < AW; k++) ar[k] = 0.0f;
+          for (int k = 0; k < ND; k++) ar[k] = real ? (float)xn[s * 3 + k] : 0.0f;
+#pragma unroll
+          for (int k = 0; k < K0; k++) ar[ND + k] = real ? (float)fs[s * K0 + k] : 0.0f;
+#pragma unroll
+          for (int k = ND + K0; k < AW; k++) ar[k] = 0.0f;
         }
       }
       wave_sync();
@@ -264,18 +308,23 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
       for (int j = 0; j < TPL; j++)
 #pragma unroll
         for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
-      group_sources<STAGE>(lg, g, [&](int i) {
-        const double* pr = pos + i * 4;
+      group_sources<STAGE>(lg, g, len, [&](int i) {
+        const double* pr = pos + i * PW;
         const float* ar = aux + i * AW;
-        float n[3] = {0, 0, 0}, f[K0];
+        if constexpr (R2D) {
 #pragma unroll
-        for (int k = 0; k < ND; k++) n[k] = ar[k];
+          for (int j = 0; j < TPL; j++) lap_pair_r2d<KER, NEWTON, false>(p[j][0], tx[j], ty[j], tz[j], pr, KER == LAP_FXU ? ar[0] : 0.0f);
+        } else {
+          float n[3] = {0, 0, 0}, f[K0];
 #pragma unroll
-        for (int k = 0; k < K0; k++) f[k] = ar[ND + k];
+          for (int k = 0; k < ND; k++) n[k] = ar[k];
 #pragma unroll
-        for (int j = 0; j < TPL; j++)
-          interact<KER, NEWTON, false>(p[j], (float)(tx[j] - pr[0]), (float)(ty[j] - pr[1]), (float)(tz[j] - pr[2]),
-                                       n, f, mu);
+          for (int k = 0; k < K0; k++) f[k] = ar[ND + k];
+#pragma unroll
+          for (int j = 0; j < TPL; j++)
+            interact<KER, NEWTON, false>(p[j], (float)(tx[j] - pr[0]), (float)(ty[j] - pr[1]),
+                                         (float)(tz[j] - pr[2]), n, f, mu);
+        }
       });
       bool bad = false;
 #pragma unroll
@@ -288,15 +337,20 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
 #pragma unroll
           for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
         for (int i = g; i < len; i += G) {
-          const double* pr = pos + i * 4;
+          const double* pr = pos + i * PW;
           const float* ar = aux + i * AW;
-          float n[3] = {0, 0, 0}, f[K0];
-          for (int k = 0; k < ND; k++) n[k] = ar[k];
-          for (int k = 0; k < K0; k++) f[k] = ar[ND + k];
+          if constexpr (R2D) {
 #pragma unroll
-          for (int j = 0; j < TPL; j++)
-            interact<KER, NEWTON, true>(p[j], (float)(tx[j] - pr[0]), (float)(ty[j] - pr[1]), (float)(tz[j] - pr[2]),
-                                        n, f, mu);
+            for (int j = 0; j < TPL; j++) lap_pair_r2d<KER, NEWTON, true>(p[j][0], tx[j], ty[j], tz[j], pr, KER == LAP_FXU ? ar[0] : 0.0f);
+          } else {
+            float n[3] = {0, 0, 0}, f[K0];
+            for (int k = 0; k < ND; k++) n[k] = ar[k];
+            for (int k = 0; k < K0; k++) f[k] = ar[ND + k];
+#pragma unroll
+            for (int j = 0; j < TPL; j++)
+              interact<KER, NEWTON, true>(p[j], (float)(tx[j] - pr[0]), (float)(ty[j] - pr[1]),
+                                          (float)(tz[j] - pr[2]), n, f, mu);
+          }
         }
       }
 #pragma unroll
@@ -370,7 +424,10 @@ size_t blocks_lds(int ker, int precision, int spl) {
   int K0, K1, ND;
   kernel_dims(ker, &K0, &K1, &ND);
   const size_t stage = 64 * (size_t)spl * (BWG / 64);
-  if (precision == SCTL_AMD_MIXED) return stage * (4 * sizeof(double) + (((ND + K0) + 3) & ~3) * sizeof(float));
+  if (precision == SCTL_AMD_MIXED) {
+    if (ker == LAP_DXU) return stage * (6 * sizeof(double) + sizeof(float));  // s, f n; one unused float
+    return stage * (4 * sizeof(double) + (((ND + K0) + 3) & ~3) * sizeof(float));
+  }
   return stage * (((3 + ND + K0) + 1) & ~1) * sizeof(double);
 }
 }  // namespace
@@ -551,9 +608,13 @@ int sctl_amd_plan_create(sctl_amd_plan* plan, int kernel, int precision, int dig
   if (!rc) {
     PlanShape s;
     if (!lookup_plan_shape(kernel, precision, p->interactions, p->nonempty, &s)) {
-      s.spl = precision == SCTL_AMD_MIXED ? 2 : 1;
-      s.tpl = 1;
-      s.occ = 0;
+      // Untuned default, from sweeps of 4096- and 2048-block surface
+      // batches (Laplace and Stokes double layers) on MI355X: 256-source
+      // stages and two targets per lane; at most two workgroups per CU for
+      // mixed scalar outputs, four otherwise.
+      s.spl = 4;
+      s.tpl = 2;
+      s.occ = (precision == SCTL_AMD_MIXED && K1 == 1) ? 2 : 4;
     }
     rc = plan_configure(p, s);
   }

@@ -13,6 +13,7 @@
 // entry per line.  Lookups that miss fall back to size-based defaults.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -110,7 +111,8 @@ void store(const std::string& key, const std::vector<double>& v) {
 }
 
 // Milliseconds of one run of fn() on stream s; a run shorter than 20 ms is
-// repeated `reps` more times and the mean of those is returned instead.
+// repeated at least `reps` more times, and as often as fills about 20 ms (at
+// most 50), and the mean of those is returned instead.
 template <class F> int time_ms(F fn, hipStream_t s, int reps, double* ms) {
   hipEvent_t a, b;
   SCTL_HIP_TRY(hipEventCreate(&a));
@@ -118,7 +120,7 @@ template <class F> int time_ms(F fn, hipStream_t s, int reps, double* ms) {
   int rc = SCTL_AMD_SUCCESS;
   float t = 0;
   for (int pass = 0; pass < 2 && !rc; pass++) {
-    const int n = pass == 0 ? 1 : reps;
+    const int n = pass == 0 ? 1 : std::min(50, std::max(reps, (int)std::ceil(20.0 / std::max(t, 1e-3f))));
     if (hipEventRecord(a, s) != hipSuccess) rc = SCTL_AMD_ERROR_HIP;
     for (int r = 0; r < n && !rc; r++) rc = fn();
     if (!rc && hipEventRecord(b, s) != hipSuccess) rc = SCTL_AMD_ERROR_HIP;
@@ -144,6 +146,11 @@ int occupancy_lds(const void* kernel, int occ, size_t static_lds, size_t* dyn_by
   int per_cu = 0;
   SCTL_HIP_TRY(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
   if (per_cu <= 0) per_cu = 160 * 1024;
+  // The kernel's own static LDS as the runtime reports it; the caller's
+  // figure is a lower bound.
+  hipFuncAttributes attr;
+  SCTL_HIP_TRY(hipFuncGetAttributes(&attr, kernel));
+  static_lds = std::max(static_lds, (size_t)attr.sharedSizeBytes);
   size_t want = (size_t)per_cu / (size_t)occ;
   want -= want % 512;
   if (want <= static_lds + 512) return SCTL_AMD_SUCCESS;  // the kernel alone already fits no more than occ
@@ -314,7 +321,7 @@ int sctl_amd_plan_tune(sctl_amd_plan plan, const double* v_src, double* v_scratc
   PlanShape bs = plan_shape(plan);
   for (int spl : {1, 2, 4}) {
     for (int tpl : {1, 2}) {
-      for (int occ : {0, 4, 2}) {
+      for (int occ : {0, 4, 3, 2, 1}) {
         PlanShape sh;
         sh.spl = spl;
         sh.tpl = tpl;
