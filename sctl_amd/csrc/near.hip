@@ -30,8 +30,9 @@
 //                source per lane, 64 at a time, the target loop
 //                wave-uniform (targets broadcast from lanes 0..15), so every
 //                row segment written is 64*KDIM0 consecutive doubles;
-//   GEMV         16 rows: 16 lanes per row, 4 rows at a time, the row's
-//                reduction by cross-lane shuffles inside the 16-lane group.
+//   GEMV         16 rows, 8 at a time, the 64 lanes along the rows (64
+//                consecutive entries of a row per load), the 8 row sums
+//                reduced across the wave by a halving butterfly.
 // Nothing here allocates or issues a memset with device pointers, so the
 // calls can be captured in a hipGraph.
 #include <hip/hip_runtime.h>
@@ -52,7 +53,7 @@ namespace {
 constexpr int NWG = 256;        // 4 independent waves per workgroup
 constexpr int MAT_ST = 16;      // targets per near-matrix stripe (one wave)
 constexpr int GEMV_ROWS = 16;   // rows per GEMV wave tile
-constexpr int GEMV_LPR = 16;    // lanes per row in the GEMV
+constexpr int GEMV_GR = 8;      // GEMV rows walked together
 constexpr int NEAR_NEWTON = 2;  // KernelMatrix at digits -1 (boundary_integral.txx:1197,1211)
 
 __device__ __forceinline__ double readlane_d(double v, int l) {
@@ -145,18 +146,26 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
 // U_near[block rows] = K_near_ * F_element.  Blocks without a matrix
 // (offset < 0) or without sources get zero rows, as the reference's GEMV loop
 // writes them (boundary_integral.txx:1329-1332).  A wave owns GEMV_ROWS
-// consecutive rows (a contiguous range of K_near); 16 lanes per row, 8
-// independent loads per lane in flight, two partial sums.
+// consecutive rows (a contiguous range of K_near) and walks them GEMV_GR at a
+// time with its 64 lanes along the rows: every load is 64 consecutive
+// entries of one row, each lane's F entries serve all GEMV_GR rows, and two
+// column chunks per step keep 2*GEMV_GR loads in flight.  Rows past the
+// block's last (a tail tile) re-read its last row and columns past the row
+// end re-read its last entry, with F taken as 0, so every load stays inside
+// the block and no load is predicated.  The GEMV_GR row sums are reduced
+// across the wave by a halving butterfly: each exchange hands half of the
+// rows still held to the partner lane (4 + 2 + 1 + 3 exchanges for 8 rows).
 __global__ void __launch_bounds__(NWG)
 k_near_gemv(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ tile_dsp, int64_t ntiles,
             const int64_t* __restrict__ trg_dsp, const int32_t* __restrict__ trg_cnt,
             const int64_t* __restrict__ src_dsp, const int32_t* __restrict__ src_cnt,
             const int64_t* __restrict__ mat_off, int K0, int dof, const double* __restrict__ K,
             const double* __restrict__ F, double* __restrict__ U) {
+  static_assert(GEMV_GR == 8 && GEMV_ROWS % GEMV_GR == 0, "the butterfly below reduces 8 rows");
   const int lane = threadIdx.x & 63;
-  const int grp = lane / GEMV_LPR, li = lane % GEMV_LPR;
   const int64_t nwaves = (int64_t)gridDim.x * (NWG / 64);
-  for (int64_t tile = (int64_t)blockIdx.x * (NWG / 64) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
+  const int64_t first = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NWG / 64) + (threadIdx.x >> 6)));
+  for (int64_t tile = first; tile < ntiles; tile += nwaves) {
     const int b = __builtin_amdgcn_readfirstlane(tile_blk[tile]);
     const int64_t local = tile - tile_dsp[b];
     const int64_t rows = (int64_t)trg_cnt[b] * dof;
@@ -164,31 +173,48 @@ k_near_gemv(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ ti
     const int64_t off = mat_off[b];
     const double* Fb = F + src_dsp[b] * K0;
     double* Ub = U + trg_dsp[b] * dof;
+    for (int rr = 0; rr < GEMV_ROWS; rr += GEMV_GR) {
+      const int64_t row0 = local * GEMV_ROWS + rr;
+      if (row0 >= rows) break;
+      const int64_t nr = min((int64_t)GEMV_GR, rows - row0);
+      double acc[GEMV_GR];
 #pragma unroll
-    for (int rr = 0; rr < GEMV_ROWS; rr += 64 / GEMV_LPR) {
-      const int64_t row = local * GEMV_ROWS + rr + grp;
-      double acc0 = 0.0, acc1 = 0.0;
-      if (row < rows && off >= 0) {
-        const double* Kr = K + off + row * ld;
-        int64_t j = li;
-        for (; j + 7 * GEMV_LPR < ld; j += 8 * GEMV_LPR) {
-          double k[8], f[8];
+      for (int r = 0; r < GEMV_GR; r++) acc[r] = 0.0;
+      if (off >= 0) {
+        const double* Kr = K + off + row0 * ld;
+        for (int64_t j0 = 0; j0 < ld; j0 += 128) {
+          const int64_t ja = j0 + lane, jb = ja + 64;
+          const int64_t ca = min(ja, ld - 1), cb = min(jb, ld - 1);
+          const double fa = ja < ld ? Fb[ca] : 0.0, fb = jb < ld ? Fb[cb] : 0.0;
+          double ka[GEMV_GR], kb[GEMV_GR];
 #pragma unroll
-          for (int u = 0; u < 8; u++) k[u] = Kr[j + u * GEMV_LPR];
+          for (int r = 0; r < GEMV_GR; r++) {
+            const double* Kq = Kr + min((int64_t)r, nr - 1) * ld;
+            ka[r] = Kq[ca];
+            kb[r] = Kq[cb];
+          }
 #pragma unroll
-          for (int u = 0; u < 8; u++) f[u] = Fb[j + u * GEMV_LPR];
-#pragma unroll
-          for (int u = 0; u < 8; u += 2) {
-            acc0 = fmad(k[u], f[u], acc0);
-            acc1 = fmad(k[u + 1], f[u + 1], acc1);
+          for (int r = 0; r < GEMV_GR; r++) {
+            ka[r] = ja < ld ? ka[r] : 0.0;
+            kb[r] = jb < ld ? kb[r] : 0.0;
+            acc[r] = fmad(kb[r], fb, fmad(ka[r], fa, acc[r]));
           }
         }
-        for (; j < ld; j += GEMV_LPR) acc0 = fmad(Kr[j], Fb[j], acc0);
       }
-      double acc = acc0 + acc1;
 #pragma unroll
-      for (int m = GEMV_LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, 64);
-      if (li == 0 && row < rows) Ub[row] = acc;
+      for (int h = GEMV_GR / 2, m = 32; h >= 1; h >>= 1, m >>= 1) {
+        const bool up = lane & m;
+#pragma unroll
+        for (int i = 0; i < h; i++) {
+          const double send = up ? acc[i] : acc[i + h];
+          const double keep = up ? acc[i + h] : acc[i];
+          acc[i] = keep + __shfl_xor(send, m, 64);
+        }
+      }
+#pragma unroll
+      for (int m = 4; m >= 1; m >>= 1) acc[0] += __shfl_xor(acc[0], m, 64);
+      const int rl = lane >> 3;  // row of this lane's sum: bits 5, 4, 3 of the lane
+      if ((lane & 7) == 0 && rl < nr) Ub[row0 + rl] = acc[0];
     }
   }
 }
