@@ -69,6 +69,17 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // contiguous range of K_near: lanes 0..MAT_ST-1 hold the stripe's targets,
 // the wave walks the sources 64 at a time (one per lane) and, per chunk,
 // writes every target row's 64*KDIM0 entries.
+//
+// Stores and loads share one completion counter on this GPU, so a wait for
+// a chunk's source records would also wait for the previous chunk's stores.
+// The store mode therefore prefetches the next chunk's records before the
+// target loop and, for rows of at most 3 entries per source, issues a fixed
+// number of stores per chunk: MAT_ST target rows, the rows past the
+// stripe's last target repeating that target, and the lanes past the
+// block's last source repeating that source; a repeat writes the identical
+// value to the same entry, so the wait for the prefetched records can leave
+// the chunk's stores in flight.  The subtract mode cannot repeat an update
+// and keeps its stores exact, as do the wider kernels.
 template <int KER, bool NDOT, bool SUB>
 __global__ void __launch_bounds__(NWG)
 k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ tile_dsp, int64_t ntiles,
@@ -80,6 +91,9 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
   using Kt = Ker<KER>;
   constexpr int K0 = Kt::K0, K1 = Kt::K1, ND = Kt::ND;
   constexpr int DOF = NDOT ? K1 / 3 : K1;
+  // The pipelined store (16 unrolled target rows) for narrow rows only: the
+  // unrolled rows of wider kernels would cost their occupancy.
+  constexpr bool PIPE = !SUB && K0 * DOF <= 3;
   static_assert(!NDOT || K1 % 3 == 0, "the target-normal contraction needs 3 outputs per row");
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (NWG / 64);
@@ -88,7 +102,7 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
     const int t0 = (int)(tile - tile_dsp[b]) * MAT_ST;
     const int sc = src_cnt[b];
     const int nts = min(MAT_ST, trg_cnt[b] - t0);
-    const int64_t tg = trg_dsp[b] + t0 + (lane < nts ? lane : 0);
+    const int64_t tg = trg_dsp[b] + t0 + min(lane, nts - 1);
     const double mx = xt[tg * 3 + 0], my = xt[tg * 3 + 1], mz = xt[tg * 3 + 2];
     double mn[3] = {0, 0, 0};
     if constexpr (NDOT) {
@@ -97,48 +111,73 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
     }
     const int64_t ld = (int64_t)sc * K0;
     double* Ks = K + mat_off[b] + (int64_t)t0 * DOF * ld;
-    for (int s0 = 0; s0 < sc; s0 += 64) {
-      const int s = s0 + lane;
-      const bool active = s < sc;
-      const int64_t gs = src_dsp[b] + (active ? s : 0);
-      const double sx = xs[gs * 3 + 0], sy = xs[gs * 3 + 1], sz = xs[gs * 3 + 2];
-      double n[3] = {0, 0, 0};
+    const int64_t sb = src_dsp[b];
+    // This lane's source record of the chunk at s0: position, normal, weight
+    // (the column clamped to the block's last source).
+    auto load = [&](int s0, double(&r)[7]) {
+      const int64_t gs = sb + min(s0 + lane, sc - 1);
+      r[0] = xs[gs * 3 + 0];
+      r[1] = xs[gs * 3 + 1];
+      r[2] = xs[gs * 3 + 2];
 #pragma unroll
-      for (int k = 0; k < ND; k++) n[k] = xn[gs * 3 + k];
-      const double ws = wts ? wts[gs] : 1.0;
-      double* Kc = Ks + (int64_t)s * K0;
-      for (int t = 0; t < nts; t++) {
-        double u[K0][K1];
-        kmatrix<KER, NEAR_NEWTON>(u, readlane_d(mx, t) - sx, readlane_d(my, t) - sy, readlane_d(mz, t) - sz, n, mu);
-        if (!active) continue;
-        if constexpr (NDOT) {
-          const double n0 = readlane_d(mn[0], t), n1 = readlane_d(mn[1], t), n2 = readlane_d(mn[2], t);
+      for (int k = 0; k < 3; k++) r[3 + k] = k < ND ? xn[gs * 3 + k] : 0.0;
+      r[6] = wts ? wts[gs] : 1.0;
+    };
+    // Entries of target tt (wave-uniform) and this lane's source.
+    auto entries = [&](int tt, const double(&r)[7], double* Kc, bool active) {
+      double u[K0][K1];
+      const double n[3] = {r[3], r[4], r[5]};
+      kmatrix<KER, NEAR_NEWTON>(u, readlane_d(mx, tt) - r[0], readlane_d(my, tt) - r[1], readlane_d(mz, tt) - r[2],
+                                n, mu);
+      const double ws = r[6];
+      if constexpr (NDOT) {
+        const double n0 = readlane_d(mn[0], tt), n1 = readlane_d(mn[1], tt), n2 = readlane_d(mn[2], tt);
 #pragma unroll
-          for (int k1 = 0; k1 < DOF; k1++) {
-            double* row = Kc + (int64_t)(t * DOF + k1) * ld;
+        for (int k1 = 0; k1 < DOF; k1++) {
+          double* row = Kc + (int64_t)(tt * DOF + k1) * ld;
 #pragma unroll
-            for (int k0 = 0; k0 < K0; k0++) {
-              double v = 0.0;
-              v += (u[k0][k1 * 3 + 0] * scale) * ws * n0;
-              v += (u[k0][k1 * 3 + 1] * scale) * ws * n1;
-              v += (u[k0][k1 * 3 + 2] * scale) * ws * n2;
-              if constexpr (SUB) row[k0] -= v;
-              else row[k0] = v;
+          for (int k0 = 0; k0 < K0; k0++) {
+            double v = 0.0;
+            v += (u[k0][k1 * 3 + 0] * scale) * ws * n0;
+            v += (u[k0][k1 * 3 + 1] * scale) * ws * n1;
+            v += (u[k0][k1 * 3 + 2] * scale) * ws * n2;
+            if constexpr (SUB) {
+              if (active) row[k0] -= v;
+            } else {
+              if (active) row[k0] = v;
             }
           }
-        } else {
+        }
+      } else {
 #pragma unroll
-          for (int k1 = 0; k1 < K1; k1++) {
-            double* row = Kc + (int64_t)(t * K1 + k1) * ld;
+        for (int k1 = 0; k1 < K1; k1++) {
+          double* row = Kc + (int64_t)(tt * K1 + k1) * ld;
 #pragma unroll
-            for (int k0 = 0; k0 < K0; k0++) {
-              const double v = (u[k0][k1] * scale) * ws;
-              if constexpr (SUB) row[k0] -= v;
-              else row[k0] = v;
+          for (int k0 = 0; k0 < K0; k0++) {
+            const double v = (u[k0][k1] * scale) * ws;
+            if constexpr (SUB) {
+              if (active) row[k0] -= v;
+            } else {
+              if (active) row[k0] = v;
             }
           }
         }
       }
+    };
+    double cur[7], nxt[7];
+    load(0, cur);
+    for (int s0 = 0; s0 < sc; s0 += 64) {
+      double* Kc = Ks + (int64_t)min(s0 + lane, sc - 1) * K0;
+      load(s0 + 64, nxt);  // clamped: in range past the last chunk too
+      if constexpr (PIPE) {
+#pragma unroll
+        for (int t = 0; t < MAT_ST; t++) entries(min(t, nts - 1), cur, Kc, true);
+      } else {
+        const bool active = s0 + lane < sc;
+        for (int t = 0; t < nts; t++) entries(t, cur, Kc, active);
+      }
+#pragma unroll
+      for (int i = 0; i < 7; i++) cur[i] = nxt[i];
     }
   }
 }
