@@ -212,7 +212,9 @@ int sctl_amd_plan_destroy(sctl_amd_plan plan);
  * (64 * sources_per_lane), targets per lane (1 or 2; kernels with more than
  * 4 outputs per target always run with 1), and a cap on resident workgroups
  * of 4 waves per CU (0 = no cap).  Chosen at creation from the tuning
- * database, else defaults; sctl_amd_plan_tune searches them. */
+ * database, else defaults (256-source stages, 2 targets per lane, cap 2 for
+ * mixed precision with one output per target, else 4); sctl_amd_plan_tune
+ * searches them. */
 int sctl_amd_plan_get_shape(sctl_amd_plan plan, int* sources_per_lane, int* targets_per_lane, int* wg_per_cu_cap);
 int sctl_amd_plan_set_shape(sctl_amd_plan plan, int sources_per_lane, int targets_per_lane, int wg_per_cu_cap);
 
@@ -366,7 +368,7 @@ int sctl_amd_dense_shape_get(int kernel, int precision, int digits, uint32_t fla
 int sctl_amd_dense_shape_set(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
                              int targets_per_lane, int wg_per_cu_cap, int matrix_cores);
 
-/* Plan: sources per lane {1,2,4} x targets per lane {1,2} x cap {none,4,2};
+/* Plan: sources per lane {1,2,4} x targets per lane {1,2} x cap {none,4,3,2,1};
  * leaves the plan configured with the fastest.  v_scratch (nt_total*K1,
  * device) receives garbage. */
 int sctl_amd_plan_tune(sctl_amd_plan plan, const double* v_src, double* v_scratch, void* stream, int* best_spl,
