@@ -96,21 +96,25 @@ int newton_steps_f64(int digits) {
 }
 
 int device_info(DeviceInfo* out) {
-  int dev = 0;
-  int count = 0;
-  hipError_t e = hipGetDeviceCount(&count);
-  if (e != hipSuccess || count <= 0) {
+  // The current device's cached facts; the device count (a slow call) is
+  // asked only the first time a device is seen.
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) {
     set_error("no HIP device visible");
     return SCTL_AMD_ERROR_NO_DEVICE;
   }
-  SCTL_HIP_TRY(hipGetDevice(&dev));
-  if (dev < 0 || dev >= MAX_DEV) {
+  if (dev >= MAX_DEV) {
     set_error("device index out of range");
     return SCTL_AMD_ERROR_NO_DEVICE;
   }
   std::lock_guard<std::mutex> lock(g_dev_mutex);
   DeviceInfo& d = g_dev[dev];
   if (!d.ok) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+      set_error("no HIP device visible");
+      return SCTL_AMD_ERROR_NO_DEVICE;
+    }
     hipDeviceProp_t p;
     SCTL_HIP_TRY(hipGetDeviceProperties(&p, dev));
     d.device = dev;

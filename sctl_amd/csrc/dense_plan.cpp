@@ -141,7 +141,7 @@ int sctl_amd_dense_plan_create(sctl_amd_dense_plan* out, int kernel, int precisi
       if (!rc) rc = dev_alloc(reinterpret_cast<void**>(&p->d_sperm), sizeof(uint32_t) * ns);
       if (!rc) rc = dev_alloc(reinterpret_cast<void**>(&p->d_packed), p->geo.packed_bytes);
       if (!rc) rc = dev_alloc(reinterpret_cast<void**>(&p->d_meta), p->geo.meta_bytes);
-      if (!rc) rc = mixed_order(p->d_xt, nt, p->d_xs, ns, p->d_tperm, p->d_sperm, s);
+      if (!rc) rc = mixed_order(p->d_xt, nt, p->d_xs, ns, p->d_tperm, p->d_sperm, nullptr, s);
     }
     if (!rc) rc = dev_alloc(reinterpret_cast<void**>(&p->d_part), part);
   }

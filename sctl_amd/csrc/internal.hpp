@@ -77,6 +77,17 @@ class Scratch {
   hipStream_t stream_ = nullptr;
 };
 
+// Offsets of sub-buffers carved from one device allocation (256-byte aligned),
+// so that a call's scratch costs one stream-ordered allocation and free.
+struct Carve {
+  size_t bytes = 0;
+  size_t take(size_t n) {
+    const size_t off = (bytes + 255) & ~(size_t)255;
+    bytes = off + n;
+    return off;
+  }
+};
+
 // Launch-shape knobs of the dense kernels (chosen by the tuner or defaults).
 struct DenseShape {
   int tpl = 0;   // targets per lane: 1, 2 or 4 (0: default for the size)
@@ -117,8 +128,11 @@ struct MixedGeometry {
   size_t packed_bytes = 0, meta_bytes = 0, part_bytes = 0;
 };
 int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, MixedGeometry* g);
+// Morton orders of targets and sources; `work` (device) holds
+// mixed_order_bytes(nt, ns) bytes, or is null to allocate it per call.
+size_t mixed_order_bytes(int64_t nt, int64_t ns);
 int mixed_order(const double* r_trg, int64_t nt, const double* r_src, int64_t ns, uint32_t* tperm, uint32_t* sperm,
-                hipStream_t stream);
+                void* work, hipStream_t stream);
 int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& shape, const double* r_trg, int64_t nt,
                 const uint32_t* tperm, const double* r_src, const double* n_src, const double* v_src, int64_t ns,
                 const uint32_t* sperm, float4* packed, double4* meta, double* part, double* v_trg, double ctx,
@@ -127,8 +141,10 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
 bool stream_capturing(hipStream_t s);
 
 // Morton ordering of points (30-bit keys over the points' bounding box);
-// writes a permutation of [0, n).  Implemented in sort.hip.
-int morton_order(const double* pts, int64_t n, uint32_t* perm, hipStream_t stream);
+// writes a permutation of [0, n).  `work` (device) holds morton_order_bytes(n)
+// bytes.  Implemented in sort.hip.
+size_t morton_order_bytes(int64_t n);
+int morton_order(const double* pts, int64_t n, uint32_t* perm, void* work, hipStream_t stream);
 // Axis-aligned bounding box of two point sets (either may be empty):
 // box6 (device) = {min x, min y, min z, max x, max y, max z}.  sort.hip.
 int bounding_box(const double* a, int64_t na, const double* b, int64_t nb, double* box6, hipStream_t stream);

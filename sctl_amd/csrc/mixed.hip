@@ -571,10 +571,18 @@ int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, Mix
   return SCTL_AMD_SUCCESS;
 }
 
+size_t mixed_order_bytes(int64_t nt, int64_t ns) { return std::max(morton_order_bytes(nt), morton_order_bytes(ns)); }
+
+// The two orders run one after the other on the stream and share `work`.
 int mixed_order(const double* r_trg, int64_t nt, const double* r_src, int64_t ns, uint32_t* tperm, uint32_t* sperm,
-                hipStream_t stream) {
-  SCTL_TRY(morton_order(r_src, ns, sperm, stream));
-  SCTL_TRY(morton_order(r_trg, nt, tperm, stream));
+                void* work, hipStream_t stream) {
+  Scratch own;
+  if (!work) {
+    SCTL_TRY(own.alloc(mixed_order_bytes(nt, ns), stream));
+    work = own.as<void>();
+  }
+  SCTL_TRY(morton_order(r_src, ns, sperm, work, stream));
+  SCTL_TRY(morton_order(r_trg, nt, tperm, work, stream));
   return SCTL_AMD_SUCCESS;
 }
 
@@ -614,16 +622,21 @@ int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, cons
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
   MixedGeometry g;
   SCTL_TRY(mixed_geometry(ker, nt, ns, shape, &g));
-  Scratch sperm, tperm, packed, meta, part;
-  SCTL_TRY(sperm.alloc(sizeof(uint32_t) * ns, stream));
-  SCTL_TRY(tperm.alloc(sizeof(uint32_t) * nt, stream));
-  SCTL_TRY(mixed_order(r_trg, nt, r_src, ns, tperm.as<uint32_t>(), sperm.as<uint32_t>(), stream));
-  SCTL_TRY(packed.alloc(g.packed_bytes, stream));
-  SCTL_TRY(meta.alloc(g.meta_bytes, stream));
-  SCTL_TRY(part.alloc(g.part_bytes, stream));
-  return mixed_apply(ker, newton, g, shape, r_trg, nt, tperm.as<uint32_t>(), r_src, n_src, v_src, ns,
-                     sperm.as<uint32_t>(), packed.as<float4>(), meta.as<double4>(), part.as<double>(), v_trg, ctx,
-                     stream);
+  // One allocation for the call: orders, packed tiles, partial sums and the
+  // sorts' workspace.
+  Carve c;
+  const size_t o_sperm = c.take(sizeof(uint32_t) * ns), o_tperm = c.take(sizeof(uint32_t) * nt);
+  const size_t o_packed = c.take(g.packed_bytes), o_meta = c.take(g.meta_bytes), o_part = c.take(g.part_bytes);
+  const size_t o_work = c.take(mixed_order_bytes(nt, ns));
+  Scratch buf;
+  SCTL_TRY(buf.alloc(c.bytes, stream));
+  char* w = buf.as<char>();
+  uint32_t* sperm = reinterpret_cast<uint32_t*>(w + o_sperm);
+  uint32_t* tperm = reinterpret_cast<uint32_t*>(w + o_tperm);
+  SCTL_TRY(mixed_order(r_trg, nt, r_src, ns, tperm, sperm, w + o_work, stream));
+  return mixed_apply(ker, newton, g, shape, r_trg, nt, tperm, r_src, n_src, v_src, ns, sperm,
+                     reinterpret_cast<float4*>(w + o_packed), reinterpret_cast<double4*>(w + o_meta),
+                     reinterpret_cast<double*>(w + o_part), v_trg, ctx, stream);
 }
 
 }  // namespace sctl_amd

@@ -128,28 +128,49 @@ int bounding_box(const double* a, int64_t na, const double* b, int64_t nb, doubl
   return SCTL_AMD_SUCCESS;
 }
 
-int morton_order(const double* pts, int64_t n, uint32_t* perm, hipStream_t stream) {
+namespace {
+struct MortonLayout {
+  size_t part, keys_in, keys_out, idx, tmp, tmp_bytes, bytes;
+};
+MortonLayout morton_layout(int64_t n) {
+  MortonLayout m;
+  const int64_t nblk = (n + 8191) / 8192;
+  Carve c;
+  m.part = c.take(sizeof(double) * 6 * nblk);
+  m.keys_in = c.take(sizeof(uint32_t) * n);
+  m.keys_out = c.take(sizeof(uint32_t) * n);
+  m.idx = c.take(sizeof(uint32_t) * n);
+  m.tmp_bytes = 0;
+  (void)rocprim::radix_sort_pairs(nullptr, m.tmp_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  (uint32_t*)nullptr, (size_t)n, 0, 30);
+  m.tmp = c.take(m.tmp_bytes);
+  m.bytes = c.bytes;
+  return m;
+}
+}  // namespace
+
+size_t morton_order_bytes(int64_t n) { return n > 0 ? morton_layout(n).bytes : 0; }
+
+int morton_order(const double* pts, int64_t n, uint32_t* perm, void* work, hipStream_t stream) {
   if (n <= 0) return SCTL_AMD_SUCCESS;
+  SCTL_CHECK_ARG(work, "no workspace for the Morton order");
   const int64_t per_blk = 8192;
   const int nblk = (int)((n + per_blk - 1) / per_blk);
-  Scratch part, keys_in, keys_out, idx, tmp;
-  SCTL_TRY(part.alloc(sizeof(double) * 6 * nblk, stream));
-  SCTL_TRY(keys_in.alloc(sizeof(uint32_t) * n, stream));
-  SCTL_TRY(keys_out.alloc(sizeof(uint32_t) * n, stream));
-  SCTL_TRY(idx.alloc(sizeof(uint32_t) * n, stream));
-  hipLaunchKernelGGL(k_bbox_partial, dim3(nblk), dim3(BB_WG), 0, stream, pts, n, per_blk, part.as<double>());
+  const MortonLayout m = morton_layout(n);
+  char* w = static_cast<char*>(work);
+  double* part = reinterpret_cast<double*>(w + m.part);
+  uint32_t* keys_in = reinterpret_cast<uint32_t*>(w + m.keys_in);
+  uint32_t* keys_out = reinterpret_cast<uint32_t*>(w + m.keys_out);
+  uint32_t* idx = reinterpret_cast<uint32_t*>(w + m.idx);
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nblk), dim3(BB_WG), 0, stream, pts, n, per_blk, part);
   SCTL_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_WG), 0, stream, part.as<double>(), nblk);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_WG), 0, stream, part, nblk);
   SCTL_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_morton_keys, dim3((unsigned)((n + BB_WG - 1) / BB_WG)), dim3(BB_WG), 0, stream, pts, n,
-                     part.as<double>(), keys_in.as<uint32_t>(), idx.as<uint32_t>());
+  hipLaunchKernelGGL(k_morton_keys, dim3((unsigned)((n + BB_WG - 1) / BB_WG)), dim3(BB_WG), 0, stream, pts, n, part,
+                     keys_in, idx);
   SCTL_HIP_TRY(hipGetLastError());
-  size_t bytes = 0;
-  SCTL_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, keys_in.as<uint32_t>(), keys_out.as<uint32_t>(),
-                                         idx.as<uint32_t>(), perm, (size_t)n, 0, 30, stream));
-  SCTL_TRY(tmp.alloc(bytes, stream));
-  SCTL_HIP_TRY(rocprim::radix_sort_pairs(tmp.as<void>(), bytes, keys_in.as<uint32_t>(), keys_out.as<uint32_t>(),
-                                         idx.as<uint32_t>(), perm, (size_t)n, 0, 30, stream));
+  size_t bytes = m.tmp_bytes;
+  SCTL_HIP_TRY(rocprim::radix_sort_pairs(w + m.tmp, bytes, keys_in, keys_out, idx, perm, (size_t)n, 0, 30, stream));
   return SCTL_AMD_SUCCESS;
 }
 
