@@ -140,11 +140,12 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
 // True while `s` is being captured into a graph (no host-side counting then).
 bool stream_capturing(hipStream_t s);
 
-// Morton ordering of points (30-bit keys over the points' bounding box);
-// writes a permutation of [0, n).  `work` (device) holds morton_order_bytes(n)
-// bytes.  Implemented in sort.hip.
-size_t morton_order_bytes(int64_t n);
-int morton_order(const double* pts, int64_t n, uint32_t* perm, void* work, hipStream_t stream);
+// Morton orderings of two point sets (30-bit keys over each set's bounding
+// box) by one sort: permutations of [0, na) and [0, nb).  `work` (device)
+// holds morton_order_bytes(na, nb) bytes.  Implemented in sort.hip.
+size_t morton_order_bytes(int64_t na, int64_t nb);
+int morton_order(const double* a, int64_t na, uint32_t* perm_a, const double* b, int64_t nb, uint32_t* perm_b,
+                 void* work, hipStream_t stream);
 // Axis-aligned bounding box of two point sets (either may be empty):
 // box6 (device) = {min x, min y, min z, max x, max y, max z}.  sort.hip.
 int bounding_box(const double* a, int64_t na, const double* b, int64_t nb, double* box6, hipStream_t stream);

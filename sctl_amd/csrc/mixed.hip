@@ -571,9 +571,8 @@ int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, Mix
   return SCTL_AMD_SUCCESS;
 }
 
-size_t mixed_order_bytes(int64_t nt, int64_t ns) { return std::max(morton_order_bytes(nt), morton_order_bytes(ns)); }
+size_t mixed_order_bytes(int64_t nt, int64_t ns) { return morton_order_bytes(ns, nt); }
 
-// The two orders run one after the other on the stream and share `work`.
 int mixed_order(const double* r_trg, int64_t nt, const double* r_src, int64_t ns, uint32_t* tperm, uint32_t* sperm,
                 void* work, hipStream_t stream) {
   Scratch own;
@@ -581,9 +580,7 @@ int mixed_order(const double* r_trg, int64_t nt, const double* r_src, int64_t ns
     SCTL_TRY(own.alloc(mixed_order_bytes(nt, ns), stream));
     work = own.as<void>();
   }
-  SCTL_TRY(morton_order(r_src, ns, sperm, work, stream));
-  SCTL_TRY(morton_order(r_trg, nt, tperm, work, stream));
-  return SCTL_AMD_SUCCESS;
+  return morton_order(r_src, ns, sperm, r_trg, nt, tperm, work, stream);
 }
 
 int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& shape, const double* r_trg, int64_t nt,
