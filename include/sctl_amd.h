@@ -368,8 +368,9 @@ int sctl_amd_dense_shape_get(int kernel, int precision, int digits, uint32_t fla
 int sctl_amd_dense_shape_set(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
                              int targets_per_lane, int wg_per_cu_cap, int matrix_cores);
 
-/* Plan: sources per lane {1,2,4} x targets per lane {1,2} x cap {none,4,3,2,1};
- * leaves the plan configured with the fastest.  v_scratch (nt_total*K1,
+/* Plan: sources per lane {1,2,4} x targets per lane {1,2} x cap {none,4,3,2,1},
+ * each timed once, the three fastest timed again; leaves the plan configured
+ * with the fastest of that second pass.  v_scratch (nt_total*K1,
  * device) receives garbage. */
 int sctl_amd_plan_tune(sctl_amd_plan plan, const double* v_src, double* v_scratch, void* stream, int* best_spl,
                        int* best_tpl, int* best_occ, double* best_ms);

@@ -317,8 +317,17 @@ int sctl_amd_plan_tune(sctl_amd_plan plan, const double* v_src, double* v_scratc
   SCTL_CHECK_ARG(ns == 0 || v_src, "v_src is NULL");
   SCTL_CHECK_ARG(nt == 0 || v_scratch, "v_scratch is NULL");
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread;
-  double best = 1e300;
-  PlanShape bs = plan_shape(plan);
+  // Every candidate once (as configured: a kernel without two targets per
+  // lane runs one), then the three fastest again, the best of that second
+  // pass winning, so that a clock change during the search does not pick.
+  struct Cand {
+    double ms;
+    PlanShape sh;
+  };
+  std::vector<Cand> cands;
+  auto timed = [&](double* ms) {
+    return time_ms([&]() { return sctl_amd_plan_apply(plan, v_src, v_scratch, 0, s); }, s, 3, ms);
+  };
   for (int spl : {1, 2, 4}) {
     for (int tpl : {1, 2}) {
       for (int occ : {0, 4, 3, 2, 1}) {
@@ -327,13 +336,27 @@ int sctl_amd_plan_tune(sctl_amd_plan plan, const double* v_src, double* v_scratc
         sh.tpl = tpl;
         sh.occ = occ;
         SCTL_TRY(plan_configure(plan, sh));
+        const PlanShape act = plan_shape(plan);
+        bool seen = false;
+        for (const Cand& c : cands) seen |= c.sh.spl == act.spl && c.sh.tpl == act.tpl && c.sh.occ == act.occ;
+        if (seen) continue;
         double ms = 0;
-        SCTL_TRY(time_ms([&]() { return sctl_amd_plan_apply(plan, v_src, v_scratch, 0, s); }, s, 3, &ms));
-        if (ms < best) {
-          best = ms;
-          bs = sh;
-        }
+        SCTL_TRY(timed(&ms));
+        cands.push_back({ms, act});
       }
+    }
+  }
+  std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.ms < b.ms; });
+  const size_t top = std::min<size_t>(3, cands.size());
+  double best = 1e300;
+  PlanShape bs = cands[0].sh;
+  for (size_t i = 0; i < top; i++) {
+    SCTL_TRY(plan_configure(plan, cands[i].sh));
+    double ms = 0;
+    SCTL_TRY(timed(&ms));
+    if (ms < best) {
+      best = ms;
+      bs = cands[i].sh;
     }
   }
   SCTL_TRY(plan_configure(plan, bs));

@@ -74,7 +74,7 @@ def test_plan_ragged_and_empty_blocks(gpu, prec, tol):
     src_dsp = np.array([0, 100, 200, 0, 900, 1000, 1500], dtype=np.int64)
     src_cnt = np.array([100, 1, 700, 10, 33, 1000, 500], dtype=np.int64)
     B = dict(r_trg=xt, r_src=xs, n_src=n, trg_dsp=trg_dsp, trg_cnt=trg_cnt, src_dsp=src_dsp, src_cnt=src_cnt)
-    for k in (1, 2, 5, 8, 11, 13):
+    for k in (0, 1, 2, 5, 8, 11, 13):
         _, K0, K1, ND, _ = oracle.dims(k)
         v = rng.uniform(-1, 1, ns * K0)
         P = gpu.Plan(k, trg_dsp, trg_cnt, src_dsp, src_cnt, nt, ns, precision=prec, ctx=ctx_of(k))
