@@ -171,17 +171,18 @@ DenseShape pick_dense_shape(int ker, int precision, int newton, int64_t nt, int6
     s.mfma = v.size() >= 4 ? (int)v[3] : 0;
     return s;
   }
-  // FP64 on 2^20 or more targets: four targets per lane (the tuner's choice on
-  // 2^20 x 2^20).  Otherwise enough target blocks to fill the device with two
-  // targets per lane, else one target per lane and the source range split
+  // FP64 on 2^16 or more targets: four targets per lane (clamped per kernel;
+  // the tuner's choice on 2^20 x 2^20 and 7-16 % faster than one on 2^16-2^18
+  // cubes for the Laplace and Stokes single and double layers), except for
+  // the Helmholtz kernels, whose phase terms make four targets per lane up to
+  // a third slower.  Otherwise enough target blocks to fill the device with
+  // two targets per lane, else one target per lane and the source range split
   // over the grid.
-  if (precision == SCTL_AMD_FP64 && nt >= (1 << 20)) {
-    s.tpl = 4;
-    s.occ = 0;
-    return s;
-  }
-  s.tpl = nt >= 256LL * 2 * 2048 ? 2 : 1;
   s.occ = 0;
+  if (precision == SCTL_AMD_FP64)
+    s.tpl = (!kernel_needs_ctx(ker) && nt >= (1 << 16)) ? 4 : 1;
+  else
+    s.tpl = nt >= 256LL * 2 * 2048 ? 2 : 1;
   return s;
 }
 
