@@ -79,7 +79,10 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // block's last source repeating that source; a repeat writes the identical
 // value to the same entry, so the wait for the prefetched records can leave
 // the chunk's stores in flight.  The subtract mode cannot repeat an update
-// and keeps its stores exact, as do the wider kernels.
+// and keeps its stores exact, as do the wider kernels; on rows of one entry
+// per source it loads the old entries of 16 target rows before rewriting any
+// of them, where a row-by-row update would wait for the previous row's store
+// before each load.
 template <int KER, bool NDOT, bool SUB>
 __global__ void __launch_bounds__(NWG)
 k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ tile_dsp, int64_t ntiles,
@@ -94,6 +97,10 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
   // The pipelined store (16 unrolled target rows) for narrow rows only: the
   // unrolled rows of wider kernels would cost their occupancy.
   constexpr bool PIPE = !SUB && K0 * DOF <= 3;
+  // The grouped subtract for rows of one entry per source (W), GT rows per
+  // group; wider rows would cost their occupancy.
+  constexpr int W = K0 * DOF, GT = 16;
+  constexpr bool SUBPRE = SUB && W == 1;
   static_assert(!NDOT || K1 % 3 == 0, "the target-normal contraction needs 3 outputs per row");
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (NWG / 64);
@@ -124,7 +131,9 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
       r[6] = wts ? wts[gs] : 1.0;
     };
     // Entries of target tt (wave-uniform) and this lane's source.
-    auto entries = [&](int tt, const double(&r)[7], double* Kc, bool active) {
+    // old (subtract mode, optional): the entries' values loaded beforehand,
+    // indexed k1 * K0 + k0.
+    auto entries = [&](int tt, const double(&r)[7], double* Kc, bool active, const double* old) {
       double u[K0][K1];
       const double n[3] = {r[3], r[4], r[5]};
       kmatrix<KER, NEAR_NEWTON>(u, readlane_d(mx, tt) - r[0], readlane_d(my, tt) - r[1], readlane_d(mz, tt) - r[2],
@@ -142,7 +151,7 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
             v += (u[k0][k1 * 3 + 1] * scale) * ws * n1;
             v += (u[k0][k1 * 3 + 2] * scale) * ws * n2;
             if constexpr (SUB) {
-              if (active) row[k0] -= v;
+              if (active) row[k0] = (old ? old[k1 * K0 + k0] : row[k0]) - v;
             } else {
               if (active) row[k0] = v;
             }
@@ -156,7 +165,7 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
           for (int k0 = 0; k0 < K0; k0++) {
             const double v = (u[k0][k1] * scale) * ws;
             if constexpr (SUB) {
-              if (active) row[k0] -= v;
+              if (active) row[k0] = (old ? old[k1 * K0 + k0] : row[k0]) - v;
             } else {
               if (active) row[k0] = v;
             }
@@ -171,10 +180,29 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
       load(s0 + 64, nxt);  // clamped: in range past the last chunk too
       if constexpr (PIPE) {
 #pragma unroll
-        for (int t = 0; t < MAT_ST; t++) entries(min(t, nts - 1), cur, Kc, true);
+        for (int t = 0; t < MAT_ST; t++) entries(min(t, nts - 1), cur, Kc, true, nullptr);
+      } else if constexpr (SUBPRE) {
+        // The old entries of GT target rows are loaded together before any of
+        // them is rewritten (rows past the stripe's last target re-read it and
+        // are not rewritten), one memory round trip per group, not per row.
+        const bool active = s0 + lane < sc;
+        for (int g0 = 0; g0 < nts; g0 += GT) {
+          double old[GT][W];
+#pragma unroll
+          for (int t = 0; t < GT; t++) {
+            const int tt = min(g0 + t, nts - 1);
+#pragma unroll
+            for (int k1 = 0; k1 < DOF; k1++)
+#pragma unroll
+              for (int k0 = 0; k0 < K0; k0++) old[t][k1 * K0 + k0] = Kc[(int64_t)(tt * DOF + k1) * ld + k0];
+          }
+#pragma unroll
+          for (int t = 0; t < GT; t++)
+            if (g0 + t < nts) entries(g0 + t, cur, Kc, active, old[t]);
+        }
       } else {
         const bool active = s0 + lane < sc;
-        for (int t = 0; t < nts; t++) entries(t, cur, Kc, active);
+        for (int t = 0; t < nts; t++) entries(t, cur, Kc, active, nullptr);
       }
 #pragma unroll
       for (int i = 0; i < 7; i++) cur[i] = nxt[i];
