@@ -86,6 +86,21 @@ def timed_evals(kernel, dev, out, precision, digits, steps, warmup, stream):
     return dict(total_ms=ms, wall_s=wall, kernel_ms=prof["kernel_ms"], launches=prof["timed_launches"])
 
 
+def host_cpu(avx512):
+    """The host the CPU baseline ran on: model, logical CPUs, ISA class, OpenMP threads."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "isa": "avx512" if avx512 else "below avx512",
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(kernel, xt, xs, n, f, seconds):
     """The reference's own Eval<double,true> on a target subset (all sources)."""
     import oracle
@@ -115,7 +130,7 @@ def cpu_baseline(kernel, xt, xs, n, f, seconds):
     tmed = float(np.median(times))
     return {"value": round(m * ns / tmed / 1e9, 3), "unit": "G interactions/s", "cores": cores, "kind": kind,
             "sample": "%s: %d of the %d targets x all %d sources, median of 3 runs (%.2f s each)" % (
-                what, m, xt.size // 3, ns, tmed)}
+                what, m, xt.size // 3, ns, tmed), "host": host_cpu(oracle.have_avx512())}
 
 
 def time_once(fn, stream, reps=1):
