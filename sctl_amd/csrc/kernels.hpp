@@ -272,13 +272,12 @@ __device__ __forceinline__ void interact(T* acc, T rx, T ry, T rz, const T* n, c
     const T rdotn = fmad(rz, n[2], fmad(ry, n[1], rx * n[0]));
     const T z3 = z * z * z;
     acc[0] = fmad(rdotn * z3, f[0], acc[0]);
-  } else if constexpr (KER == STK_FXU) {
+  } else if constexpr (KER == STK_FXU) {  // z (f + r (f.r) z^2): 11 operations after z
     if constexpr (NEWTON > 0) z = z * (T)newton_c2(NEWTON);
-    const T z3 = z * z * z;
-    const T fr = fmad(f[2], rz, fmad(f[1], ry, f[0] * rx)) * z3;
-    acc[0] = fmad(f[0], z, fmad(rx, fr, acc[0]));
-    acc[1] = fmad(f[1], z, fmad(ry, fr, acc[1]));
-    acc[2] = fmad(f[2], z, fmad(rz, fr, acc[2]));
+    const T w = fmad(f[2], rz, fmad(f[1], ry, f[0] * rx)) * (z * z);
+    acc[0] = fmad(fmad(rx, w, f[0]), z, acc[0]);
+    acc[1] = fmad(fmad(ry, w, f[1]), z, acc[1]);
+    acc[2] = fmad(fmad(rz, w, f[2]), z, acc[2]);
   } else if constexpr (KER == STK_DXU) {
     const T z2 = z * z;
     const T z5 = z2 * z2 * z;
