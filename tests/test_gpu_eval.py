@@ -111,6 +111,41 @@ def test_eval_mixed_matches_oracle(gpu, k, newton, nt, ns, coincide):
     assert e_inf < MIXED_TOL and e_rms < MIXED_TOL
 
 
+@pytest.mark.parametrize("case", ["sources_one_point", "targets_one_point", "one_source", "one_target", "far_apart"])
+@pytest.mark.parametrize("k", [0, 1, 2, 3], ids=IDS[:4])
+def test_eval_degenerate_point_sets(gpu, k, case):
+    """Point sets on which the Morton ordering of the mixed path degenerates:
+    a set collapsed to one point (a zero-extent box, every key equal), a
+    single source or target, and targets in a tiny box far from the sources.
+    Both precisions against the oracle.  With random-sign densities these
+    sums cancel heavily (the far case sums 900 terms of ~1e-4 to ~5e-5), and
+    FP32 pairwise terms are accurate relative to their own size, so the
+    mixed leg is bounded relative to the sum of the terms' magnitudes,
+    |K|^T |f| from the oracle's kernel matrix; FP64 keeps the normwise bar."""
+    rng = np.random.default_rng(77)
+    _, K0, K1, ND, _ = oracle.dims(k)
+    nt = 1 if case == "one_target" else 700
+    ns = 1 if case == "one_source" else 900
+    xs, xt = rng.random(ns * 3), rng.random(nt * 3)
+    if case == "sources_one_point":
+        xs = np.tile([0.3, 0.4, 0.5], ns)
+    if case == "targets_one_point":
+        xt = np.tile([0.7, 0.1, 0.9], nt)
+    if case == "far_apart":
+        xt = xt * 1e-3 + 50.0
+    n = rng.standard_normal(ns * 3)
+    n = (n.reshape(-1, 3) / np.linalg.norm(n.reshape(-1, 3), axis=1)[:, None]).ravel()
+    v = rng.uniform(-1, 1, ns * K0)
+    ref = oracle.eval(k, xt, xs, n, v)
+    out = gpu.eval(k, xt, xs, n, v, precision=gpu.FP64)
+    assert np.isfinite(out).all() and nerr(out, ref) < FP64_TOL, (case, nerr(out, ref))
+    magnitude = np.abs(oracle.kernel_matrix(k, xt, xs, n)).T @ np.abs(v)
+    out = gpu.eval(k, xt, xs, n, v, precision=gpu.MIXED)
+    assert np.isfinite(out).all()
+    e = np.max(np.abs(out - ref)) / np.max(magnitude)
+    assert e < MIXED_TOL, (case, e)
+
+
 @pytest.mark.parametrize("k", KERNELS[2:], ids=IDS[2:])
 def test_eval_mixed_clusters(gpu, k):
     """Gaussian clusters (sigma 0.02): the case that needs tile centring."""
