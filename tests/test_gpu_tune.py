@@ -32,6 +32,24 @@ def test_every_plan_shape_matches_oracle(gpu, prec, tol):
                 assert W.rel_errors(out, ref)[0] < tol, (spl, tpl, cap)
 
 
+def test_plan_tune_reports_the_configured_shape(gpu, tmp_path):
+    """A kernel with nine outputs per target runs one target per lane whatever
+    is asked: the tuner reports (and stores) the shape it actually ran."""
+    gpu.tune_db_path(str(tmp_path / "tune9.db"))
+    try:
+        B = W.torus_blocks(8, 8, seed=1, k0=1)
+        nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
+        P = gpu.Plan(6, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=1)
+        P.set_geometry(B["r_trg"], B["r_src"], None)
+        v = gpu.DeviceArray.from_host(B["v_src"])
+        scratch = gpu.DeviceArray(nt * 9)
+        spl, tpl, cap, ms = P.tune(v, scratch)
+        assert tpl == 1 and ms > 0
+        assert P.shape() == (spl, tpl, cap)
+    finally:
+        gpu.tune_db_path(None)
+
+
 def test_plan_tune_picks_a_candidate(gpu, tmp_path):
     gpu.tune_db_path(str(tmp_path / "tune.db"))
     try:
