@@ -100,6 +100,17 @@ __global__ void k_reset_counter(int* counter) {
 }
 
 // ------------------------------------------------------------- FP64 blocks
+// The Laplace double layer with f n staged per source: z^3 (f n).r, one
+// multiplication per pair fewer than n.r z^3 f.
+template <int NEWTON, bool MASK>
+__device__ __forceinline__ void lap_dxu_fn(double& acc, double rx, double ry, double rz, const double* fn) {
+  const double r2 = fmad(rz, rz, fmad(ry, ry, rx * rx));
+  double z = rsq_deferred<NEWTON>(r2);
+  if constexpr (MASK) z = r2 > 0.0 ? z : 0.0;
+  const double nd = fmad(rz, fn[2], fmad(ry, fn[1], rx * fn[0]));
+  acc = fmad(nd, z * z * z, acc);
+}
+
 template <int KER, int NEWTON, int SPL, int TPL>
 __global__ void __launch_bounds__(BWG)
 k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ counter,
@@ -146,8 +157,9 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
         rec[0] = xs[s * 3 + 0];
         rec[1] = xs[s * 3 + 1];
         rec[2] = xs[s * 3 + 2];
+        const double fn = KER == LAP_DXU && real ? fs[s] : 1.0;  // f n staged for the double layer
 #pragma unroll
-        for (int k = 0; k < ND; k++) rec[3 + k] = real ? xn[s * 3 + k] : 0.0;
+        for (int k = 0; k < ND; k++) rec[3 + k] = real ? fn * xn[s * 3 + k] : 0.0;
 #pragma unroll
         for (int k = 0; k < K0; k++) rec[3 + ND + k] = real ? fs[s * K0 + k] : 0.0;
 #pragma unroll
@@ -161,14 +173,20 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
         for (int k = 0; k < K1; k++) saved[j][k] = acc[j][k];
       group_sources<STAGE>(lg, g, len, [&](int i) {
         const double* rec = lds + i * R;
-        double n[3] = {0, 0, 0}, f[K0];
+        if constexpr (KER == LAP_DXU) {
 #pragma unroll
-        for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
+          for (int j = 0; j < TPL; j++)
+            lap_dxu_fn<NEWTON, false>(acc[j][0], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], rec + 3);
+        } else {
+          double n[3] = {0, 0, 0}, f[K0];
 #pragma unroll
-        for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
+          for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
 #pragma unroll
-        for (int j = 0; j < TPL; j++)
-          interact<KER, NEWTON, false>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f, mu);
+          for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
+#pragma unroll
+          for (int j = 0; j < TPL; j++)
+            interact<KER, NEWTON, false>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f, mu);
+        }
       });
       bool bad = false;
 #pragma unroll
@@ -182,12 +200,18 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
           for (int k = 0; k < K1; k++) acc[j][k] = saved[j][k];
         for (int i = g; i < len; i += G) {
           const double* rec = lds + i * R;
-          double n[3] = {0, 0, 0}, f[K0];
-          for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
-          for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
+          if constexpr (KER == LAP_DXU) {
 #pragma unroll
-          for (int j = 0; j < TPL; j++)
-            interact<KER, NEWTON, true>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f, mu);
+            for (int j = 0; j < TPL; j++)
+              lap_dxu_fn<NEWTON, true>(acc[j][0], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], rec + 3);
+          } else {
+            double n[3] = {0, 0, 0}, f[K0];
+            for (int k = 0; k < ND; k++) n[k] = rec[3 + k];
+            for (int k = 0; k < K0; k++) f[k] = rec[3 + ND + k];
+#pragma unroll
+            for (int j = 0; j < TPL; j++)
+              interact<KER, NEWTON, true>(acc[j], tx[j] - rec[0], ty[j] - rec[1], tz[j] - rec[2], n, f, mu);
+          }
         }
       }
     }
