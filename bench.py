@@ -456,6 +456,11 @@ def main():
                     help="only the near-field operator and near-list lines of config 4 (for kernel traces)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the other micro-kernels' FP64 and mixed Eval lines (for kernel traces)")
+    ap.add_argument("--batched-only", choices=["mixed", "fp64"], default=None,
+                    help="only applies of the config-4 batched plan (4096 torus blocks, Laplace DxU) in this "
+                         "precision, at --plan-shape or the default shape, without tuning (for counters)")
+    ap.add_argument("--plan-shape", default=None, metavar="SPL,TPL,CAP",
+                    help="launch shape of the --batched-only plan (sources per lane, targets per lane, cap)")
     ap.add_argument("--shape", default=None, metavar="TPL,CAP,MC",
                     help="pin the mixed headline launch shape (targets per lane, workgroups-per-CU cap, "
                          "matrix cores 0/1) instead of tuning it")
@@ -474,6 +479,26 @@ def main():
         out = {"c4_near_matrix_laplace_dxu_4096": near_ops(S.LAPLACE3D_DXU, B, stream, reps=3),
                "c4_near_list_4096": near_list_build(B, 0.05, stream, cpu=False)}
         print(json.dumps({"near_only": True, "configs": out}), flush=True)
+        return
+
+    if args.batched_only:
+        stream = ctypes.c_void_p()
+        S.check(S.lib().sctl_amd_stream_create(ctypes.byref(stream)))
+        B = W.torus_blocks(64, 64, seed=0, k0=1)
+        nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
+        prec = S.MIXED if args.batched_only == "mixed" else S.FP64
+        P = S.Plan(S.LAPLACE3D_DXU, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=prec)
+        P.set_geometry(*[S.DeviceArray.from_host(B[a]) for a in ("r_trg", "r_src", "n_src")], stream=stream)
+        if args.plan_shape:
+            P.set_shape(*(int(x) for x in args.plan_shape.split(",")))
+        dv = S.DeviceArray.from_host(B["v_src"])
+        o = S.DeviceArray.from_host(np.zeros(nt))
+        log("config 4 batched plan, %s, shape %s: 3 applies" % (args.batched_only, P.shape()))
+        P.apply(dv, o, stream=stream)
+        S.synchronize(stream)
+        ms = time_once(lambda: P.apply(dv, o, stream=stream), stream, 2)
+        print(json.dumps({"batched_only": args.batched_only, "shape": list(P.shape()),
+                          "interactions_per_apply": int(P.interactions()), "ms_per_apply": round(ms, 4)}), flush=True)
         return
 
     if args.kernels_only:
