@@ -40,6 +40,7 @@
 #ifndef SCTL_AMD_GENERIC_KERNEL_HPP_
 #define SCTL_AMD_GENERIC_KERNEL_HPP_
 
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -199,12 +200,18 @@ template <class uKernel> class GenericKernel : public uKernel {
   const void* GetCtxPtr() const { return ctx_ptr; }
 
   // Type-erased form stored as a function pointer by ParticleFMM
-  // (generic-kernel.txx:72-100, fmm-wrapper.txx:153-154).
-  template <class Real, bool enable_openmp, class VectorT>
+  // (generic-kernel.txx:72-100, fmm-wrapper.txx:153-154,450-451).  The
+  // reference types `self` as sctl::ConstIterator<char>: `const char*` in
+  // default builds, a checked iterator class under SCTL_MEMDEBUG
+  // (common.hpp:110-118).  SelfT is deduced from the function-pointer type
+  // the caller assigns to, so `KerS2T::template Eval<Real,false>` binds in
+  // both builds.
+  template <class Real, bool enable_openmp, class VectorT, class SelfT>
   static void Eval(VectorT& v_trg, const VectorT& r_trg, const VectorT& r_src, const VectorT& n_src,
-                   const VectorT& v_src, Integer digits, const char* self) {
+                   const VectorT& v_src, Integer digits, SelfT self) {
     // self is the kernel object (generic-kernel.txx:75): its context travels along.
-    const void* ctx = self ? reinterpret_cast<const GenericKernel*>(self)->ctx_ptr : nullptr;
+    const char* p = address_of(self);
+    const void* ctx = p ? reinterpret_cast<const GenericKernel*>(p)->ctx_ptr : nullptr;
     EvalImpl<Real>(v_trg, r_trg, r_src, n_src, v_src, (digits >= 16 ? -1 : digits), ctx);
   }
 
@@ -246,6 +253,15 @@ template <class uKernel> class GenericKernel : public uKernel {
   }
 
  private:
+  // Address held by `self`: a plain pointer, or an iterator class whose
+  // operator-> yields it (sctl::ConstIterator under SCTL_MEMDEBUG,
+  // iterator.hpp:19-78).
+  template <class T> static const char* address_of(T* p) { return reinterpret_cast<const char*>(p); }
+  static const char* address_of(std::nullptr_t) { return nullptr; }
+  template <class It> static const char* address_of(const It& it) {
+    return reinterpret_cast<const char*>(it.operator->());
+  }
+
   // The kernels read their context as *(const Real*)ctx_ptr (kernel_functions.hpp:485-489);
   // the library takes it as a double.
   template <class Real> static const double* ctx_value(const void* ctx, double* tmp) {

@@ -1,7 +1,10 @@
-// Compile-only check: the drop-in accepts SCTL's own containers and its
-// static Eval has the type ParticleFMM stores as a function pointer
-// (fmm-wrapper.txx:153-154).  Built against the reference headers where they
-// are present; never linked or run.
+// Compile-only check: the drop-in accepts SCTL's own containers, and its
+// static Eval binds to the function pointers ParticleFMM stores
+// (fmm-wrapper.txx:153-154) through ParticleFMM's own assignment,
+// `data.ker_s2t_eval = KerS2T::template Eval<Real,false>;`
+// (fmm-wrapper.txx:450-451), in the default build and under SCTL_MEMDEBUG,
+// where sctl::ConstIterator<char> is a class (common.hpp:110-118).  Built
+// against the reference headers where they are present; never linked or run.
 #include "sctl/common.hpp"
 #include "sctl/profile.hpp"  // the drop-in then feeds the profiler's FLOP counter
 #include "sctl/iterator.hpp"
@@ -13,16 +16,29 @@
 
 #include "sctl_amd/generic_kernel.hpp"
 
-template <class Real>
-using EvalFn = void (*)(sctl::Vector<Real>& v_trg, const sctl::Vector<Real>& r_trg, const sctl::Vector<Real>& r_src,
-                        const sctl::Vector<Real>& n_src, const sctl::Vector<Real>& v_src, sctl::Integer digits,
-                        sctl::ConstIterator<char> self);
+// The members of ParticleFMM<Real,DIM>::S2TData that hold the kernel
+// (fmm-wrapper.txx:150-156), with their declared types.
+template <class Real> struct S2TData {
+  void (*ker_s2t_eval)(sctl::Vector<Real>& v_trg, const sctl::Vector<Real>& r_trg, const sctl::Vector<Real>& r_src,
+                       const sctl::Vector<Real>& n_src, const sctl::Vector<Real>& v_src, sctl::Integer digits,
+                       sctl::ConstIterator<char> self) = nullptr;
+  void (*ker_s2t_eval_omp)(sctl::Vector<Real>& v_trg, const sctl::Vector<Real>& r_trg,
+                           const sctl::Vector<Real>& r_src, const sctl::Vector<Real>& n_src,
+                           const sctl::Vector<Real>& v_src, sctl::Integer digits,
+                           sctl::ConstIterator<char> self) = nullptr;
+};
+
+template <class Real, class KerS2T> void add_kernel(const KerS2T& ker_s2t) {
+  S2TData<Real> data;
+  data.ker_s2t_eval = KerS2T::template Eval<Real,false>;
+  data.ker_s2t_eval_omp = KerS2T::template Eval<Real,true>;
+  (void)ker_s2t;
+  (void)data;
+}
 
 template <class KerS2T> void use(const KerS2T& ker) {
-  EvalFn<double> fn = KerS2T::template Eval<double, false, sctl::Vector<double>>;
-  EvalFn<double> fn_omp = KerS2T::template Eval<double, true, sctl::Vector<double>>;
-  (void)fn;
-  (void)fn_omp;
+  add_kernel<double>(ker);
+  add_kernel<float>(ker);
   sctl::Vector<double> vt, xt, xs, xn, vs;
   ker.template Eval<double, true>(vt, xt, xs, xn, vs);
   ker.template Eval<double, false, 7>(vt, xt, xs, xn, vs);
@@ -30,6 +46,15 @@ template <class KerS2T> void use(const KerS2T& ker) {
   ker.template KernelMatrix<double, false>(M, xt, xs, xn);
   sctl::Vector<float> ft, fx, fs, fn_, fv;
   ker.template Eval<float, false>(ft, fx, fs, fn_, fv);
+  // the call ParticleFMM makes through the stored pointer (fmm-wrapper.txx:618)
+  S2TData<double> data;
+  data.ker_s2t_eval = KerS2T::template Eval<double,false>;
+#ifdef SCTL_MEMDEBUG
+  sctl::ConstIterator<char> self((const char*)&ker, (sctl::Long)sizeof(ker));
+#else
+  sctl::ConstIterator<char> self = (const char*)&ker;
+#endif
+  data.ker_s2t_eval(vt, xt, xs, xn, vs, 10, self);
 }
 
 void instantiate() {
@@ -37,4 +62,5 @@ void instantiate() {
   use(sctl_amd::Laplace3D_DxU());
   use(sctl_amd::Stokes3D_FxU());
   use(sctl_amd::Stokes3D_DxU());
+  use(sctl_amd::Helmholtz3D_FxU());
 }
