@@ -46,15 +46,14 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X published FP64 vector peak
 FLOP_PER_INT = {0: 11, 1: 19, 2: 29, 3: 30}  # algorithmic flop per interaction
 
 
-TRAFFIC_FILES = {False: os.path.join(ROOT, "profiles", "traffic_k_eval_mixed_laplace_fxu_sphere_1M.json"),
-                 True: os.path.join(ROOT, "profiles", "traffic_k_eval_mixed_mfma_laplace_fxu_sphere_1M.json")}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_eval_mixed_laplace_fxu_sphere_1M.json")
 
 
-def measured_traffic(n, matrix_cores):
-    """HBM bytes per launch of the headline kernel (VALU or matrix-core far
-    path) from the committed FETCH_SIZE / WRITE_SIZE counter runs
-    (copy-calibrated), for the full-size workload only."""
-    path = TRAFFIC_FILES[bool(matrix_cores)]
+def measured_traffic(n):
+    """HBM bytes per launch of the headline kernel from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs (copy-calibrated), for the full-size
+    workload only."""
+    path = TRAFFIC_FILE
     if n != 1 << 20 or not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -461,9 +460,9 @@ def main():
                          "precision, at --plan-shape or the default shape, without tuning (for counters)")
     ap.add_argument("--plan-shape", default=None, metavar="SPL,TPL,CAP",
                     help="launch shape of the --batched-only plan (sources per lane, targets per lane, cap)")
-    ap.add_argument("--shape", default=None, metavar="TPL,CAP,MC",
-                    help="pin the mixed headline launch shape (targets per lane, workgroups-per-CU cap, "
-                         "matrix cores 0/1) instead of tuning it")
+    ap.add_argument("--shape", default=None, metavar="TPL,CAP",
+                    help="pin the mixed headline launch shape (targets per lane, workgroups-per-CU cap) "
+                         "instead of tuning it")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -528,20 +527,20 @@ def main():
         log("calibration copy: %d bytes device to device" % nb)
     tuned = {}
     if args.shape:
-        tpl, cap, mc = (int(x) for x in args.shape.split(","))
-        S.set_dense_shape(K, N, N, tpl, cap, mc, precision=S.MIXED)
-        tuned["mixed"] = {"targets_per_lane": tpl, "wg_per_cu_cap": cap, "matrix_cores": mc, "pinned": True}
+        tpl, cap = (int(x) for x in args.shape.split(","))
+        S.set_dense_shape(K, N, N, tpl, cap, precision=S.MIXED)
+        tuned["mixed"] = {"targets_per_lane": tpl, "wg_per_cu_cap": cap, "pinned": True}
     if not args.headline_only and not args.no_tune:
         log("autotuning the launch shapes (not timed)")
         for name, prec, dig in (("mixed", S.MIXED, -1), ("fp64", S.FP64, -1), ("fp64_digits14", S.FP64, 14)):
             if name in tuned:
                 continue
-            tpl, cap, ms, mc = S.tune_eval(K, dev[0], dev[1], None, dev[3], out, precision=prec, digits=dig,
-                                           stream=stream)
-            tuned[name] = {"targets_per_lane": tpl, "wg_per_cu_cap": cap, "matrix_cores": mc, "ms": round(ms, 3)}
+            tpl, cap, ms = S.tune_eval(K, dev[0], dev[1], None, dev[3], out, precision=prec, digits=dig,
+                                       stream=stream)
+            tuned[name] = {"targets_per_lane": tpl, "wg_per_cu_cap": cap, "ms": round(ms, 3)}
         out.upload(np.zeros(N))
     hs = S.dense_shape(K, N, N, precision=S.MIXED)
-    headline_kernel = "k_eval_mixed%s<Laplace3D_FxU, TPL %d>" % ("_mfma" if hs[2] else "", hs[0])
+    headline_kernel = "k_eval_mixed<Laplace3D_FxU, TPL %d>" % hs[0]
     log("mixed precision: %d warmup + %d timed steps (%s)" % (args.warmup, args.steps, headline_kernel))
     hp = S.FP64 if (args.headline_only and args.headline_precision == "fp64") else S.MIXED
     mx = timed_evals(K, dev, out, hp, -1, args.steps, args.warmup, stream)
@@ -619,7 +618,7 @@ def main():
         "speedup_vs_fp64_digits14": round(v_mx / v_64b, 3),
         "pct_fp32_valu_peak": round(100 * step_tflops / PEAK_FP32_TFLOPS, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(N, hs[2]),
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(N),
                      "kernel": headline_kernel, "mean_launch_ms": round(kern_ms, 3),
                      "flop_per_interaction": FLOP_PER_INT[K]},
         "autotuned_shapes": tuned or None,

@@ -36,7 +36,7 @@ extern "C" {
 typedef enum {
   SCTL_AMD_SUCCESS = 0,
   SCTL_AMD_ERROR_INVALID_VALUE = 1,  /* bad size, pointer, flag, digits or layout */
-  SCTL_AMD_ERROR_UNKNOWN_KERNEL = 2, /* kernel id outside the twelve micro-kernels */
+  SCTL_AMD_ERROR_UNKNOWN_KERNEL = 2, /* kernel id outside the sixteen micro-kernels */
   SCTL_AMD_ERROR_UNSUPPORTED = 3,    /* precision/option combination not provided */
   SCTL_AMD_ERROR_NO_DEVICE = 4,      /* no HIP device visible */
   SCTL_AMD_ERROR_OUT_OF_MEMORY = 5,  /* device allocation failed */
@@ -79,7 +79,6 @@ enum {
 /* flags */
 #define SCTL_AMD_HOST_PTRS 0x1u     /* array arguments are host memory; call is synchronous */
 #define SCTL_AMD_MIXED_NEWTON 0x2u  /* MIXED: one FP32 Newton step after v_rsq_f32 */
-#define SCTL_AMD_TUNE_MATRIX_CORES 0x4u /* sctl_amd_tune_eval: also time the matrix-core far path */
 
 /* ------------------------------------------------------------------ setup */
 
@@ -163,8 +162,7 @@ int sctl_amd_dense_plan_apply(sctl_amd_dense_plan plan, const double* v_src, dou
                               void* stream);
 
 /* The resolved precision and launch shape (any output may be NULL). */
-int sctl_amd_dense_plan_shape(sctl_amd_dense_plan plan, int* precision, int* targets_per_lane, int* wg_per_cu_cap,
-                              int* matrix_cores);
+int sctl_amd_dense_plan_shape(sctl_amd_dense_plan plan, int* precision, int* targets_per_lane, int* wg_per_cu_cap);
 
 int sctl_amd_dense_plan_destroy(sctl_amd_dense_plan plan);
 
@@ -324,7 +322,11 @@ int sctl_amd_near_list_destroy(sctl_amd_near_list list);
  * with N(0,1) draws of the counter-based generator for (seed + s, stream_id)
  * (as sctl_amd_gen_normal), then v_trg += apply(v_src).  v_src (ns_total*K0)
  * and v_trg (nt_total*K1) are device buffers that the graph keeps using on
- * every launch.  The graph allocates nothing and contains no memset. */
+ * every launch.  The graph allocates nothing and contains no memset.
+ * A graph bakes in its plan's launch shape, work items and context value:
+ * after sctl_amd_plan_set_shape, sctl_amd_plan_tune or sctl_amd_plan_set_ctx
+ * on that plan, sctl_amd_graph_launch refuses it (SCTL_AMD_ERROR_INVALID_VALUE)
+ * until it is captured again.  A graph must not outlive its plan or buffers. */
 typedef struct sctl_amd_graph_s* sctl_amd_graph;
 
 int sctl_amd_plan_graph_create(sctl_amd_graph* graph, sctl_amd_plan plan, int nsteps, uint64_t seed,
@@ -349,24 +351,21 @@ int sctl_amd_gen_normal(uint64_t seed, uint64_t stream_id, int64_t n, double* ou
  * sctl_amd_eval / sctl_amd_plan_create with a matching key use the result. */
 int sctl_amd_tune_db_path(const char* path);
 
-/* Dense Eval: targets per lane {1,2,4} x workgroups-per-CU cap {none,3,2} on
- * the VALU; with SCTL_AMD_TUNE_MATRIX_CORES in flags, for the mixed Laplace
- * kernels also r^2 of the far tiles on the matrix cores.  Device pointers
- * only; v_scratch (nt*K1) receives garbage. */
+/* Dense Eval: targets per lane {1,2,4} x workgroups-per-CU cap {none,3,2}.
+ * Device pointers only; v_scratch (nt*K1) receives garbage. */
 int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_trg, int64_t nt, const double* r_src,
                        const double* n_src, const double* v_src, int64_t ns, double* v_scratch, uint32_t flags,
-                       void* stream, int* best_tpl, int* best_occ, double* best_ms, int* best_matrix_cores);
+                       void* stream, int* best_tpl, int* best_occ, double* best_ms);
 
 /* The launch shape sctl_amd_eval uses for this (kernel, precision, digits,
  * flags, nt, ns) size class: get resolves it (tuning database, else
  * defaults); set records one in the database (targets_per_lane 1, 2 or 4,
- * capped at 2 for 4 outputs per target and at 1 for 9; wg_per_cu_cap >= 0;
- * matrix_cores 0/1, honoured by the mixed Laplace FxU / DxU kernels only).
+ * capped at 2 for 4 outputs per target and at 1 for 9; wg_per_cu_cap >= 0).
  * Host-only. */
 int sctl_amd_dense_shape_get(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
-                             int* targets_per_lane, int* wg_per_cu_cap, int* matrix_cores);
+                             int* targets_per_lane, int* wg_per_cu_cap);
 int sctl_amd_dense_shape_set(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
-                             int targets_per_lane, int wg_per_cu_cap, int matrix_cores);
+                             int targets_per_lane, int wg_per_cu_cap);
 
 /* Plan: sources per lane {1,2,4} x targets per lane {1,2} x cap {none,4,3,2,1},
  * each timed once, the three fastest timed again; leaves the plan configured

@@ -36,7 +36,7 @@ KERNEL_NAMES = {0: "Laplace3D_FxU", 1: "Laplace3D_DxU", 2: "Stokes3D_FxU", 3: "S
 CTX_KERNELS = frozenset(range(12, 16))
 NUM_KERNELS = len(KERNEL_NAMES)
 FP64, MIXED, AUTO = 0, 1, 2
-HOST_PTRS, MIXED_NEWTON, TUNE_MATRIX_CORES = 0x1, 0x2, 0x4
+HOST_PTRS, MIXED_NEWTON = 0x1, 0x2
 NEAR_SUBTRACT, NEAR_STORE = 0, 1
 
 STATUS = {0: "SUCCESS", 1: "INVALID_VALUE", 2: "UNKNOWN_KERNEL", 3: "UNSUPPORTED", 4: "NO_DEVICE",
@@ -95,7 +95,7 @@ def lib():
     L.sctl_amd_plan_set_ctx.argtypes = [_vp, _dp]
     L.sctl_amd_dense_plan_create.argtypes = [ctypes.POINTER(_vp), i32, i32, i32, u32, _vp, i64, _vp, _vp, i64, _dp, _vp]
     L.sctl_amd_dense_plan_apply.argtypes = [_vp, _vp, _vp, u32, _vp]
-    L.sctl_amd_dense_plan_shape.argtypes = [_vp] + [ctypes.POINTER(i32)] * 4
+    L.sctl_amd_dense_plan_shape.argtypes = [_vp] + [ctypes.POINTER(i32)] * 3
     L.sctl_amd_dense_plan_destroy.argtypes = [_vp]
     L.sctl_amd_dense_plan_graph_create.argtypes = [ctypes.POINTER(_vp), _vp, i32, ctypes.c_uint64, ctypes.c_uint64,
                                                    _vp, _vp]
@@ -119,9 +119,9 @@ def lib():
     L.sctl_amd_plan_get_shape.argtypes = [_vp, ip, ip, ip]
     L.sctl_amd_plan_set_shape.argtypes = [_vp, i32, i32, i32]
     L.sctl_amd_tune_db_path.argtypes = [ctypes.c_char_p]
-    L.sctl_amd_tune_eval.argtypes = [i32, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp, ip, ip, _dp, ip]
-    L.sctl_amd_dense_shape_get.argtypes = [i32, i32, i32, u32, i64, i64, ip, ip, ip]
-    L.sctl_amd_dense_shape_set.argtypes = [i32, i32, i32, u32, i64, i64, i32, i32, i32]
+    L.sctl_amd_tune_eval.argtypes = [i32, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp, ip, ip, _dp]
+    L.sctl_amd_dense_shape_get.argtypes = [i32, i32, i32, u32, i64, i64, ip, ip]
+    L.sctl_amd_dense_shape_set.argtypes = [i32, i32, i32, u32, i64, i64, i32, i32]
     L.sctl_amd_plan_tune.argtypes = [_vp, _vp, _vp, _vp, ip, ip, ip, _dp]
     L.sctl_amd_near_blocks.argtypes = [i64, _i64p, _i64p, i32, _i64p, _i64p, _i64p, _i64p]
     L.sctl_amd_plan_near_layout.argtypes = [_vp, _i64p, i32, _i64p]
@@ -481,8 +481,8 @@ class DensePlan:
         return v_trg
 
     def shape(self):
-        """(precision, targets_per_lane, wg_per_cu_cap, matrix_cores)."""
-        v = [ctypes.c_int() for _ in range(4)]
+        """(precision, targets_per_lane, wg_per_cu_cap)."""
+        v = [ctypes.c_int() for _ in range(3)]
         check(lib().sctl_amd_dense_plan_shape(self.handle, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
@@ -541,29 +541,27 @@ def tune_db_path(path):
 
 def tune_eval(kernel, r_trg, r_src, n_src, v_src, v_scratch, precision=FP64, digits=-1, flags=0, stream=None):
     """Autotunes the dense Eval shape for these device arrays; returns
-    (targets_per_lane, wg_per_cu_cap, ms, matrix_cores)."""
+    (targets_per_lane, wg_per_cu_cap, ms)."""
     nt, ns = r_trg.nbytes // 24, r_src.nbytes // 24
-    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    a, b = ctypes.c_int(), ctypes.c_int()
     ms = ctypes.c_double()
     check(lib().sctl_amd_tune_eval(kernel, precision, digits, r_trg.ptr, nt, r_src.ptr, _ptr(n_src), v_src.ptr, ns,
-                                   v_scratch.ptr, flags, stream, ctypes.byref(a), ctypes.byref(b), ctypes.byref(ms),
-                                   ctypes.byref(c)))
-    return a.value, b.value, ms.value, c.value
+                                   v_scratch.ptr, flags, stream, ctypes.byref(a), ctypes.byref(b), ctypes.byref(ms)))
+    return a.value, b.value, ms.value
 
 
 def dense_shape(kernel, nt, ns, precision=FP64, digits=-1, flags=0):
-    """(targets_per_lane, wg_per_cu_cap, matrix_cores) that Eval uses for this size class."""
-    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    """(targets_per_lane, wg_per_cu_cap) that Eval uses for this size class."""
+    a, b = ctypes.c_int(), ctypes.c_int()
     check(lib().sctl_amd_dense_shape_get(kernel, precision, digits, flags, int(nt), int(ns), ctypes.byref(a),
-                                         ctypes.byref(b), ctypes.byref(c)))
-    return a.value, b.value, c.value
+                                         ctypes.byref(b)))
+    return a.value, b.value
 
 
-def set_dense_shape(kernel, nt, ns, targets_per_lane, wg_per_cu_cap=0, matrix_cores=0, precision=FP64, digits=-1,
-                    flags=0):
+def set_dense_shape(kernel, nt, ns, targets_per_lane, wg_per_cu_cap=0, precision=FP64, digits=-1, flags=0):
     """Records the Eval launch shape for this size class in the tuning database."""
     check(lib().sctl_amd_dense_shape_set(kernel, precision, digits, flags, int(nt), int(ns), targets_per_lane,
-                                         wg_per_cu_cap, matrix_cores))
+                                         wg_per_cu_cap))
 
 
 def gen_normal(seed, stream_id, out, stream=None):

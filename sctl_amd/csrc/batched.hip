@@ -518,6 +518,7 @@ int plan_configure(sctl_amd_plan_s* p, PlanShape s) {
   p->grid = std::max(1, per_cu) * dev.num_cus;
   p->dyn_lds = dyn;
   p->shape = s;
+  p->generation++;
   return SCTL_AMD_SUCCESS;
 }
 
@@ -531,6 +532,7 @@ int plan_sizes(sctl_amd_plan p, int64_t* ns, int64_t* nt, int* K0, int* K1, int6
   return SCTL_AMD_SUCCESS;
 }
 int plan_kernel(sctl_amd_plan p) { return p ? p->kernel : -1; }
+uint64_t plan_generation(sctl_amd_plan p) { return p ? p->generation : 0; }
 int plan_precision(sctl_amd_plan p) { return p ? p->precision : -1; }
 int64_t plan_nonempty_blocks(sctl_amd_plan p) { return p ? p->nonempty : 0; }
 PlanShape plan_shape(sctl_amd_plan p) { return p ? p->shape : PlanShape(); }
@@ -678,6 +680,7 @@ int sctl_amd_plan_set_ctx(sctl_amd_plan p, const double* ctx) {
   SCTL_CHECK_ARG(!ctx || std::isfinite(*ctx), "the context value is not finite");
   p->has_ctx = ctx != nullptr;
   p->ctx = ctx ? *ctx : 0.0;
+  p->generation++;
   return SCTL_AMD_SUCCESS;
 }
 

@@ -173,6 +173,22 @@ int check_ctx(int kernel, const double* ctx) {
 double ctx_value(int kernel, const double* ctx) { return needs_ctx(kernel) && ctx ? *ctx : 0.0; }
 
 }  // namespace
+
+int eval_args(int kernel, const double* ctx, int* precision, int digits, const double* r_trg, int64_t nt,
+              const double* r_src, const double* n_src, const double* v_src, int64_t ns, const double* v_trg,
+              uint32_t flags, uint32_t allowed_flags) {
+  SCTL_TRY(check_common(kernel, r_trg, nt, r_src, n_src, ns, digits));
+  SCTL_TRY(check_ctx(kernel, ctx));
+  SCTL_CHECK_ARG(ns == 0 || v_src, "v_src is NULL");
+  SCTL_CHECK_ARG(nt == 0 || v_trg, "v_trg is NULL");
+  SCTL_CHECK_ARG((flags & ~allowed_flags) == 0, "unknown flag bits");
+  SCTL_CHECK_ARG(*precision >= SCTL_AMD_FP64 && *precision <= SCTL_AMD_AUTO, "unknown precision");
+  if (*precision == SCTL_AMD_AUTO) *precision = (digits >= 0 && digits <= 7) ? SCTL_AMD_MIXED : SCTL_AMD_FP64;
+  return SCTL_AMD_SUCCESS;
+}
+
+double eval_ctx_value(int kernel, const double* ctx) { return ctx_value(kernel, ctx); }
+
 }  // namespace sctl_amd
 
 using namespace sctl_amd;
@@ -222,13 +238,8 @@ int sctl_amd_eval(int kernel, int precision, int digits, const double* r_trg, in
 int sctl_amd_eval_ctx(int kernel, const double* ctx, int precision, int digits, const double* r_trg, int64_t nt,
                       const double* r_src, const double* n_src, const double* v_src, int64_t ns, double* v_trg,
                       uint32_t flags, void* stream) {
-  SCTL_TRY(check_common(kernel, r_trg, nt, r_src, n_src, ns, digits));
-  SCTL_TRY(check_ctx(kernel, ctx));
-  SCTL_CHECK_ARG(ns == 0 || v_src, "v_src is NULL");
-  SCTL_CHECK_ARG(nt == 0 || v_trg, "v_trg is NULL");
-  SCTL_CHECK_ARG((flags & ~(SCTL_AMD_HOST_PTRS | SCTL_AMD_MIXED_NEWTON)) == 0, "unknown flag bits");
-  SCTL_CHECK_ARG(precision >= SCTL_AMD_FP64 && precision <= SCTL_AMD_AUTO, "unknown precision");
-  if (precision == SCTL_AMD_AUTO) precision = (digits >= 0 && digits <= 7) ? SCTL_AMD_MIXED : SCTL_AMD_FP64;
+  SCTL_TRY(eval_args(kernel, ctx, &precision, digits, r_trg, nt, r_src, n_src, v_src, ns, v_trg, flags,
+                     SCTL_AMD_HOST_PTRS | SCTL_AMD_MIXED_NEWTON));
   DeviceInfo dev;
   SCTL_TRY(device_info(&dev));
   if (nt == 0 || ns == 0) return SCTL_AMD_SUCCESS;

@@ -226,8 +226,9 @@ void split_sources(int64_t target_blocks, int64_t ntiles, int num_cus, int64_t* 
 
 }  // namespace
 
-// Bytes of the per-split partial sums an FP64 Eval of this size and shape needs.
-int f64_part_bytes(int ker, int64_t nt, int64_t ns, const DenseShape& shape, size_t* bytes) {
+// Bytes of the per-split partial sums an FP64 Eval of this size and shape
+// needs, bounded by part_budget (split_group).
+int f64_part_bytes(int ker, int64_t nt, int64_t ns, const DenseShape& shape, size_t* bytes, size_t part_budget) {
   *bytes = 0;
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
   DeviceInfo dev;
@@ -238,13 +239,15 @@ int f64_part_bytes(int ker, int64_t nt, int64_t ns, const DenseShape& shape, siz
   const int64_t tb = (nt + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
   int64_t nsplit, tps;
   split_sources(tb, (ns + TILE - 1) / TILE, dev.num_cus, &nsplit, &tps);
-  *bytes = nsplit > 1 ? sizeof(double) * nsplit * nt * K1 : 0;
+  const size_t split_bytes = sizeof(double) * (size_t)nt * K1;
+  const int64_t group = split_group(nsplit, split_bytes, part_budget);
+  *bytes = group > 1 ? split_bytes * group : 0;
   return SCTL_AMD_SUCCESS;
 }
 
 int f64_apply(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src, const double* n_src,
               const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape, double ctx, double* part,
-              hipStream_t stream) {
+              hipStream_t stream, size_t part_budget) {
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
   DeviceInfo dev;
   SCTL_TRY(device_info(&dev));
@@ -256,24 +259,32 @@ int f64_apply(int ker, int newton, const double* r_trg, int64_t nt, const double
   const int64_t ntiles = (ns + TILE - 1) / TILE;
   int64_t nsplit, tps;
   split_sources(tb, ntiles, dev.num_cus, &nsplit, &tps);
-  SCTL_CHECK_ARG(nsplit == 1 || part, "no buffer for the split partial sums");
+  const int64_t group = split_group(nsplit, sizeof(double) * (size_t)nt * K1, part_budget);
+  SCTL_CHECK_ARG(group <= 1 || part, "no buffer for the split partial sums");
   const double scale = kernel_scale(ker) * fold_factor(ker, newton);
   const EvalF64Fn fn = f64_fn(ker, newton, ti);
   size_t dyn = 0;
   SCTL_TRY(occupancy_lds((const void*)fn, shape.occ, sizeof(double) * TILE * rec_words_f64(ker), &dyn));
-  dim3 grid((unsigned)tb, (unsigned)nsplit);
   if (!stream_capturing(stream)) profile_count(ker, nt * ns);
-  {
-    ProfileScope prof(stream);
-    hipLaunchKernelGGL(fn, grid, dim3(WG), dyn, stream, r_trg, nt, r_src, n_src, v_src, ns, tps * TILE,
-                       nsplit > 1 ? part : v_trg, nsplit > 1 ? 0 : 1, scale, ctx);
-  }
-  SCTL_HIP_TRY(hipGetLastError());
-  if (nsplit > 1) {
-    const int64_t n = nt * K1;
-    hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part, n,
-                       (int)nsplit, v_trg, scale);
+  const int64_t chunk = tps * TILE;
+  // Groups of `group` splits: sources [y0*chunk, (y0+gy)*chunk) per launch.
+  for (int64_t y0 = 0; y0 < nsplit; y0 += group) {
+    const int64_t gy = std::min(group, nsplit - y0);
+    const int64_t s0 = y0 * chunk, ns_g = std::min(ns - s0, gy * chunk);
+    const bool direct = gy == 1;
+    {
+      ProfileScope prof(stream);
+      hipLaunchKernelGGL(fn, dim3((unsigned)tb, (unsigned)gy), dim3(WG), dyn, stream, r_trg, nt, r_src + s0 * 3,
+                         ND ? n_src + s0 * 3 : n_src, v_src + s0 * K0, ns_g, chunk, direct ? v_trg : part,
+                         direct ? 1 : 0, scale, ctx);
+    }
     SCTL_HIP_TRY(hipGetLastError());
+    if (!direct) {
+      const int64_t n = nt * K1;
+      hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part, n,
+                         (int)gy, v_trg, scale);
+      SCTL_HIP_TRY(hipGetLastError());
+    }
   }
   return SCTL_AMD_SUCCESS;
 }

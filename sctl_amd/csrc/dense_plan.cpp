@@ -178,13 +178,11 @@ int sctl_amd_dense_plan_apply(sctl_amd_dense_plan p, const double* v_src, double
   return SCTL_AMD_SUCCESS;
 }
 
-int sctl_amd_dense_plan_shape(sctl_amd_dense_plan p, int* precision, int* targets_per_lane, int* wg_per_cu_cap,
-                              int* matrix_cores) {
+int sctl_amd_dense_plan_shape(sctl_amd_dense_plan p, int* precision, int* targets_per_lane, int* wg_per_cu_cap) {
   SCTL_CHECK_ARG(p, "plan is NULL");
   if (precision) *precision = p->precision;
   if (targets_per_lane) *targets_per_lane = p->shape.tpl;
   if (wg_per_cu_cap) *wg_per_cu_cap = p->shape.occ;
-  if (matrix_cores) *matrix_cores = p->shape.mfma;
   return SCTL_AMD_SUCCESS;
 }
 

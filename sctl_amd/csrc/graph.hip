@@ -66,6 +66,8 @@ struct sctl_amd_graph_s {
   hipGraphExec_t exec = nullptr;
   int nsteps = 0, kernel = 0;
   int64_t interactions = 0;
+  sctl_amd_plan plan = nullptr;  // batched plan captured (null for dense plans)
+  uint64_t generation = 0;       // its generation at capture
   ~sctl_amd_graph_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -145,8 +147,11 @@ int sctl_amd_plan_graph_create(sctl_amd_graph* out, sctl_amd_plan plan, int nste
   SCTL_TRY(plan_sizes(plan, &ns, &nt, &k0, &k1, &inter));
   SCTL_CHECK_ARG(ns == 0 || v_src, "v_src is NULL");
   SCTL_CHECK_ARG(nt == 0 || v_trg, "v_trg is NULL");
-  return capture_applies(out, plan_kernel(plan), ns * k0, inter, nsteps, seed, stream_id, v_src,
-                         [&](hipStream_t cap) { return sctl_amd_plan_apply(plan, v_src, v_trg, 0, cap); });
+  SCTL_TRY(capture_applies(out, plan_kernel(plan), ns * k0, inter, nsteps, seed, stream_id, v_src,
+                           [&](hipStream_t cap) { return sctl_amd_plan_apply(plan, v_src, v_trg, 0, cap); }));
+  (*out)->plan = plan;
+  (*out)->generation = plan_generation(plan);
+  return SCTL_AMD_SUCCESS;
 }
 
 int sctl_amd_dense_plan_graph_create(sctl_amd_graph* out, sctl_amd_dense_plan plan, int nsteps, uint64_t seed,
@@ -166,6 +171,9 @@ int sctl_amd_dense_plan_graph_create(sctl_amd_graph* out, sctl_amd_dense_plan pl
 
 int sctl_amd_graph_launch(sctl_amd_graph g, void* stream) {
   SCTL_CHECK_ARG(g && g->exec, "graph is NULL");
+  SCTL_CHECK_ARG(!g->plan || plan_generation(g->plan) == g->generation,
+                 "the plan's launch shape or context changed after this graph was captured "
+                 "(sctl_amd_plan_set_shape, sctl_amd_plan_tune, sctl_amd_plan_set_ctx): capture it again");
   SCTL_HIP_TRY(hipGraphLaunch(g->exec, stream ? static_cast<hipStream_t>(stream) : hipStreamPerThread));
   profile_count(g->kernel, g->interactions);
   return SCTL_AMD_SUCCESS;

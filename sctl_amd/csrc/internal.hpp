@@ -90,12 +90,23 @@ struct Carve {
 
 // Launch-shape knobs of the dense kernels (chosen by the tuner or defaults).
 struct DenseShape {
-  int tpl = 0;   // targets per lane: 1, 2 or 4 (0: default for the size)
-  int occ = 0;   // cap on resident workgroups per CU (0: no cap); enforced with dynamic LDS
-  int mfma = 0;  // mixed Laplace kernels: far tiles on the matrix cores
+  int tpl = 0;  // targets per lane: 1, 2 or 4 (0: default for the size)
+  int occ = 0;  // cap on resident workgroups per CU (0: no cap); enforced with dynamic LDS
 };
-// Kernels whose mixed-precision far path has a matrix-core form.
-inline bool mfma_supported(int ker) { return ker == 0 || ker == 1; }
+
+// Per-split partial sums of a dense Eval whose sources are split over
+// gridDim.y take nsplit * nt * K1 doubles when every split runs in one
+// launch.  Above `budget` bytes the splits run in groups of successive
+// launches over one group's worth of partials, each group reduced into
+// v_trg before the next; a group of one split accumulates straight into
+// v_trg and needs no partials at all.  Returns the splits per launch.
+constexpr size_t PART_BUDGET = (size_t)256 << 20;
+inline int64_t split_group(int64_t nsplit, size_t split_bytes, size_t budget) {
+  if (nsplit <= 1) return 1;
+  if (split_bytes == 0 || (size_t)nsplit * split_bytes <= budget) return nsplit;
+  const int64_t g = (int64_t)(budget / split_bytes);
+  return g < 2 ? 1 : g;
+}
 
 // Dynamic LDS bytes that cap a kernel with `static_lds` bytes of LDS at `occ`
 // resident workgroups per CU (0 when occ <= 0).  Sets the kernel's dynamic-LDS
@@ -117,17 +128,21 @@ int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, c
 // The dense Eval launchers in two parts, for callers that keep the geometry
 // (dense plans): a one-time part (sizes; for mixed precision the Morton
 // orders) and an apply that allocates nothing and issues no memset.
-int f64_part_bytes(int ker, int64_t nt, int64_t ns, const DenseShape& shape, size_t* bytes);
+// Both take the same part_budget (split_group); part holds f64_part_bytes.
+int f64_part_bytes(int ker, int64_t nt, int64_t ns, const DenseShape& shape, size_t* bytes,
+                   size_t part_budget = PART_BUDGET);
 int f64_apply(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src, const double* n_src,
               const double* v_src, int64_t ns, double* v_trg, const DenseShape& shape, double ctx, double* part,
-              hipStream_t stream);
+              hipStream_t stream, size_t part_budget = PART_BUDGET);
 struct MixedGeometry {
-  bool mfma = false;
   int ti = 0, Q = 0;
   int64_t tb = 0, ntiles = 0, nsplit = 1, tps = 0;
+  int64_t group = 1;  // splits per launch (split_group)
   size_t packed_bytes = 0, meta_bytes = 0, part_bytes = 0;
 };
-int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, MixedGeometry* g);
+// part_budget bounds part_bytes (split_group).
+int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, MixedGeometry* g,
+                   size_t part_budget = PART_BUDGET);
 // Morton orders of targets and sources; `work` (device) holds
 // mixed_order_bytes(nt, ns) bytes, or is null to allocate it per call.
 size_t mixed_order_bytes(int64_t nt, int64_t ns);
@@ -137,6 +152,13 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
                 const uint32_t* tperm, const double* r_src, const double* n_src, const double* v_src, int64_t ns,
                 const uint32_t* sperm, float4* packed, double4* meta, double* part, double* v_trg, double ctx,
                 hipStream_t stream);
+// A whole mixed Eval in a caller-provided workspace `work` of
+// mixed_fixed_bytes(g, nt, ns) + g.part_bytes bytes (256-byte aligned):
+// orders, packing, summation; allocates nothing.
+size_t mixed_fixed_bytes(const MixedGeometry& g, int64_t nt, int64_t ns);
+int mixed_eval_in(int ker, int newton, const MixedGeometry& g, const DenseShape& shape, const double* r_trg,
+                  int64_t nt, const double* r_src, const double* n_src, const double* v_src, int64_t ns,
+                  double* v_trg, double ctx, char* work, hipStream_t stream);
 // True while `s` is being captured into a graph (no host-side counting then).
 bool stream_capturing(hipStream_t s);
 
@@ -182,6 +204,16 @@ int dense_plan_apply_device(sctl_amd_dense_plan p, const double* v_src, double* 
 // Plan sizes (batched.hip): source/target totals, K0/K1, interactions per apply.
 int plan_sizes(sctl_amd_plan p, int64_t* ns, int64_t* nt, int* K0, int* K1, int64_t* interactions);
 int plan_kernel(sctl_amd_plan p);
+// Changes whenever the plan's launch state changes (shape, tuning, context).
+uint64_t plan_generation(sctl_amd_plan p);
+
+// Argument checks of an Eval call (capi.cpp); resolves SCTL_AMD_AUTO in
+// *precision.  Flag bits outside allowed_flags are an error.
+int eval_args(int kernel, const double* ctx, int* precision, int digits, const double* r_trg, int64_t nt,
+              const double* r_src, const double* n_src, const double* v_src, int64_t ns, const double* v_trg,
+              uint32_t flags, uint32_t allowed_flags);
+// The kernel's context value as the launchers take it (0 when unused).
+double eval_ctx_value(int kernel, const double* ctx);
 
 int kernel_dims(int ker, int* K0, int* K1, int* ND);
 // True for the kernels that need a context value (the Helmholtz wavenumber).

@@ -297,192 +297,6 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
   }
 }
 
-// Matrix-core far path for the Laplace kernels.  For a wave whose targets
-// are all far from a source tile, r^2 = |t'|^2 + |s'|^2 - 2 t'.s' of a
-// 16-target x 16-source block is one v_mfma_f32_16x16x4f32:
-//   A (16x4, targets) = [-2t'_0, -2t'_1, -2t'_2, 1],
-//   B (4x16, sources) = [s'_0, s'_1, s'_2, |s'|^2]   (the first four words of
-//                                                     the source record),
-//   C (16x16)         = |t'|^2 of the row's target,
-// and, for the double layer, r.(n q) = t'.(n q) - s'.(n q) is a second one
-// with A = [t'_0, t'_1, t'_2, 1] and B = [nq_0, nq_1, nq_2, -s'.(n q)].  The
-// VALU is left with the rsqrt and the density FMA (Laplace FxU) or rsqrt,
-// z^3 and FMA (Laplace DxU) per pair.  Operand layout of the instruction on
-// gfx950: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane
-// j + 16(i/4), element i%4.  The wave's 64*TPL targets form 4*TPL row tiles;
-// each lane keeps FP32 partials for its column and 4 rows per row tile, folds
-// them into FP64 after every source tile, and the 16 column partials of a
-// row are summed at the end.  Tiles with a near target (or a non-finite
-// partial) take the VALU near path of k_eval_mixed.
-typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-
-template <int KER, int TPL, int NEWTON>
-__global__ void __launch_bounds__(WG)
-k_eval_mixed_mfma(const double* __restrict__ xt, int64_t nt, const uint32_t* __restrict__ tperm,
-                  const float4* __restrict__ packed, const double4* __restrict__ meta, int64_t ntiles,
-                  int64_t tiles_per_split, double* __restrict__ out, int direct, double scale, double ctx) {
-  static_assert(KER == LAP_FXU || KER == LAP_DXU, "matrix-core far path: Laplace kernels only");
-  constexpr int Q = MixLayout<KER>::Q, R = 4 * Q, NR = 4 * TPL;
-  static_assert(sizeof(float4) * TILE * Q >= sizeof(double) * 16 * NR * (WG / 64), "LDS too small for the fold");
-  __shared__ float4 lds[TILE * Q];
-
-  const int tid = threadIdx.x, lane = tid & 63, col = lane & 15, grp = lane >> 4;
-  const int64_t tile_begin = (int64_t)blockIdx.y * tiles_per_split;
-  const int64_t tile_end = tile_begin + tiles_per_split < ntiles ? tile_begin + tiles_per_split : ntiles;
-  const int64_t q0 = (int64_t)blockIdx.x * (WG * TPL) + tid;
-
-  double t64[TPL][3], acc[TPL], comp[TPL], accf[NR][4];
-  int64_t tidx[TPL];
-#pragma unroll
-  for (int j = 0; j < TPL; j++) {
-    const int64_t q = q0 + (int64_t)j * WG;
-    const bool ok = q < nt;
-    tidx[j] = ok ? (int64_t)tperm[q] : -1;
-#pragma unroll
-    for (int k = 0; k < 3; k++) t64[j][k] = ok ? xt[tidx[j] * 3 + k] : 0.0;
-    acc[j] = comp[j] = 0.0;
-  }
-#pragma unroll
-  for (int rt = 0; rt < NR; rt++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) accf[rt][r] = 0.0;
-
-  for (int64_t tile = tile_begin; tile < tile_end; tile++) {
-    __syncthreads();
-    const float4* src = packed + tile * (TILE * Q);
-#pragma unroll
-    for (int q = 0; q < Q; q++) lds[q * WG + tid] = src[q * WG + tid];
-    const double4 m = meta[tile];
-    __syncthreads();
-
-    const float near_r2 = 9.0f * (float)m.w;
-    float th[TPL][3], T[TPL];
-    bool near = false;
-#pragma unroll
-    for (int j = 0; j < TPL; j++) {
-      th[j][0] = (float)(t64[j][0] - m.x);
-      th[j][1] = (float)(t64[j][1] - m.y);
-      th[j][2] = (float)(t64[j][2] - m.z);
-      T[j] = fmad(th[j][2], th[j][2], fmad(th[j][1], th[j][1], th[j][0] * th[j][0]));
-      near |= T[j] < near_r2;
-    }
-    const float* recs = reinterpret_cast<const float*>(lds);
-    bool use_near = __builtin_amdgcn_ballot_w64(near) != 0;
-    if (!use_near) {
-      float A[NR], A2[NR];
-      mfma_f4 C[NR], p[NR];
-#pragma unroll
-      for (int j = 0; j < TPL; j++)
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-          const int rt = 4 * j + a, src_lane = 16 * a + col;
-          const float x0 = __shfl(th[j][0], src_lane, 64), x1 = __shfl(th[j][1], src_lane, 64);
-          const float x2 = __shfl(th[j][2], src_lane, 64);
-          const float tk = grp == 0 ? x0 : (grp == 1 ? x1 : x2);
-          A[rt] = grp < 3 ? -2.0f * tk : 1.0f;
-          A2[rt] = grp < 3 ? tk : 1.0f;
-#pragma unroll
-          for (int r = 0; r < 4; r++) C[rt][r] = __shfl(T[j], 16 * a + 4 * grp + r, 64);
-          p[rt] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
-        }
-      const mfma_f4 zero4 = {0.0f, 0.0f, 0.0f, 0.0f};
-      for (int s0 = 0; s0 < TILE; s0 += 16) {
-        const float* rec = recs + (s0 + col) * R;
-        const float b = rec[grp];
-        // All row tiles' products first (independent, back to back on the
-        // matrix pipe), then the VALU work on their results.
-        mfma_f4 d[NR];
-#pragma unroll
-        for (int rt = 0; rt < NR; rt++) d[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[rt], b, C[rt], 0, 0, 0);
-        if constexpr (KER == LAP_FXU) {
-          const float qd = rec[4];
-#pragma unroll
-          for (int rt = 0; rt < NR; rt++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) p[rt][r] = fmad(rsq_deferred<NEWTON>(d[rt][r]), qd, p[rt][r]);
-        } else {
-          const float b2 = rec[4 + grp];
-          mfma_f4 e[NR];
-#pragma unroll
-          for (int rt = 0; rt < NR; rt++) e[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A2[rt], b2, zero4, 0, 0, 0);
-#pragma unroll
-          for (int rt = 0; rt < NR; rt++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-              const float z = rsq_deferred<NEWTON>(d[rt][r]);
-              p[rt][r] = fmad(e[rt][r], z * z * z, p[rt][r]);
-            }
-        }
-      }
-      bool bad = false;
-#pragma unroll
-      for (int rt = 0; rt < NR; rt++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) bad |= !__builtin_isfinite(p[rt][r]);
-      use_near = __builtin_amdgcn_ballot_w64(bad) != 0;
-      if (!use_near) {
-#pragma unroll
-        for (int rt = 0; rt < NR; rt++)
-#pragma unroll
-          for (int r = 0; r < 4; r++) accf[rt][r] += (double)p[rt][r];
-      }
-    }
-    if (use_near) {
-      float tl[TPL][3], pn[TPL][1];
-#pragma unroll
-      for (int j = 0; j < TPL; j++) {
-        tl[j][0] = (float)((t64[j][0] - m.x) - (double)th[j][0]);
-        tl[j][1] = (float)((t64[j][1] - m.y) - (double)th[j][1]);
-        tl[j][2] = (float)((t64[j][2] - m.z) - (double)th[j][2]);
-        pn[j][0] = 0.0f;
-      }
-#pragma unroll 2
-      for (int i = 0; i < TILE; i++) {
-        const float* rec = recs + i * R;
-#pragma unroll
-        for (int j = 0; j < TPL; j++) mix_near<KER, NEWTON>(pn[j], th[j], tl[j], rec, 0.0f);
-      }
-#pragma unroll
-      for (int j = 0; j < TPL; j++) {  // Kahan: acc += pn
-        const double y = (double)pn[j][0] - comp[j];
-        const double s = acc[j] + y;
-        comp[j] = (s - acc[j]) - y;
-        acc[j] = s;
-      }
-    }
-  }
-
-  // Sum the 16 column partials of every row, then hand each target its sum
-  // through LDS (the row tiles' layout is not the targets' lane layout).
-#pragma unroll
-  for (int rt = 0; rt < NR; rt++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      double v = accf[rt][r];
-#pragma unroll
-      for (int mk = 1; mk < 16; mk <<= 1) v += __shfl_xor(v, mk, 64);
-      accf[rt][r] = v;
-    }
-  __syncthreads();
-  double* fs = reinterpret_cast<double*>(lds) + (tid >> 6) * (16 * NR);
-  if (col == 0) {
-#pragma unroll
-    for (int rt = 0; rt < NR; rt++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) fs[rt * 16 + 4 * grp + r] = accf[rt][r];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < TPL; j++) {
-    const int64_t q = q0 + (int64_t)j * WG;
-    if (q < nt) {
-      const double v = (acc[j] - comp[j]) + fs[(4 * j + grp) * 16 + col];
-      if (direct) out[tidx[j]] += v * scale;
-      else out[(int64_t)blockIdx.y * nt + q] = v;
-    }
-  }
-}
-
 // v_trg[tperm[q]*K1+k] += scale * sum_y part[y][q*K1+k], in split order.
 __global__ void __launch_bounds__(WG)
 k_reduce_splits_perm(const double* __restrict__ part, int64_t nt, int K1, int nsplit, const uint32_t* __restrict__ tperm,
@@ -509,19 +323,9 @@ template <int KER, int NEWTON> EvalMixFn mix_fn(int ti) {
   if (ti >= 1 && MT >= 2) return k_eval_mixed<KER, (MT >= 2 ? 2 : 1), NEWTON>;
   return k_eval_mixed<KER, 1, NEWTON>;
 }
-template <int KER, int NEWTON> EvalMixFn mfma_fn(int ti) {
-  switch (ti) {
-    case 0: return k_eval_mixed_mfma<KER, 1, NEWTON>;
-    case 1: return k_eval_mixed_mfma<KER, 2, NEWTON>;
-    default: return k_eval_mixed_mfma<KER, 4, NEWTON>;
-  }
-}
-EvalMixFn mix_fn(int ker, int newton, int ti, bool mfma) {
+EvalMixFn mix_fn(int ker, int newton, int ti) {
   return with_kernel(ker, [&](auto kc) -> EvalMixFn {
     constexpr int K = decltype(kc)::value;
-    if constexpr (K == LAP_FXU || K == LAP_DXU) {
-      if (mfma) return newton ? mfma_fn<K, 1>(ti) : mfma_fn<K, 0>(ti);
-    }
     return newton ? mix_fn<K, 1>(ti) : mix_fn<K, 0>(ti);
   });
 }
@@ -543,13 +347,13 @@ int max_tpl_index(int ker) {
 // split exceeds SPLIT_TILES tiles: workgroups are dispatched in order of
 // their linear index (blockIdx.x fastest), so the ones resident at a time
 // share one split's records (2 MiB for Laplace FxU) in their XCD's 4 MiB L2
-// instead of drifting apart over the whole source array.
-int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, MixedGeometry* g) {
+// instead of drifting apart over the whole source array.  The splits' partial
+// sums stay within part_budget bytes (split_group).
+int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, MixedGeometry* g, size_t part_budget) {
   DeviceInfo dev;
   SCTL_TRY(device_info(&dev));
   SCTL_CHECK_ARG(ns <= 0xffffffffLL && nt <= 0xffffffffLL, "mixed precision supports at most 2^32-1 points");
-  g->mfma = shape.mfma && mfma_supported(ker);
-  g->ti = g->mfma ? tpl_index(shape.tpl) : std::min(tpl_index(shape.tpl), max_tpl_index(ker));
+  g->ti = std::min(tpl_index(shape.tpl), max_tpl_index(ker));
   const int tpl = TPLS[g->ti];
   g->tb = (nt + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
   SCTL_CHECK_ARG(g->tb <= 0x7fffffffLL, "too many targets for one launch");
@@ -567,7 +371,9 @@ int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, Mix
   kernel_dims(ker, &K0, &K1, &ND);
   g->packed_bytes = sizeof(float4) * g->Q * TILE * g->ntiles;
   g->meta_bytes = sizeof(double4) * g->ntiles;
-  g->part_bytes = g->nsplit > 1 ? sizeof(double) * g->nsplit * nt * K1 : 0;
+  const size_t split_bytes = sizeof(double) * (size_t)nt * K1;
+  g->group = split_group(g->nsplit, split_bytes, part_budget);
+  g->part_bytes = g->group > 1 ? split_bytes * g->group : 0;
   return SCTL_AMD_SUCCESS;
 }
 
@@ -588,29 +394,72 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
                 const uint32_t* sperm, float4* packed, double4* meta, double* part, double* v_trg, double ctx,
                 hipStream_t stream) {
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
+  SCTL_CHECK_ARG(g.group <= 1 || part, "no buffer for the split partial sums");
   int K0, K1, ND;
   kernel_dims(ker, &K0, &K1, &ND);
   hipLaunchKernelGGL(pack_fn(ker), dim3((unsigned)g.ntiles), dim3(WG), 0, stream, r_src, n_src, v_src, ns, sperm,
                      packed, meta);
   SCTL_HIP_TRY(hipGetLastError());
   const double scale = kernel_scale(ker) * fold_factor(ker, newton ? 1 : 0);
-  const EvalMixFn fn = mix_fn(ker, newton, g.ti, g.mfma);
+  const EvalMixFn fn = mix_fn(ker, newton, g.ti);
   size_t dyn = 0;
   SCTL_TRY(occupancy_lds((const void*)fn, shape.occ, sizeof(float4) * TILE * g.Q, &dyn));
   if (!stream_capturing(stream)) profile_count(ker, nt * ns);
-  {
-    ProfileScope prof(stream);
-    hipLaunchKernelGGL(fn, dim3((unsigned)g.tb, (unsigned)g.nsplit), dim3(WG), dyn, stream, r_trg, nt, tperm, packed,
-                       meta, g.ntiles, g.tps, g.nsplit > 1 ? part : v_trg, g.nsplit > 1 ? 0 : 1, scale, ctx);
-  }
-  SCTL_HIP_TRY(hipGetLastError());
-  if (g.nsplit > 1) {
-    const int64_t n = nt * K1;
-    hipLaunchKernelGGL(k_reduce_splits_perm, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part, nt, K1,
-                       (int)g.nsplit, tperm, v_trg, scale);
+  // Groups of g.group splits: tiles [y0*tps, (y0+gy)*tps) per launch.
+  for (int64_t y0 = 0; y0 < g.nsplit; y0 += g.group) {
+    const int64_t gy = std::min(g.group, g.nsplit - y0);
+    const int64_t tile0 = y0 * g.tps, ntiles = std::min(g.ntiles - tile0, gy * g.tps);
+    const bool direct = gy == 1;
+    {
+      ProfileScope prof(stream);
+      hipLaunchKernelGGL(fn, dim3((unsigned)g.tb, (unsigned)gy), dim3(WG), dyn, stream, r_trg, nt, tperm,
+                         packed + tile0 * (int64_t)(TILE * g.Q), meta + tile0, ntiles, g.tps, direct ? v_trg : part,
+                         direct ? 1 : 0, scale, ctx);
+    }
     SCTL_HIP_TRY(hipGetLastError());
+    if (!direct) {
+      const int64_t n = nt * K1;
+      hipLaunchKernelGGL(k_reduce_splits_perm, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part, nt, K1,
+                         (int)gy, tperm, v_trg, scale);
+      SCTL_HIP_TRY(hipGetLastError());
+    }
   }
   return SCTL_AMD_SUCCESS;
+}
+
+namespace {
+// Workspace of one mixed Eval: orders, packed tiles, tile centres and the
+// sorts' workspace, then the split partial sums last.
+struct MixedCarve {
+  size_t sperm, tperm, packed, meta, work, part, fixed_bytes;
+};
+MixedCarve mixed_carve(const MixedGeometry& g, int64_t nt, int64_t ns) {
+  MixedCarve m;
+  Carve c;
+  m.sperm = c.take(sizeof(uint32_t) * ns);
+  m.tperm = c.take(sizeof(uint32_t) * nt);
+  m.packed = c.take(g.packed_bytes);
+  m.meta = c.take(g.meta_bytes);
+  m.work = c.take(mixed_order_bytes(nt, ns));
+  m.part = c.take(0);
+  m.fixed_bytes = m.part;
+  return m;
+}
+}  // namespace
+
+size_t mixed_fixed_bytes(const MixedGeometry& g, int64_t nt, int64_t ns) { return mixed_carve(g, nt, ns).fixed_bytes; }
+
+int mixed_eval_in(int ker, int newton, const MixedGeometry& g, const DenseShape& shape, const double* r_trg,
+                  int64_t nt, const double* r_src, const double* n_src, const double* v_src, int64_t ns,
+                  double* v_trg, double ctx, char* w, hipStream_t stream) {
+  if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
+  const MixedCarve m = mixed_carve(g, nt, ns);
+  uint32_t* sperm = reinterpret_cast<uint32_t*>(w + m.sperm);
+  uint32_t* tperm = reinterpret_cast<uint32_t*>(w + m.tperm);
+  SCTL_TRY(mixed_order(r_trg, nt, r_src, ns, tperm, sperm, w + m.work, stream));
+  return mixed_apply(ker, newton, g, shape, r_trg, nt, tperm, r_src, n_src, v_src, ns, sperm,
+                     reinterpret_cast<float4*>(w + m.packed), reinterpret_cast<double4*>(w + m.meta),
+                     g.part_bytes ? reinterpret_cast<double*>(w + m.part) : nullptr, v_trg, ctx, stream);
 }
 
 int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, const double* r_src,
@@ -619,21 +468,12 @@ int launch_eval_mixed(int ker, int newton, const double* r_trg, int64_t nt, cons
   if (nt <= 0 || ns <= 0) return SCTL_AMD_SUCCESS;
   MixedGeometry g;
   SCTL_TRY(mixed_geometry(ker, nt, ns, shape, &g));
-  // One allocation for the call: orders, packed tiles, partial sums and the
-  // sorts' workspace.
-  Carve c;
-  const size_t o_sperm = c.take(sizeof(uint32_t) * ns), o_tperm = c.take(sizeof(uint32_t) * nt);
-  const size_t o_packed = c.take(g.packed_bytes), o_meta = c.take(g.meta_bytes), o_part = c.take(g.part_bytes);
-  const size_t o_work = c.take(mixed_order_bytes(nt, ns));
+  // One allocation for the call: orders, packed tiles, the sorts' workspace
+  // and the partial sums.
   Scratch buf;
-  SCTL_TRY(buf.alloc(c.bytes, stream));
-  char* w = buf.as<char>();
-  uint32_t* sperm = reinterpret_cast<uint32_t*>(w + o_sperm);
-  uint32_t* tperm = reinterpret_cast<uint32_t*>(w + o_tperm);
-  SCTL_TRY(mixed_order(r_trg, nt, r_src, ns, tperm, sperm, w + o_work, stream));
-  return mixed_apply(ker, newton, g, shape, r_trg, nt, tperm, r_src, n_src, v_src, ns, sperm,
-                     reinterpret_cast<float4*>(w + o_packed), reinterpret_cast<double4*>(w + o_meta),
-                     reinterpret_cast<double*>(w + o_part), v_trg, ctx, stream);
+  SCTL_TRY(buf.alloc(mixed_fixed_bytes(g, nt, ns) + g.part_bytes, stream));
+  return mixed_eval_in(ker, newton, g, shape, r_trg, nt, r_src, n_src, v_src, ns, v_trg, ctx, buf.as<char>(),
+                       stream);
 }
 
 }  // namespace sctl_amd

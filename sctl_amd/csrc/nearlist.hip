@@ -90,7 +90,7 @@ __global__ void k_nl_cube(const double* box, double* cube) {  // in place: cube 
 }
 __device__ __forceinline__ uint32_t quant(double x, double x0, double scale) {
   double u = (x - x0) * scale;
-  u = u < 0 ? 0 : (u > 2097151.0 ? 2097151.0 : u);  // also maps NaN to 0
+  u = fmin(fmax(u, 0.0), 2097151.0);  // fmax returns 0 for a NaN u
   return (uint32_t)u;
 }
 

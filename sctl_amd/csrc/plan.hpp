@@ -50,6 +50,10 @@ struct sctl_amd_plan_s {
   bool has_geometry = false;
   double ctx = 0.0;     // wavenumber of the Helmholtz kernels (sctl_amd_plan_set_ctx)
   bool has_ctx = false;
+  // Bumped whenever state a captured graph has baked in changes (work items,
+  // launch shape and grid, context value): graphs record it at capture and
+  // refuse to launch once it differs.
+  uint64_t generation = 0;
   sctl_amd::Item* d_items = nullptr;
   int64_t *d_trg_dsp = nullptr, *d_src_dsp = nullptr;
   int32_t *d_src_cnt = nullptr, *d_trg_cnt = nullptr;

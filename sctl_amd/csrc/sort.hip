@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(BB_WG) k_morton_keys(const double* __restrict_
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     double u = (p[k] - box[k]) * inv;
-    u = u < 0 ? 0 : (u > 1023 ? 1023 : u);  // also maps NaN to 0
+    u = fmin(fmax(u, 0.0), 1023.0);  // fmax returns 0 for a NaN u
     q[k] = (uint32_t)u;
   }
   keys[i] = (spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2)) | (in_a ? 0u : 1u << 30);

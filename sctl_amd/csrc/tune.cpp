@@ -168,7 +168,6 @@ DenseShape pick_dense_shape(int ker, int precision, int newton, int64_t nt, int6
   if (lookup(dense_key(ker, precision, newton, nt, ns), &v) && v.size() >= 2) {
     s.tpl = (int)v[0];
     s.occ = (int)v[1];
-    s.mfma = v.size() >= 4 ? (int)v[3] : 0;
     return s;
   }
   // FP64 on 2^16 or more targets: four targets per lane (clamped per kernel;
@@ -229,41 +228,38 @@ int resolve_dense(int kernel, int* precision, int digits, uint32_t flags, int* n
 }  // namespace
 
 int sctl_amd_dense_shape_get(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
-                             int* targets_per_lane, int* wg_per_cu_cap, int* matrix_cores) {
+                             int* targets_per_lane, int* wg_per_cu_cap) {
   int newton = 0;
   SCTL_TRY(resolve_dense(kernel, &precision, digits, flags, &newton));
   SCTL_CHECK_ARG(nt >= 0 && ns >= 0, "negative size");
   const DenseShape s = pick_dense_shape(kernel, precision, newton, nt, ns);
   if (targets_per_lane) *targets_per_lane = s.tpl;
   if (wg_per_cu_cap) *wg_per_cu_cap = s.occ;
-  if (matrix_cores) *matrix_cores = s.mfma;
   return SCTL_AMD_SUCCESS;
 }
 
 int sctl_amd_dense_shape_set(int kernel, int precision, int digits, uint32_t flags, int64_t nt, int64_t ns,
-                             int targets_per_lane, int wg_per_cu_cap, int matrix_cores) {
+                             int targets_per_lane, int wg_per_cu_cap) {
   int newton = 0;
   SCTL_TRY(resolve_dense(kernel, &precision, digits, flags, &newton));
   SCTL_CHECK_ARG(nt >= 0 && ns >= 0, "negative size");
   SCTL_CHECK_ARG(targets_per_lane == 1 || targets_per_lane == 2 || targets_per_lane == 4,
                  "targets_per_lane must be 1, 2 or 4");
   SCTL_CHECK_ARG(wg_per_cu_cap >= 0, "wg_per_cu_cap must be >= 0");
-  SCTL_CHECK_ARG(matrix_cores == 0 || matrix_cores == 1, "matrix_cores must be 0 or 1");
-  store(dense_key(kernel, precision, newton, nt, ns),
-        {(double)targets_per_lane, (double)wg_per_cu_cap, 0.0, (double)matrix_cores});
+  store(dense_key(kernel, precision, newton, nt, ns), {(double)targets_per_lane, (double)wg_per_cu_cap, 0.0});
   return SCTL_AMD_SUCCESS;
 }
 
 int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_trg, int64_t nt, const double* r_src,
                        const double* n_src, const double* v_src, int64_t ns, double* v_scratch, uint32_t flags,
-                       void* stream, int* best_tpl, int* best_occ, double* best_ms, int* best_matrix_cores) {
+                       void* stream, int* best_tpl, int* best_occ, double* best_ms) {
   int K0, K1, ND;
   if (kernel_dims(kernel, &K0, &K1, &ND) != SCTL_AMD_SUCCESS) {
     set_error("unknown kernel id " + std::to_string(kernel));
     return SCTL_AMD_ERROR_UNKNOWN_KERNEL;
   }
   SCTL_CHECK_ARG(!(flags & SCTL_AMD_HOST_PTRS), "the tuner takes device pointers");
-  SCTL_CHECK_ARG((flags & ~(SCTL_AMD_MIXED_NEWTON | SCTL_AMD_TUNE_MATRIX_CORES)) == 0, "unknown flag bits");
+  SCTL_CHECK_ARG((flags & ~SCTL_AMD_MIXED_NEWTON) == 0, "unknown flag bits");
   SCTL_CHECK_ARG(nt > 0 && ns > 0 && r_trg && r_src && v_src && v_scratch && (!ND || n_src), "bad arguments");
   SCTL_CHECK_ARG(digits >= -1, "digits must be -1 or >= 0");
   SCTL_CHECK_ARG(precision >= SCTL_AMD_FP64 && precision <= SCTL_AMD_AUTO, "unknown precision");
@@ -276,36 +272,29 @@ int sctl_amd_tune_eval(int kernel, int precision, int digits, const double* r_tr
   DenseShape bs;
   // The Helmholtz kernels are timed at wavenumber 1 (the shape does not depend on it).
   const double mu = kernel_needs_ctx(kernel) ? 1.0 : 0.0;
-  const bool with_mfma =
-      precision == SCTL_AMD_MIXED && mfma_supported(kernel) && (flags & SCTL_AMD_TUNE_MATRIX_CORES);
-  for (int mfma = 0; mfma <= (with_mfma ? 1 : 0); mfma++) {
-    for (int tpl : {1, 2, 4}) {
-      if (mfma && tpl == 4) continue;  // 16 row tiles per wave do not fit the register file
-      if (!mfma && tpl > kernel_max_tpl(kernel)) continue;
-      for (int occ : {0, 3, 2}) {
-        DenseShape sh;
-        sh.tpl = tpl;
-        sh.occ = occ;
-        sh.mfma = mfma;
-        double ms = 0;
-        auto run = [&]() {
-          return precision == SCTL_AMD_FP64
-                     ? launch_eval_f64(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, mu, s)
-                     : launch_eval_mixed(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, mu, s);
-        };
-        SCTL_TRY(time_ms(run, s, 2, &ms));
-        if (ms < best) {
-          best = ms;
-          bs = sh;
-        }
+  for (int tpl : {1, 2, 4}) {
+    if (tpl > kernel_max_tpl(kernel)) continue;
+    for (int occ : {0, 3, 2}) {
+      DenseShape sh;
+      sh.tpl = tpl;
+      sh.occ = occ;
+      double ms = 0;
+      auto run = [&]() {
+        return precision == SCTL_AMD_FP64
+                   ? launch_eval_f64(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, mu, s)
+                   : launch_eval_mixed(kernel, newton, r_trg, nt, r_src, n_src, v_src, ns, v_scratch, sh, mu, s);
+      };
+      SCTL_TRY(time_ms(run, s, 2, &ms));
+      if (ms < best) {
+        best = ms;
+        bs = sh;
       }
     }
   }
-  store(dense_key(kernel, precision, newton, nt, ns), {(double)bs.tpl, (double)bs.occ, best, (double)bs.mfma});
+  store(dense_key(kernel, precision, newton, nt, ns), {(double)bs.tpl, (double)bs.occ, best});
   if (best_tpl) *best_tpl = bs.tpl;
   if (best_occ) *best_occ = bs.occ;
   if (best_ms) *best_ms = best;
-  if (best_matrix_cores) *best_matrix_cores = bs.mfma;
   return SCTL_AMD_SUCCESS;
 }
 

@@ -68,3 +68,42 @@ def test_graph_replays_match_eager(gpu, k, prec, tol):
     G.launch()
     gpu.synchronize()
     assert W.rel_errors(v_trg.download(), 2 * ref)[0] < (1e-15 if prec == 0 else 1e-12)
+
+
+def test_graph_refused_after_plan_changes(gpu):
+    """A graph bakes in its plan's work items, launch shape and context: after
+    set_shape, tune or set_ctx on the plan it is refused, not replayed with
+    freed or stale state; a graph captured again replays correctly."""
+    B = W.torus_blocks(4, 4, seed=2, k0=1, ns_range=(64, 512), nt_range=(16, 200))
+    nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
+    P = gpu.Plan(1, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=0)
+    P.set_geometry(B["r_trg"], B["r_src"], B["n_src"])
+    P.set_shape(1, 1, 0)
+    v_src = gpu.DeviceArray(ns)
+    v_trg = gpu.DeviceArray.from_host(np.zeros(nt))
+    G = P.graph(2, 11, DENS_STREAM, v_src, v_trg)
+    G.launch()
+    gpu.synchronize()
+    first = v_trg.download()
+    P.set_shape(2, 2, 0)  # two targets per lane: new work items
+    with pytest.raises(gpu.SctlAmdError, match="capture it again"):
+        G.launch()
+    scratch = gpu.DeviceArray(nt)
+    P.tune(v_src, scratch)
+    with pytest.raises(gpu.SctlAmdError, match="capture it again"):
+        G.launch()
+    G2 = P.graph(2, 11, DENS_STREAM, v_src, v_trg)
+    v_trg.upload(np.zeros(nt))
+    G2.launch()
+    gpu.synchronize()
+    assert W.rel_errors(v_trg.download(), first)[0] < 1e-13
+    H = gpu.Plan(12, B["trg_dsp"], B["trg_cnt"], B["src_dsp"], B["src_cnt"], nt, ns, precision=0)
+    H.set_geometry(B["r_trg"], B["r_src"], None)
+    H.set_ctx(1.5)
+    h_src, h_trg = gpu.DeviceArray(ns * 2), gpu.DeviceArray.from_host(np.zeros(nt * 2))
+    GH = H.graph(1, 3, DENS_STREAM, h_src, h_trg)
+    GH.launch()
+    H.set_ctx(2.5)
+    with pytest.raises(gpu.SctlAmdError, match="capture it again"):
+        GH.launch()
+    gpu.synchronize()

@@ -107,50 +107,12 @@ def test_tune_eval_records_choice(gpu, tmp_path):
         xt, xs, f = W.cube(40000, 30000, seed=2)
         d = [gpu.DeviceArray.from_host(a) for a in (xt, xs, f)]
         scratch = gpu.DeviceArray(40000)
-        tpl, cap, ms, mc = gpu.tune_eval(0, d[0], d[1], None, d[2], scratch, precision=1)
-        assert tpl in (1, 2, 4) and cap in (0, 3, 2) and mc == 0 and ms > 0  # VALU shapes only by default
-        assert gpu.dense_shape(0, 30000, 40000, precision=1) == (tpl, cap, mc)  # cube(n_src, n_trg)
-        tpl, cap, ms, mc = gpu.tune_eval(0, d[0], d[1], None, d[2], scratch, precision=1,
-                                         flags=gpu.TUNE_MATRIX_CORES)
-        assert tpl in (1, 2, 4) and cap in (0, 3, 2) and mc in (0, 1) and ms > 0
-        assert gpu.dense_shape(0, 30000, 40000, precision=1) == (tpl, cap, mc)
+        tpl, cap, ms = gpu.tune_eval(0, d[0], d[1], None, d[2], scratch, precision=1)
+        assert tpl in (1, 2, 4) and cap in (0, 3, 2) and ms > 0
+        assert gpu.dense_shape(0, 30000, 40000, precision=1) == (tpl, cap)  # cube(n_src, n_trg)
         assert "dense k0 p1 n0" in open(tmp_path / "t.db").read()
         ref = oracle.eval(0, xt[:3000], xs, None, f)
         out = gpu.eval(0, xt, xs, None, f, precision=1)
         assert W.rel_errors(out[:1000], ref)[0] < 1e-6
-    finally:
-        gpu.tune_db_path(None)
-
-
-@pytest.mark.parametrize("kernel", [0, 1], ids=["LaplaceFxU", "LaplaceDxU"])
-@pytest.mark.parametrize("case", ["sphere", "cube", "coincident"])
-@pytest.mark.parametrize("tpl", [1, 2])
-def test_matrix_core_far_path_matches_oracle(gpu, tmp_path, kernel, case, tpl):
-    """The mixed Laplace kernels with far tiles on the matrix cores
-    (v_mfma_f32_16x16x4f32 for r^2 and r.n) against the oracle: far-dominated
-    (sphere sources, shell targets), mixed near/far (uniform cube), and
-    targets that coincide with sources (r = 0 contributes zero)."""
-    nd = kernel == 1
-    if case == "sphere":
-        xt, xs, n, f = W.sphere(24000, 5000, seed=3)
-    elif case == "cube":
-        xt, xs, f = W.cube(24000, 5000, seed=3)
-        n = W.normal(3, 9, 24000 * 3)
-    else:
-        _, xs, f = W.cube(24000, 10, seed=4)
-        n = W.normal(4, 9, 24000 * 3)
-        xt = xs[:3 * 5000].copy()
-    nt, ns = xt.size // 3, xs.size // 3
-    ref = oracle.eval(kernel, xt, xs, n if nd else None, f)
-    gpu.tune_db_path(str(tmp_path / "mc.db"))
-    try:
-        gpu.set_dense_shape(kernel, nt, ns, tpl, 0, 1, precision=1)
-        assert gpu.dense_shape(kernel, nt, ns, precision=1) == (tpl, 0, 1)
-        out = gpu.eval(kernel, xt, xs, n if nd else None, f, precision=1)
-        e_inf, e_rms = W.rel_errors(out, ref)
-        assert e_inf < 1e-6 and e_rms < 1e-6, (e_inf, e_rms)
-        gpu.set_dense_shape(kernel, nt, ns, tpl, 0, 0, precision=1)
-        valu = gpu.eval(kernel, xt, xs, n if nd else None, f, precision=1)
-        assert W.rel_errors(out, valu)[0] < 1e-6
     finally:
         gpu.tune_db_path(None)
