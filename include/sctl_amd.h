@@ -124,6 +124,24 @@ int sctl_amd_eval_ctx(int kernel, const double* ctx, int precision, int digits, 
                       const double* r_src, const double* n_src, const double* v_src, int64_t ns, double* v_trg,
                       uint32_t flags, void* stream);
 
+/* No counterpart in the reference: sctl_amd_eval_ctx for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * hosts that forbid hipMalloc).  Same arguments and results, device pointers
+ * only (SCTL_AMD_HOST_PTRS is refused; flags: SCTL_AMD_MIXED_NEWTON).  It
+ * allocates nothing: its scratch is one fixed device array per GPU of
+ * sctl_amd_eval_noalloc_scratch_bytes() bytes (128 MiB), held by the library
+ * until the process ends.  FP64 Evals of any size fit (their split partial
+ * sums run in as many launches as the array needs); a mixed-precision Eval
+ * whose Morton orders, packed source tiles and sort workspace exceed the
+ * array (about 2^20 + 2^20 Laplace points is the largest that fits) fails
+ * with SCTL_AMD_ERROR_INVALID_VALUE.  Every call on a GPU uses that GPU's one
+ * array: calls that overlap on a GPU (other streams or host threads) race,
+ * and the caller must serialise them. */
+int sctl_amd_eval_noalloc(int kernel, const double* ctx, int precision, int digits, const double* r_trg, int64_t nt,
+                          const double* r_src, const double* n_src, const double* v_src, int64_t ns, double* v_trg,
+                          uint32_t flags, void* stream);
+size_t sctl_amd_eval_noalloc_scratch_bytes(void);
+
 /* ------------------------------------------------------------ KernelMatrix
  * Replaces GenericKernel<uKernel>::KernelMatrix<double,omp,digits>
  * (generic-kernel.hpp:124, generic-kernel.txx:213-329):

@@ -56,6 +56,7 @@ SYMBOLS = [
     "sctl_amd_near_list_size", "sctl_amd_near_list_get", "sctl_amd_near_list_destroy", "sctl_amd_eval_ctx",
     "sctl_amd_kernel_matrix_ctx", "sctl_amd_plan_set_ctx", "sctl_amd_dense_plan_create", "sctl_amd_dense_plan_apply",
     "sctl_amd_dense_plan_shape", "sctl_amd_dense_plan_destroy", "sctl_amd_dense_plan_graph_create",
+    "sctl_amd_eval_noalloc", "sctl_amd_eval_noalloc_scratch_bytes",
 ]
 
 
@@ -90,6 +91,8 @@ def lib():
     L.sctl_amd_kernel_info.argtypes = [i32] + [ctypes.POINTER(i32)] * 5 + [_dp, ctypes.POINTER(ctypes.c_char_p)]
     L.sctl_amd_eval.argtypes = [i32, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp]
     L.sctl_amd_eval_ctx.argtypes = [i32, _dp, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp]
+    L.sctl_amd_eval_noalloc.argtypes = [i32, _dp, i32, i32, _vp, i64, _vp, _vp, _vp, i64, _vp, u32, _vp]
+    L.sctl_amd_eval_noalloc_scratch_bytes.restype = ctypes.c_size_t
     L.sctl_amd_kernel_matrix.argtypes = [i32, i32, _vp, i64, _vp, _vp, i64, _vp, u32, _vp]
     L.sctl_amd_kernel_matrix_ctx.argtypes = [i32, _dp, i32, _vp, i64, _vp, _vp, i64, _vp, u32, _vp]
     L.sctl_amd_plan_set_ctx.argtypes = [_vp, _dp]

@@ -20,8 +20,8 @@ LIB = os.path.join(PKG, "libsctl_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["dense.hip", "mixed.hip", "sort.hip", "batched.hip", "near.hip", "nearlist.hip", "graph.hip", "capi.cpp",
-           "tune.cpp", "profile.cpp", "dense_plan.cpp"]
+SOURCES = ["dense.hip", "mixed.hip", "sort.hip", "batched.hip", "near.hip", "nearlist.hip", "graph.hip", "noalloc.hip",
+           "capi.cpp", "tune.cpp", "profile.cpp", "dense_plan.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
