@@ -64,10 +64,10 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def timed_evals(kernel, dev, out, precision, digits, steps, warmup, stream):
+def timed_evals(kernel, dev, out, precision, digits, steps, warmup, stream, flags=0):
     xt, xs, n, f = dev
     for _ in range(warmup):
-        S.eval(kernel, xt, xs, n, f, out, precision=precision, digits=digits, stream=stream)
+        S.eval(kernel, xt, xs, n, f, out, precision=precision, digits=digits, stream=stream, flags=flags)
     S.synchronize(stream)
     S.profile_reset()
     S.profile_enable(True)
@@ -75,7 +75,7 @@ def timed_evals(kernel, dev, out, precision, digits, steps, warmup, stream):
     t0 = time.perf_counter()
     e0.record(stream)
     for _ in range(steps):
-        S.eval(kernel, xt, xs, n, f, out, precision=precision, digits=digits, stream=stream)
+        S.eval(kernel, xt, xs, n, f, out, precision=precision, digits=digits, stream=stream, flags=flags)
     e1.record(stream)
     S.synchronize(stream)
     wall = time.perf_counter() - t0
@@ -105,10 +105,11 @@ def cpu_baseline(kernel, xt, xs, n, f, seconds):
     import oracle
     cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     kind = "reference"
+    kname = S.kernel_info(kernel)[6].replace("-", "_")
     if oracle.Reference.available("v4") and oracle.have_avx512():
         ref = oracle.Reference("v4")
         run = lambda m: ref.eval(kernel, xt[:3 * m], xs, n, f, omp=True)
-        what = "reference GenericKernel<Laplace3D_FxU>::Eval<double,true> (AVX-512 + OpenMP)"
+        what = "reference GenericKernel<%s>::Eval<double,true> (AVX-512 + OpenMP)" % kname
     else:
         kind = "port"
         mode = oracle.AVX512 if oracle.have_avx512() else oracle.EXACT
@@ -130,6 +131,87 @@ def cpu_baseline(kernel, xt, xs, n, f, seconds):
     return {"value": round(m * ns / tmed / 1e9, 3), "unit": "G interactions/s", "cores": cores, "kind": kind,
             "sample": "%s: %d of the %d targets x all %d sources, median of 3 runs (%.2f s each)" % (
                 what, m, xt.size // 3, ns, tmed), "host": host_cpu(oracle.have_avx512())}
+
+
+def cpu_blocks_baseline(kernel, B, seconds, mode=0):
+    """BoundaryIntegralOp's per-block loop of the reference, compiled from its
+    sources (OpenMP over the blocks, boundary_integral.txx:1165-1211): each
+    block by GenericKernel::Eval<double,false> (mode 0) or
+    KernelMatrix<double,false> (mode 1), on the first m blocks, m grown until
+    one run takes about `seconds`; rate = the sample's Ns*Nt / time."""
+    import oracle
+    if not (oracle.Reference.available("v4") and oracle.have_avx512()):
+        return None
+    ref = oracle.Reference("v4")
+    cnt = B["trg_cnt"] * B["src_cnt"]
+    total = int(cnt.sum())
+
+    def run(m):
+        args = (B["trg_dsp"][:m], B["trg_cnt"][:m], B["src_dsp"][:m], B["src_cnt"][:m], B["r_trg"], B["r_src"],
+                B["n_src"])
+        if mode == 0:
+            ref.blocks(kernel, 0, *args, v_src=B["v_src"])
+        else:
+            ref.blocks(kernel, 1, *args)
+
+    m = 16
+    t = time.perf_counter()
+    run(m)
+    dt = max(time.perf_counter() - t, 1e-6)
+    inter = int(cnt[:m].sum())
+    while dt < seconds / 3 and m < cnt.size:
+        m = int(min(cnt.size, max(2 * m, m * seconds / dt)))
+        t = time.perf_counter()
+        run(m)
+        dt = max(time.perf_counter() - t, 1e-6)
+        inter = int(cnt[:m].sum())
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        run(m)
+        times.append(time.perf_counter() - t)
+    tmed = float(np.median(times))
+    what = "Eval<double,false>" if mode == 0 else "KernelMatrix<double,false>"
+    return {"value": round(inter / tmed / 1e9, 3), "unit": "G interactions/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1), "kind": "reference",
+            "sample": "reference per-block %s loop (OpenMP over blocks, AVX-512): first %d of %d blocks (%.3g of "
+                      "%.3g interactions), median of 3 runs (%.2f s each)" % (what, m, cnt.size, inter, total, tmed),
+            "host": host_cpu(oracle.have_avx512())}
+
+
+def kernel_matrix_end_to_end(kernel, B, nblocks, threads=8):
+    """The drop-in's KernelMatrix as BoundaryIntegralOp calls it: per block,
+    host arrays in, the (Ns*K0) x (Nt*K1) matrix back in host memory
+    (staging, kernel and device-to-host copy included), from `threads` host
+    threads at once over the first `nblocks` blocks."""
+    from concurrent.futures import ThreadPoolExecutor
+    _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
+    L = S.lib()
+    sizes = [(int(B["trg_cnt"][b]) * K1, int(B["src_cnt"][b]) * K0) for b in range(nblocks)]
+    biggest = max(a * c for a, c in sizes)
+    bufs = [np.empty(biggest) for _ in range(threads)]
+    xt, xs, xn = B["r_trg"], B["r_src"], B["n_src"]
+
+    def one(b, buf):
+        t0, tc, s0, sc = (int(B[a][b]) for a in ("trg_dsp", "trg_cnt", "src_dsp", "src_cnt"))
+        S.check(L.sctl_amd_kernel_matrix(kernel, -1, xt[3 * t0:].ctypes.data, tc, xs[3 * s0:].ctypes.data,
+                                         xn[3 * s0:].ctypes.data if ND else None, sc, buf.ctypes.data,
+                                         S.HOST_PTRS, None))
+
+    def worker(w):
+        for b in range(w, nblocks, threads):
+            one(b, bufs[w])
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda w: one(w, bufs[w]), range(threads)))  # warm-up (per-thread streams, pool)
+        t = time.perf_counter()
+        list(ex.map(worker, range(threads)))
+        dt = time.perf_counter() - t
+    entries = int(sum(B["trg_cnt"][:nblocks] * B["src_cnt"][:nblocks])) * K0 * K1
+    inter = int(sum(B["trg_cnt"][:nblocks] * B["src_cnt"][:nblocks]))
+    return {"blocks": nblocks, "host_threads": threads, "s": round(dt, 4),
+            "G_int_per_s": round(inter / dt / 1e9, 2), "matrix_GB": round(entries * 8 / 1e9, 3),
+            "matrix_GB_per_s_to_host": round(entries * 8 / dt / 1e9, 2)}
 
 
 def time_once(fn, stream, reps=1):
@@ -362,13 +444,31 @@ def hbm_copy_reference(stream, gib=4):
                      "frac_hbm_peak": round(n / ms_fill / 1e6 / 8000.0, 3)}}
 
 
-def other_configs(stream, quick=False):
-    """The other configurations: measured, not the headline."""
+def add_roofline(r, kernel, key_ms="ms", inter=None):
+    """Per-path roofline of one configuration line: algorithmic TFLOP/s
+    (flop per interaction x rate) against the FP64 (78.6) or FP32 (157.3)
+    VALU peak."""
+    flop = FLOP_PER_INT.get(kernel, S.kernel_info(kernel)[4])
+    out = {"flop_per_interaction": flop}
+    for name, peak in (("fp64", PEAK_FP64_TFLOPS), ("mixed", PEAK_FP32_TFLOPS)):
+        if name in r and "G_int_per_s" in r[name]:
+            tf = r[name]["G_int_per_s"] * flop / 1e3
+            out[name] = {"achieved_TFLOP_per_s": round(tf, 2), "peak": peak, "frac": round(tf / peak, 4)}
+    r["roofline"] = out
+    return r
+
+
+def other_configs(stream, quick=False, cpu=True, cpu_seconds=3.0):
+    """The other configurations: measured, not the headline.  With cpu, the
+    reference's own CPU code is timed beside each on a bounded sample."""
     import oracle
     out = {}
     log("config 1: Laplace FxU FP64, 20000 x 20000 uniform cube, parity vs the oracle")
     xt, xs, f = W.cube(20000, 20000, seed=0)
     r, o = dense_pair(S.LAPLACE3D_FXU, xt, xs, None, f, stream, reps=5)
+    add_roofline(r, S.LAPLACE3D_FXU)
+    if cpu:
+        r["cpu_reference"] = cpu_baseline(S.LAPLACE3D_FXU, xt, xs, None, f, 1.0)
     ref = oracle.eval(S.LAPLACE3D_FXU, xt, xs, None, f)
     r["fp64_vs_oracle"] = dict(zip(("max_rel", "rms_rel"), (float("%.3g" % e) for e in W.rel_errors(o["fp64"], ref))))
     r["mixed_vs_oracle"] = dict(zip(("max_rel", "rms_rel"), (float("%.3g" % e) for e in W.rel_errors(o["mixed"], ref))))
@@ -377,7 +477,12 @@ def other_configs(stream, quick=False):
     N = 1 << 20
     log("config 2 (DxU leg): Laplace DxU, 2^20 sphere sources x 2^20 shell targets")
     xt, xs, nrm, f = W.sphere(N, N, seed=0)
-    out["c2_laplace_dxu_sphere_1M"], _ = dense_pair(S.LAPLACE3D_DXU, xt, xs, nrm, f, stream, reps=1 if quick else 2)
+    r, _ = dense_pair(S.LAPLACE3D_DXU, xt, xs, nrm, f, stream, reps=1 if quick else 2)
+    add_roofline(r, S.LAPLACE3D_DXU)
+    if cpu:
+        log("config 2: reference CPU Eval on a target sample")
+        r["cpu_reference"] = cpu_baseline(S.LAPLACE3D_DXU, xt, xs, nrm, f, cpu_seconds)
+    out["c2_laplace_dxu_sphere_1M"] = r
     log("config 3: Stokes FxU, 524288 points in 64 Gaussian clusters")
     M = 524288
     xt, xs, f = W.clusters(M, M, seed=0, k0=3)
@@ -390,10 +495,24 @@ def other_configs(stream, quick=False):
     r["fp64_vs_oracle_subset"] = dict(zip(("max_rel", "rms_rel"), (float("%.3g" % e) for e in W.rel_errors(
         o["fp64"][q], ref))))
     r["oracle_subset_targets"] = sub
+    add_roofline(r, S.STOKES3D_FXU)
+    if cpu:
+        log("config 3: reference CPU Eval on a target sample")
+        r["cpu_reference"] = cpu_baseline(S.STOKES3D_FXU, xt, xs, None, f, cpu_seconds)
     out["c3_stokes_fxu_clusters_512k"] = r
     log("config 4: batched Laplace DxU, 4096 torus blocks")
     B = W.torus_blocks(64, 64, seed=0, k0=1)
-    out["c4_batched_laplace_dxu_4096"], _ = plan_pair(S.LAPLACE3D_DXU, B, stream, reps=5)
+    r, _ = plan_pair(S.LAPLACE3D_DXU, B, stream, reps=5)
+    add_roofline(r, S.LAPLACE3D_DXU)
+    if cpu:
+        log("config 4: reference CPU per-block Eval loop on a block sample")
+        r["cpu_reference"] = cpu_blocks_baseline(S.LAPLACE3D_DXU, B, cpu_seconds, mode=0)
+    out["c4_batched_laplace_dxu_4096"] = r
+    log("config 4 blocks: drop-in KernelMatrix per block, host arrays in and out, 8 host threads")
+    km = kernel_matrix_end_to_end(S.LAPLACE3D_DXU, B, 512)
+    if cpu:
+        km["cpu_reference"] = cpu_blocks_baseline(S.LAPLACE3D_DXU, B, cpu_seconds, mode=1)
+    out["c4_kernel_matrix_end_to_end"] = km
     log("HBM reference: device-to-device copy and fill")
     out["hbm_copy_4GiB"] = hbm_copy_reference(stream)
     log("config 4 blocks: near-field matrices (KNearSubtract, GEMV), Laplace DxU")
@@ -402,7 +521,12 @@ def other_configs(stream, quick=False):
     out["c4_near_list_4096"] = near_list_build(B, 0.05, stream)
     log("config 5: batched Stokes DxU, 2048 torus blocks, 100 applies in one hipGraph")
     B = W.torus_blocks(64, 32, seed=0, k0=3)
-    out["c5_stokes_dxu_graph_2048x100"] = graph_pair(S.STOKES3D_DXU, B, stream, nsteps=20 if quick else 100)
+    r = graph_pair(S.STOKES3D_DXU, B, stream, nsteps=20 if quick else 100)
+    add_roofline(r, S.STOKES3D_DXU)
+    if cpu:
+        log("config 5: reference CPU per-block Eval loop on a block sample")
+        r["cpu_reference"] = cpu_blocks_baseline(S.STOKES3D_DXU, B, cpu_seconds, mode=0)
+    out["c5_stokes_dxu_graph_2048x100"] = r
     log("config 5 blocks: near-field matrices (KNearSubtract, GEMV), Stokes DxU")
     out["c5_near_matrix_stokes_dxu_2048"] = near_ops(S.STOKES3D_DXU, B, stream, reps=2)
     log("other micro-kernels: FP64 and mixed Eval, 2^17 x 2^17 uniform cube")
@@ -556,6 +680,9 @@ def main():
     log("fp64 (digits 14): timed steps")
     f64b = timed_evals(K, dev, out, S.FP64, 14, args.steps, min(args.warmup, 1), stream)
 
+    log("mixed with one FP32 Newton step after v_rsq_f32 (SCTL_AMD_MIXED_NEWTON): timed steps")
+    mxn = timed_evals(K, dev, out, S.MIXED, -1, args.steps, min(args.warmup, 1), stream, flags=S.MIXED_NEWTON)
+
     log("accuracy: mixed vs fp64 on the full output")
     o64 = S.DeviceArray.from_host(np.zeros(N))
     omx = S.DeviceArray.from_host(np.zeros(N))
@@ -563,11 +690,15 @@ def main():
     S.eval(K, dev[0], dev[1], None, dev[3], omx, precision=S.MIXED, stream=stream)
     S.synchronize(stream)
     e_inf, e_rms = W.rel_errors(omx.download(), o64.download())
+    omn = S.DeviceArray.from_host(np.zeros(N))
+    S.eval(K, dev[0], dev[1], None, dev[3], omn, precision=S.MIXED, stream=stream, flags=S.MIXED_NEWTON)
+    S.synchronize(stream)
+    n_inf, n_rms = W.rel_errors(omn.download(), o64.download())
 
     def rate(r):
         return inter * args.steps / (r["total_ms"] * 1e-3) / 1e9
 
-    v_mx, v_64, v_64b = rate(mx), rate(f64), rate(f64b)
+    v_mx, v_64, v_64b, v_mxn = rate(mx), rate(f64), rate(f64b), rate(mxn)
     kern_ms = mx["kernel_ms"] / max(mx["launches"], 1)
     achieved = FLOP_PER_INT[K] * inter / (kern_ms * 1e-3) / 1e12
     step_tflops = FLOP_PER_INT[K] * v_mx * 1e9 / 1e12
@@ -577,7 +708,7 @@ def main():
     configs = None
     if not args.no_configs:
         try:
-            configs = other_configs(stream)
+            configs = other_configs(stream, cpu=not args.no_cpu_baseline)
         except Exception as ex:
             log("other configurations failed:", repr(ex))
             configs = {"error": repr(ex)}
@@ -614,6 +745,11 @@ def main():
         "fp64": {"value": round(v_64, 2), "ms_per_step": round(f64["total_ms"] / args.steps, 3), "digits": 15,
                  "pct_fp64_valu_peak": round(100 * achieved64 / PEAK_FP64_TFLOPS, 1),
                  "digits14_value": round(v_64b, 2)},
+        "mixed_newton": {"value": round(v_mxn, 2), "ms_per_step": round(mxn["total_ms"] / args.steps, 3),
+                         "max_rel_err_vs_fp64": float("%.3g" % n_inf), "rms_rel_err_vs_fp64": float("%.3g" % n_rms),
+                         "speedup_vs_fp64": round(v_mxn / v_64, 3),
+                         "pct_fp32_valu_peak": round(100 * FLOP_PER_INT[K] * v_mxn / 1e3 / PEAK_FP32_TFLOPS, 1),
+                         "form": "v_rsq_f32 and one FP32 Newton step per pair (SCTL_AMD_MIXED_NEWTON)"},
         "speedup_vs_fp64": round(v_mx / v_64, 3),
         "speedup_vs_fp64_digits14": round(v_mx / v_64b, 3),
         "pct_fp32_valu_peak": round(100 * step_tflops / PEAK_FP32_TFLOPS, 1),

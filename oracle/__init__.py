@@ -297,6 +297,8 @@ class Reference:
                                        _dp]
         L.sctlref_kernel_matrix.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _L, _dp, _dp, _L, _dp, _dp]
         L.sctlref_rsqrt.argtypes = [ctypes.c_int, _dp, _L, _dp]
+        _lp = ctypes.POINTER(ctypes.c_long)
+        L.sctlref_blocks.argtypes = [ctypes.c_int, ctypes.c_int, _L, _lp, _lp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.sctlref_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp, ctypes.c_char_p, ctypes.c_int]
         self.lib = L
         self.flavour = flavour
@@ -359,6 +361,26 @@ class Reference:
         if self.lib.sctlref_rsqrt(digits, _p(x), x.size, _p(y)) != 0:
             raise ValueError("digits %r not instantiated" % digits)
         return y
+
+    def blocks(self, kernel, mode, trg_dsp, trg_cnt, src_dsp, src_cnt, r_trg, r_src, n_src, v_src=None,
+               v_trg=None):
+        """BoundaryIntegralOp's per-block OpenMP loop (boundary_integral.txx:1165-1211): Eval<double,false>
+        per block into v_trg (mode 0, returned), or KernelMatrix<double,false> per block (mode 1; returns a
+        checksum of the matrices, which are not kept)."""
+        (_, K0, K1, ND, _), _, _ = self.info(kernel)
+        a = [np.ascontiguousarray(x, dtype=np.int64) for x in (trg_dsp, trg_cnt, src_dsp, src_cnt)]
+        lp = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
+        r_trg, r_src = _c(r_trg), _c(r_src)
+        n_src = _c(n_src) if ND else None
+        v_src = _c(v_src) if v_src is not None else None
+        if mode == 0 and v_trg is None:
+            v_trg = np.zeros(r_trg.size // 3 * K1)
+        chk = ctypes.c_double(0)
+        rc = self.lib.sctlref_blocks(kernel, mode, a[0].size, lp(a[0]), lp(a[1]), lp(a[2]), lp(a[3]), _p(r_trg),
+                                     _p(r_src), _p(n_src), _p(v_src), _p(v_trg), ctypes.pointer(chk))
+        if rc != 0:
+            raise RuntimeError("sctlref_blocks failed rc=%d" % rc)
+        return v_trg if mode == 0 else chk.value
 
 
 class ReferenceNearList:

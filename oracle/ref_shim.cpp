@@ -152,6 +152,41 @@ int sctlref_kernel_matrix(int kernel, int digits, const double* r_trg, long nt, 
   });
 }
 
+// BoundaryIntegralOp's per-block loop (boundary_integral.txx:1165-1211):
+// `#pragma omp parallel for schedule(dynamic)` over the blocks, each block
+// (targets [trg_dsp, +trg_cnt), sources [src_dsp, +src_cnt)) evaluated by
+// GenericKernel::Eval<double,false> into v_trg (mode 0), or by
+// KernelMatrix<double,false> into a per-block matrix whose entries are summed
+// into *checksum (mode 1; the matrices themselves are not kept).
+int sctlref_blocks(int kernel, int mode, long nblocks, const long* trg_dsp, const long* trg_cnt, const long* src_dsp,
+                   const long* src_cnt, const double* r_trg, const double* r_src, const double* n_src,
+                   const double* v_src, double* v_trg, double* checksum) {
+  if (kernel >= 12 || mode < 0 || mode > 1) return -1;
+  return with_kernel(kernel, [&](auto ker, auto) {
+    using K = decltype(ker);
+    const Long K0 = K::SrcDim(), K1 = K::TrgDim(), nd = K::NormalDim();
+    double sum = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : sum)
+    for (long b = 0; b < nblocks; b++) {
+      const long nt = trg_cnt[b], ns = src_cnt[b];
+      if (!nt || !ns) continue;
+      Vec<double> Xt(nt * 3, (sctl::Iterator<double>)(r_trg + trg_dsp[b] * 3), false);
+      Vec<double> Xs(ns * 3, (sctl::Iterator<double>)(r_src + src_dsp[b] * 3), false);
+      Vec<double> Xn(nd ? ns * 3 : 0, (sctl::Iterator<double>)(nd ? n_src + src_dsp[b] * 3 : nullptr), false);
+      if (mode == 0) {
+        Vec<double> Vs(ns * K0, (sctl::Iterator<double>)(v_src + src_dsp[b] * K0), false);
+        Vec<double> Vt(nt * K1, v_trg + trg_dsp[b] * K1, false);
+        ker.template Eval<double, false>(Vt, Xt, Xs, Xn, Vs);
+      } else {
+        sctl::Matrix<double> M;
+        ker.template KernelMatrix<double, false>(M, Xt, Xs, Xn);
+        sum += M[0][0] + M[M.Dim(0) - 1][M.Dim(1) - 1];
+      }
+    }
+    if (checksum) *checksum = sum;
+  });
+}
+
 int sctlref_rsqrt(int digits, const double* x, long n, double* y) {
   switch (digits) {
     case 4: rsqrt_digits<4>(x, n, y); return 0;

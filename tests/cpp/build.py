@@ -3,6 +3,7 @@
   test_generic_kernel    the drop-in on the library's own containers
   test_concurrent_calls  the drop-in called from 8 OpenMP threads at once,
                          checked against the oracle (links oracle/liboracle.so)
+  test_near_field_adapter  the near-field operator set up from element lists
   test_dropin_sctl       the drop-in with SCTL's own types under SCTL_MEMDEBUG
                          and SCTL_PROFILE; built only where the reference
                          headers are present (nothing from them is needed at
@@ -43,6 +44,10 @@ def build(verbose=True):
     cmd, out = _base("test_concurrent_calls")
     _run(cmd + ["-fopenmp", "-L" + PKG, "-lsctl_amd", "-L" + ORACLE, "-loracle",
                 "-Wl,-rpath,$ORIGIN/../../sctl_amd", "-Wl,-rpath,$ORIGIN/../../oracle", "-o", out])
+    built.append(out)
+
+    cmd, out = _base("test_near_field_adapter")
+    _run(cmd + ["-L" + PKG, "-lsctl_amd", "-Wl,-rpath,$ORIGIN/../../sctl_amd", "-o", out])
     built.append(out)
 
     if os.path.isdir(REF_INC):

@@ -18,6 +18,11 @@ def test_dropin_header_compiles():
     assert r.returncode == 0, r.stderr
 
 
+def test_near_field_header_compiles():
+    r = _syntax(os.path.join(ROOT, "tests", "cpp", "test_near_field_adapter.cpp"))
+    assert r.returncode == 0, r.stderr
+
+
 @pytest.mark.skipif(not os.path.isdir(REF_INC), reason="reference headers not present")
 @pytest.mark.parametrize("memdebug", [False, True], ids=["default", "SCTL_MEMDEBUG"])
 def test_dropin_accepts_sctl_types(memdebug):
@@ -64,6 +69,14 @@ def test_cpp_dropin_concurrent_openmp_calls(gpu):
     BoundaryIntegralOp calls them (boundary_integral.txx:1165-1211): bit-identical to one
     thread, and the oracle at 1e-13."""
     _run_program("test_concurrent_calls")
+
+
+@pytest.mark.gpu
+def test_cpp_near_field_from_element_lists(gpu):
+    """sctl_amd::NearFieldOp set up from two element lists (one matrix-free) through the
+    ElementListBase interface: near list, KNearSubtract + NearInterac and the direct part
+    against long-double sums over the lists' own nodes."""
+    _run_program("test_near_field_adapter")
 
 
 @pytest.mark.gpu
