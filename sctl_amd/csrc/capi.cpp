@@ -2,10 +2,12 @@
 // precision/Newton selection, and dispatch to the HIP launchers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "internal.hpp"
 #include "kernels.hpp"
@@ -146,6 +148,79 @@ int stage_in(Scratch& buf, const double* host, int64_t n, hipStream_t s, const d
   SCTL_HIP_TRY(hipMemcpyAsync(buf.as<double>(), host, sizeof(double) * n, hipMemcpyHostToDevice, s));
   *dev = buf.as<double>();
   return SCTL_AMD_SUCCESS;
+}
+
+// Pinned host staging for large device-to-host results (a host KernelMatrix):
+// chunks of PIN_CHUNK bytes taken from a process-wide pool of page-locked
+// buffers (allocated with hipHostMalloc on first use, reused, never returned
+// to the driver), two per call so that the copy of one chunk into the
+// caller's memory overlaps the DMA of the next.
+constexpr size_t PIN_CHUNK = (size_t)8 << 20;
+std::mutex g_pin_mutex;
+std::vector<void*> g_pin_free;
+
+int pin_take(void** p) {
+  {
+    std::lock_guard<std::mutex> lock(g_pin_mutex);
+    if (!g_pin_free.empty()) {
+      *p = g_pin_free.back();
+      g_pin_free.pop_back();
+      return SCTL_AMD_SUCCESS;
+    }
+  }
+  SCTL_HIP_TRY(hipHostMalloc(p, PIN_CHUNK, hipHostMallocDefault));
+  return SCTL_AMD_SUCCESS;
+}
+void pin_give(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lock(g_pin_mutex);
+  g_pin_free.push_back(p);
+}
+
+// host[0, bytes) = dev[0, bytes), through two pinned chunks on stream s
+// (synchronous on return).
+int copy_to_host_pinned(void* host, const void* dev, size_t bytes, hipStream_t s) {
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = SCTL_AMD_SUCCESS;
+  for (int i = 0; i < 2 && rc == SCTL_AMD_SUCCESS; i++) {
+    rc = pin_take(&pin[i]);
+    if (rc == SCTL_AMD_SUCCESS && hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) {
+      set_error("hipEventCreateWithFlags failed");
+      rc = SCTL_AMD_ERROR_HIP;
+    }
+  }
+  size_t prev_off = 0, prev_len = 0;
+  for (size_t off = 0, i = 0; rc == SCTL_AMD_SUCCESS && off < bytes + PIN_CHUNK; off += PIN_CHUNK, i++) {
+    const size_t len = off < bytes ? std::min(PIN_CHUNK, bytes - off) : 0;
+    if (len) {
+      hipError_t e = hipMemcpyAsync(pin[i % 2], static_cast<const char*>(dev) + off, len, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipEventRecord(ev[i % 2], s);
+      if (e != hipSuccess) {
+        set_error(std::string("pinned staging copy: ") + hipGetErrorString(e));
+        rc = SCTL_AMD_ERROR_HIP;
+        break;
+      }
+    }
+    if (prev_len) {  // the previous chunk: wait for its DMA, copy it out
+      const hipError_t e = hipEventSynchronize(ev[(i + 1) % 2]);
+      if (e != hipSuccess) {
+        set_error(std::string("pinned staging wait: ") + hipGetErrorString(e));
+        rc = SCTL_AMD_ERROR_HIP;
+        break;
+      }
+      std::memcpy(static_cast<char*>(host) + prev_off, pin[(i + 1) % 2], prev_len);
+    }
+    prev_off = off;
+    prev_len = len;
+    if (!len) break;
+  }
+  if (rc != SCTL_AMD_SUCCESS) (void)hipStreamSynchronize(s);
+  for (int i = 0; i < 2; i++) {
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+    pin_give(pin[i]);
+  }
+  return rc;
 }
 
 int check_common(int kernel, const double* r_trg, int64_t nt, const double* r_src, const double* n_src, int64_t ns,
@@ -306,9 +381,8 @@ int sctl_amd_kernel_matrix_ctx(int kernel, const double* ctx, int digits, const 
   SCTL_TRY(launch_kernel_matrix(kernel, newton_steps_f64(digits), rt, nt, rs, nsrc, ns, Md, ctx_value(kernel, ctx),
                                 s));
   if (host) {
-    SCTL_HIP_TRY(hipMemcpyAsync(M, Md, sizeof(double) * nM, hipMemcpyDeviceToHost, s));
+    SCTL_TRY(copy_to_host_pinned(M, Md, sizeof(double) * nM, s));
     b_rt.release(); b_rs.release(); b_ns.release(); b_M.release();
-    SCTL_HIP_TRY(hipStreamSynchronize(s));
   }
   return SCTL_AMD_SUCCESS;
 }

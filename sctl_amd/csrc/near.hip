@@ -56,6 +56,12 @@ constexpr int GEMV_ROWS = 16;   // rows per GEMV wave tile
 constexpr int GEMV_GR = 8;      // GEMV rows walked together
 constexpr int NEAR_NEWTON = 2;  // KernelMatrix at digits -1 (boundary_integral.txx:1197,1211)
 
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long i = __double_as_longlong(v);
   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(i & 0xffffffffLL), l);
@@ -103,19 +109,29 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
   constexpr bool SUBPRE = SUB && W == 1;
   static_assert(!NDOT || K1 % 3 == 0, "the target-normal contraction needs 3 outputs per row");
   const int lane = threadIdx.x & 63;
+  __shared__ double tslot_all[(NWG / 64) * MAT_ST * 6];
+  double* tslot = tslot_all + (threadIdx.x >> 6) * (MAT_ST * 6);
   const int64_t nwaves = (int64_t)gridDim.x * (NWG / 64);
   for (int64_t tile = (int64_t)blockIdx.x * (NWG / 64) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
     const int b = __builtin_amdgcn_readfirstlane(tile_blk[tile]);
     const int t0 = (int)(tile - tile_dsp[b]) * MAT_ST;
     const int sc = src_cnt[b];
     const int nts = min(MAT_ST, trg_cnt[b] - t0);
-    const int64_t tg = trg_dsp[b] + t0 + min(lane, nts - 1);
-    const double mx = xt[tg * 3 + 0], my = xt[tg * 3 + 1], mz = xt[tg * 3 + 2];
-    double mn[3] = {0, 0, 0};
-    if constexpr (NDOT) {
+    // The stripe's targets (and normals) in the wave's LDS slot, read back per
+    // row by broadcast (one address for the whole wave): no VALU work per row
+    // and no registers held across the chunk loop.
+    wave_sync();
+    if (lane < MAT_ST) {
+      const int64_t tg = trg_dsp[b] + t0 + min(lane, nts - 1);
 #pragma unroll
-      for (int l = 0; l < 3; l++) mn[l] = n_trg[tg * 3 + l];
+      for (int k = 0; k < 3; k++) tslot[lane * 6 + k] = xt[tg * 3 + k];
+      if constexpr (NDOT) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) tslot[lane * 6 + 3 + k] = n_trg[tg * 3 + k];
+      }
     }
+    wave_sync();
+    const double* tx = tslot;
     const int64_t ld = (int64_t)sc * K0;
     double* Ks = K + mat_off[b] + (int64_t)t0 * DOF * ld;
     const int64_t sb = src_dsp[b];
@@ -130,44 +146,43 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
       for (int k = 0; k < 3; k++) r[3 + k] = k < ND ? xn[gs * 3 + k] : 0.0;
       r[6] = wts ? wts[gs] : 1.0;
     };
-    // Entries of target tt (wave-uniform) and this lane's source.
-    // old (subtract mode, optional): the entries' values loaded beforehand,
-    // indexed k1 * K0 + k0.
-    auto entries = [&](int tt, const double(&r)[7], double* Kc, bool active, const double* old) {
+    // Entries of target tt (wave-uniform) and this lane's source, written at
+    // this lane's column offset col (entries) of each row; ws = the weight
+    // times the kernel's scale factor.  old (subtract mode, optional): the
+    // entries' values loaded beforehand, indexed k1 * K0 + k0.
+    auto entries = [&](int tt, const double(&r)[7], double ws, int col, bool active, const double* old) {
       double u[K0][K1];
       const double n[3] = {r[3], r[4], r[5]};
-      kmatrix<KER, NEAR_NEWTON>(u, readlane_d(mx, tt) - r[0], readlane_d(my, tt) - r[1], readlane_d(mz, tt) - r[2],
-                                n, mu);
-      const double ws = r[6];
+      kmatrix<KER, NEAR_NEWTON>(u, tx[tt * 6 + 0] - r[0], tx[tt * 6 + 1] - r[1], tx[tt * 6 + 2] - r[2], n, mu);
       if constexpr (NDOT) {
-        const double n0 = readlane_d(mn[0], tt), n1 = readlane_d(mn[1], tt), n2 = readlane_d(mn[2], tt);
+        const double n0 = tx[tt * 6 + 3], n1 = tx[tt * 6 + 4], n2 = tx[tt * 6 + 5];
 #pragma unroll
         for (int k1 = 0; k1 < DOF; k1++) {
-          double* row = Kc + (int64_t)(tt * DOF + k1) * ld;
+          double* row = Ks + (int64_t)(tt * DOF + k1) * ld;
 #pragma unroll
           for (int k0 = 0; k0 < K0; k0++) {
             double v = 0.0;
-            v += (u[k0][k1 * 3 + 0] * scale) * ws * n0;
-            v += (u[k0][k1 * 3 + 1] * scale) * ws * n1;
-            v += (u[k0][k1 * 3 + 2] * scale) * ws * n2;
+            v += (u[k0][k1 * 3 + 0] * ws) * n0;
+            v += (u[k0][k1 * 3 + 1] * ws) * n1;
+            v += (u[k0][k1 * 3 + 2] * ws) * n2;
             if constexpr (SUB) {
-              if (active) row[k0] = (old ? old[k1 * K0 + k0] : row[k0]) - v;
+              if (active) row[col + k0] = (old ? old[k1 * K0 + k0] : row[col + k0]) - v;
             } else {
-              if (active) row[k0] = v;
+              if (active) row[col + k0] = v;
             }
           }
         }
       } else {
 #pragma unroll
         for (int k1 = 0; k1 < K1; k1++) {
-          double* row = Kc + (int64_t)(tt * K1 + k1) * ld;
+          double* row = Ks + (int64_t)(tt * K1 + k1) * ld;
 #pragma unroll
           for (int k0 = 0; k0 < K0; k0++) {
-            const double v = (u[k0][k1] * scale) * ws;
+            const double v = u[k0][k1] * ws;
             if constexpr (SUB) {
-              if (active) row[k0] = (old ? old[k1 * K0 + k0] : row[k0]) - v;
+              if (active) row[col + k0] = (old ? old[k1 * K0 + k0] : row[col + k0]) - v;
             } else {
-              if (active) row[k0] = v;
+              if (active) row[col + k0] = v;
             }
           }
         }
@@ -176,11 +191,15 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
     double cur[7], nxt[7];
     load(0, cur);
     for (int s0 = 0; s0 < sc; s0 += 64) {
-      double* Kc = Ks + (int64_t)min(s0 + lane, sc - 1) * K0;
+      // keep the per-row target reads inside the chunk loop (no registers
+      // held across chunks); no wait is issued for it
+      asm volatile("" ::: "memory");
+      const int col = min(s0 + lane, sc - 1) * K0;
+      const double ws = cur[6] * scale;  // loaded a chunk ago
       load(s0 + 64, nxt);  // clamped: in range past the last chunk too
       if constexpr (PIPE) {
 #pragma unroll
-        for (int t = 0; t < MAT_ST; t++) entries(min(t, nts - 1), cur, Kc, true, nullptr);
+        for (int t = 0; t < MAT_ST; t++) entries(min(t, nts - 1), cur, ws, col, true, nullptr);
       } else if constexpr (SUBPRE) {
         // The old entries of GT target rows are loaded together before any of
         // them is rewritten (rows past the stripe's last target re-read it and
@@ -194,15 +213,15 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
 #pragma unroll
             for (int k1 = 0; k1 < DOF; k1++)
 #pragma unroll
-              for (int k0 = 0; k0 < K0; k0++) old[t][k1 * K0 + k0] = Kc[(int64_t)(tt * DOF + k1) * ld + k0];
+              for (int k0 = 0; k0 < K0; k0++) old[t][k1 * K0 + k0] = Ks[(int64_t)(tt * DOF + k1) * ld + col + k0];
           }
 #pragma unroll
           for (int t = 0; t < GT; t++)
-            if (g0 + t < nts) entries(g0 + t, cur, Kc, active, old[t]);
+            if (g0 + t < nts) entries(g0 + t, cur, ws, col, active, old[t]);
         }
       } else {
         const bool active = s0 + lane < sc;
-        for (int t = 0; t < nts; t++) entries(t, cur, Kc, active, nullptr);
+        for (int t = 0; t < nts; t++) entries(t, cur, ws, col, active, nullptr);
       }
 #pragma unroll
       for (int i = 0; i < 7; i++) cur[i] = nxt[i];
