@@ -293,7 +293,7 @@ def plan_pair(kernel, B, stream, reps):
     return res, outs
 
 
-def graph_pair(kernel, B, stream, nsteps, seed=0):
+def graph_pair(kernel, B, stream, nsteps, seed=0, tune=True):
     _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
     nt, ns = B["r_trg"].size // 3, B["r_src"].size // 3
     geo = [S.DeviceArray.from_host(B[a]) for a in ("r_trg", "r_src", "n_src")]
@@ -304,7 +304,9 @@ def graph_pair(kernel, B, stream, nsteps, seed=0):
         vs = S.DeviceArray(ns * K0)
         vt = S.DeviceArray.from_host(np.zeros(nt * K1))
         S.gen_normal(seed, 25, vs, stream=stream)
-        spl, tpl, cap, _ = P.tune(vs, vt, stream=stream)
+        if tune:
+            P.tune(vs, vt, stream=stream)
+        spl, tpl, cap = P.shape()
         res[name + "_shape"] = {"sources_per_lane": spl, "targets_per_lane": tpl, "wg_per_cu_cap": cap}
         vt.upload(np.zeros(nt * K1))
         G = P.graph(nsteps, seed, 25, vs, vt)
@@ -323,6 +325,18 @@ def graph_pair(kernel, B, stream, nsteps, seed=0):
     res["applies"] = nsteps
     res["speedup"] = round(res["fp64"]["ms_per_graph"] / res["mixed"]["ms_per_graph"], 3)
     return res
+
+
+def calibration_copy(mb, stream):
+    """One device-to-device copy of mb MiB: the reference point that scales
+    the HBM byte counters of a profiled run."""
+    nb = mb << 20
+    a, b = S.DeviceArray(nb // 8), S.DeviceArray(nb // 8)
+    S.check(S.lib().sctl_amd_memcpy(b.ptr, a.ptr, nb, 2, stream))
+    S.synchronize(stream)
+    log("calibration copy: %d bytes device to device" % nb)
+    a.free()
+    b.free()
 
 
 def near_ops(kernel, B, stream, reps):
@@ -534,6 +548,25 @@ def other_configs(stream, quick=False, cpu=True, cpu_seconds=3.0):
     return out
 
 
+def config_only(name, stream):
+    """One configuration's FP64 and mixed legs at default launch shapes (for
+    traces and counters): c1-c3 one timed Eval each after a warm-up, c5 a
+    20-apply graph per precision."""
+    if name == "c1":
+        xt, xs, f = W.cube(20000, 20000, seed=0)
+        r, _ = dense_pair(S.LAPLACE3D_FXU, xt, xs, None, f, stream, reps=1)
+    elif name == "c2":
+        xt, xs, nrm, f = W.sphere(1 << 20, 1 << 20, seed=0)
+        r, _ = dense_pair(S.LAPLACE3D_DXU, xt, xs, nrm, f, stream, reps=1)
+    elif name == "c3":
+        xt, xs, f = W.clusters(524288, 524288, seed=0, k0=3)
+        r, _ = dense_pair(S.STOKES3D_FXU, xt, xs, None, f, stream, reps=1)
+    else:
+        B = W.torus_blocks(64, 32, seed=0, k0=3)
+        r = graph_pair(S.STOKES3D_DXU, B, stream, nsteps=20, tune=False)
+    return r
+
+
 HELMHOLTZ_MU = 3.0  # wavenumber of the Helmholtz lines (about half a wavelength across the unit cube)
 
 
@@ -584,6 +617,9 @@ def main():
                          "precision, at --plan-shape or the default shape, without tuning (for counters)")
     ap.add_argument("--plan-shape", default=None, metavar="SPL,TPL,CAP",
                     help="launch shape of the --batched-only plan (sources per lane, targets per lane, cap)")
+    ap.add_argument("--config-only", choices=["c1", "c2", "c3", "c5"], default=None,
+                    help="only the FP64 and mixed legs of one other configuration, untuned, without CPU or "
+                         "oracle (for kernel traces and counters)")
     ap.add_argument("--shape", default=None, metavar="TPL,CAP",
                     help="pin the mixed headline launch shape (targets per lane, workgroups-per-CU cap) "
                          "instead of tuning it")
@@ -594,9 +630,20 @@ def main():
         print("bench.py measures one GPU: the library has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.config_only:
+        stream = ctypes.c_void_p()
+        S.check(S.lib().sctl_amd_stream_create(ctypes.byref(stream)))
+        if args.copy_mb:
+            calibration_copy(args.copy_mb, stream)
+        print(json.dumps({"config_only": args.config_only, "result": config_only(args.config_only, stream)}),
+              flush=True)
+        return
+
     if args.near_only:
         stream = ctypes.c_void_p()
         S.check(S.lib().sctl_amd_stream_create(ctypes.byref(stream)))
+        if args.copy_mb:
+            calibration_copy(args.copy_mb, stream)
         log("config 4 blocks: near-field matrices and near lists")
         B = W.torus_blocks(64, 64, seed=0, k0=1)
         out = {"c4_near_matrix_laplace_dxu_4096": near_ops(S.LAPLACE3D_DXU, B, stream, reps=3),
@@ -644,11 +691,7 @@ def main():
     out = S.DeviceArray.from_host(np.zeros(N))
 
     if args.copy_mb:
-        nb = args.copy_mb << 20
-        a, b = S.DeviceArray(nb // 8), S.DeviceArray(nb // 8)
-        S.check(S.lib().sctl_amd_memcpy(b.ptr, a.ptr, nb, 2, stream))
-        S.synchronize(stream)
-        log("calibration copy: %d bytes device to device" % nb)
+        calibration_copy(args.copy_mb, stream)
     tuned = {}
     if args.shape:
         tpl, cap = (int(x) for x in args.shape.split(","))

@@ -2,10 +2,12 @@
  * sctl_amd.h -- C-ABI of the MI355X (gfx950) direct kernel summation.
  *
  * Drop-in device backend for SCTL's GenericKernel<uKernel>::Eval /
- * KernelMatrix (include/sctl/generic-kernel.hpp:99,112,124) for the four
- * micro-kernels Laplace3D_FxU, Laplace3D_DxU, Stokes3D_FxU, Stokes3D_DxU
- * (include/sctl/kernel_functions.hpp:32-79,125-218), and for the batches of
- * near-field blocks that BoundaryIntegralOp evaluates through them
+ * KernelMatrix (include/sctl/generic-kernel.hpp:99,112,124) for the sixteen
+ * micro-kernels of include/sctl/kernel_functions.hpp (Laplace3D FxU, DxU,
+ * FxdU, DxdU, Fxd2U; Stokes3D FxU, DxU, FxT, FSxU, FxUP; BiotSavart3D FxU,
+ * FxdU; Helmholtz3D FxU, DxU, FxdU; HelmholtzDiff3D FxdU; aliases at
+ * :669-684), and for the batches of near-field blocks that
+ * BoundaryIntegralOp evaluates through them
  * (include/sctl/boundary_integral.txx:1009-1032,1165-1240,1311-1381).
  *
  * Conventions

@@ -105,6 +105,21 @@ __device__ __forceinline__ void mix_far(float* p, const float* th, const float* 
   }
 }
 
+// Mid-range pair of the Laplace kernels (every target of the wave at least
+// 2H from the tile centre, so r >= H): one-part FP32 differences, accurate
+// to a few ulp of r there, and no mask.
+template <int KER, int NEWTON>
+__device__ __forceinline__ void mix_mid(float* p, const float* th, const float* r) {
+  const float dx = th[0] - r[0], dy = th[1] - r[1], dz = th[2] - r[2];
+  if constexpr (KER == LAP_FXU) {
+    const float f[1] = {r[4]};
+    interact<LAP_FXU, NEWTON, false>(p, dx, dy, dz, f, f);
+  } else {
+    const float one[1] = {1.0f}, nq[3] = {r[4], r[5], r[6]};
+    interact<LAP_DXU, NEWTON, false>(p, dx, dy, dz, nq, one);
+  }
+}
+
 // Near-field interaction: two-part differencing, r^2 == 0 masked.
 template <int KER, int NEWTON>
 __device__ __forceinline__ void mix_near(float* p, const float* th, const float* tl, const float* r, float mu) {
@@ -224,9 +239,13 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
     const double4 m = meta[tile];
     __syncthreads();
 
-    const float near_r2 = 9.0f * (float)m.w;
+    // |t'| >= 3H: far (the expanded r^2 of the Laplace kernels); 2H <= |t'|:
+    // mid (one-part FP32 differences, r >= H); else near.  The non-Laplace
+    // kernels difference in one part in both far and mid.
+    constexpr bool EXPANDED = KER == LAP_FXU || KER == LAP_DXU;
+    const float far_r2 = (EXPANDED ? 9.0f : 4.0f) * (float)m.w, mid_r2 = 4.0f * (float)m.w;
     float th[TPL][3], a[TPL][3], T[TPL], p[TPL][K1];
-    bool near = false;
+    bool notfar = false, near = false;
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
       th[j][0] = (float)(t64[j][0] - m.x);
@@ -235,19 +254,32 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
 #pragma unroll
       for (int k = 0; k < 3; k++) a[j][k] = -2.0f * th[j][k];
       T[j] = fmad(th[j][2], th[j][2], fmad(th[j][1], th[j][1], th[j][0] * th[j][0]));
-      near |= T[j] < near_r2;
+      notfar |= T[j] < far_r2;
+      near |= T[j] < mid_r2;
 #pragma unroll
       for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
     }
     const float* recs = reinterpret_cast<const float*>(lds);
     bool use_near = __builtin_amdgcn_ballot_w64(near) != 0;
-    if (!use_near) {
+    const bool use_mid = !use_near && EXPANDED && __builtin_amdgcn_ballot_w64(notfar) != 0;
+    if (use_mid) {
+      if constexpr (EXPANDED) {
+#pragma unroll 4
+        for (int i = 0; i < TILE; i++) {
+          const float* rec = recs + i * R;
+#pragma unroll
+          for (int j = 0; j < TPL; j++) mix_mid<KER, NEWTON>(p[j], th[j], rec);
+        }
+      }
+    } else if (!use_near) {
 #pragma unroll 4
       for (int i = 0; i < TILE; i++) {
         const float* rec = recs + i * R;
 #pragma unroll
         for (int j = 0; j < TPL; j++) mix_far<KER, NEWTON>(p[j], th[j], a[j], T[j], rec, mu);
       }
+    }
+    if (!use_near) {
       bool bad = false;
 #pragma unroll
       for (int j = 0; j < TPL; j++)

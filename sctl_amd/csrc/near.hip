@@ -69,6 +69,55 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+typedef unsigned int v2u __attribute__((__vector_size__(8)));
+
+// A wave-uniform pointer kept in scalar registers (the buffer resources built
+// from it below must be).
+__device__ __forceinline__ double* uniform_ptr(double* p) {
+  const uint64_t a = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+  return reinterpret_cast<double*>(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double bpermute_d(double v, int addr) {
+  const long long i = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(i & 0xffffffffLL));
+  const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(i >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// A row of K_near as a raw buffer (gfx9 resource word 3: 32-bit data format):
+// accesses at byte offsets outside [0, extent) are dropped (stores) or read 0
+// (loads) by the hardware's range check.  a = the row start's offset in
+// entries past a 128-byte line boundary.
+struct Row {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int a;
+};
+
+__device__ __forceinline__ Row row_of(double* Ks, int t, int nts, int64_t ld) {
+  // Built where it is used: kept opaque so that the resources of all rows
+  // are not hoisted out of the caller's loop (64 scalar registers).
+  uint64_t base = reinterpret_cast<uint64_t>(Ks);
+  asm volatile("" : "+s"(base), "+s"(ld), "+s"(nts));
+  double* p = reinterpret_cast<double*>(base) + (int64_t)min(t, nts - 1) * ld;
+  Row R;
+  R.a = (int)((reinterpret_cast<uintptr_t>(p) >> 3) & 15);
+  R.rsrc = __builtin_amdgcn_make_buffer_rsrc(p, 0, t < nts ? (int)(ld * 8) : 0, 0x00020000);
+  return R;
+}
+
+// K_near is written once and not read again before it leaves the caches
+// (13.5 GB and more), so the stores are nontemporal.
+__device__ __forceinline__ void buffer_store_d(double v, __amdgpu_buffer_rsrc_t r, int off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, off, 0, 2);
+}
+
+__device__ __forceinline__ double buffer_load_d(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
 // KNearSubtract / store of one layout's blocks.  NDOT: contract the KDIM1
 // outputs with the target normal (KDIM1 == 3 -> one row per target).
 // A wave owns one stripe (MAT_ST targets x all sources of a block), a
@@ -76,19 +125,25 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // the wave walks the sources 64 at a time (one per lane) and, per chunk,
 // writes every target row's 64*KDIM0 entries.
 //
+// Rows of one entry per source (the Laplace kernels, and any kernel whose
+// outputs are contracted to one per target) are written as whole 128-byte
+// lines: a row segment that starts inside a line would leave that line
+// half-written until the next chunk, by when it has usually left L2 and the
+// memory must read it back to merge the halves.  See the W == 1 branch.
+//
 // Stores and loads share one completion counter on this GPU, so a wait for
 // a chunk's source records would also wait for the previous chunk's stores.
-// The store mode therefore prefetches the next chunk's records before the
-// target loop and, for rows of at most 3 entries per source, issues a fixed
-// number of stores per chunk: MAT_ST target rows, the rows past the
-// stripe's last target repeating that target, and the lanes past the
-// block's last source repeating that source; a repeat writes the identical
-// value to the same entry, so the wait for the prefetched records can leave
-// the chunk's stores in flight.  The subtract mode cannot repeat an update
-// and keeps its stores exact, as do the wider kernels; on rows of one entry
-// per source it loads the old entries of 16 target rows before rewriting any
-// of them, where a row-by-row update would wait for the previous row's store
-// before each load.
+// Both modes therefore prefetch the next chunk's records before the target
+// loop and issue a fixed number of stores per chunk (MAT_ST rows), so that
+// the wait for the prefetched records can leave the chunk's stores in
+// flight: the W == 1 branch through the range check of its buffer stores,
+// the wider store mode (at most 3 entries per source) by repeating the
+// stripe's last target and the block's last source (a repeat writes the
+// identical value to the same entry).  The wider subtract mode cannot
+// repeat an update and keeps its stores exact; the W == 1 subtract loads the
+// old entries of all MAT_ST rows before rewriting any of them, where a
+// row-by-row update would wait for the previous row's store before each
+// load.
 template <int KER, bool NDOT, bool SUB>
 __global__ void __launch_bounds__(NWG)
 k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ tile_dsp, int64_t ntiles,
@@ -100,19 +155,18 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
   using Kt = Ker<KER>;
   constexpr int K0 = Kt::K0, K1 = Kt::K1, ND = Kt::ND;
   constexpr int DOF = NDOT ? K1 / 3 : K1;
-  // The pipelined store (16 unrolled target rows) for narrow rows only: the
-  // unrolled rows of wider kernels would cost their occupancy.
-  constexpr bool PIPE = !SUB && K0 * DOF <= 3;
-  // The grouped subtract for rows of one entry per source (W), GT rows per
-  // group; wider rows would cost their occupancy.
-  constexpr int W = K0 * DOF, GT = 16;
-  constexpr bool SUBPRE = SUB && W == 1;
+  // W: entries per (target, source).  The pipelined store (16 unrolled target
+  // rows) of the wider rows only up to 3 entries: more unrolled rows would
+  // cost the kernel its occupancy.
+  constexpr int W = K0 * DOF;
+  constexpr bool PIPE = !SUB && W <= 3;
   static_assert(!NDOT || K1 % 3 == 0, "the target-normal contraction needs 3 outputs per row");
   const int lane = threadIdx.x & 63;
   __shared__ double tslot_all[(NWG / 64) * MAT_ST * 6];
   double* tslot = tslot_all + (threadIdx.x >> 6) * (MAT_ST * 6);
   const int64_t nwaves = (int64_t)gridDim.x * (NWG / 64);
-  for (int64_t tile = (int64_t)blockIdx.x * (NWG / 64) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
+  const int64_t first = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (NWG / 64) + (threadIdx.x >> 6)));
+  for (int64_t tile = first; tile < ntiles; tile += nwaves) {
     const int b = __builtin_amdgcn_readfirstlane(tile_blk[tile]);
     const int t0 = (int)(tile - tile_dsp[b]) * MAT_ST;
     const int sc = src_cnt[b];
@@ -133,7 +187,7 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
     wave_sync();
     const double* tx = tslot;
     const int64_t ld = (int64_t)sc * K0;
-    double* Ks = K + mat_off[b] + (int64_t)t0 * DOF * ld;
+    double* Ks = uniform_ptr(K + mat_off[b] + (int64_t)t0 * DOF * ld);
     const int64_t sb = src_dsp[b];
     // This lane's source record of the chunk at s0: position, normal, weight
     // (the column clamped to the block's last source).
@@ -148,9 +202,8 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
     };
     // Entries of target tt (wave-uniform) and this lane's source, written at
     // this lane's column offset col (entries) of each row; ws = the weight
-    // times the kernel's scale factor.  old (subtract mode, optional): the
-    // entries' values loaded beforehand, indexed k1 * K0 + k0.
-    auto entries = [&](int tt, const double(&r)[7], double ws, int col, bool active, const double* old) {
+    // times the kernel's scale factor (rows of more than one entry per source).
+    auto entries = [&](int tt, const double(&r)[7], double ws, int col, bool active) {
       double u[K0][K1];
       const double n[3] = {r[3], r[4], r[5]};
       kmatrix<KER, NEAR_NEWTON>(u, tx[tt * 6 + 0] - r[0], tx[tt * 6 + 1] - r[1], tx[tt * 6 + 2] - r[2], n, mu);
@@ -166,7 +219,7 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
             v += (u[k0][k1 * 3 + 1] * ws) * n1;
             v += (u[k0][k1 * 3 + 2] * ws) * n2;
             if constexpr (SUB) {
-              if (active) row[col + k0] = (old ? old[k1 * K0 + k0] : row[col + k0]) - v;
+              if (active) row[col + k0] = row[col + k0] - v;
             } else {
               if (active) row[col + k0] = v;
             }
@@ -180,7 +233,7 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
           for (int k0 = 0; k0 < K0; k0++) {
             const double v = u[k0][k1] * ws;
             if constexpr (SUB) {
-              if (active) row[col + k0] = (old ? old[k1 * K0 + k0] : row[col + k0]) - v;
+              if (active) row[col + k0] = row[col + k0] - v;
             } else {
               if (active) row[col + k0] = v;
             }
@@ -188,8 +241,71 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
         }
       }
     };
+    // The entry of target tt (wave-uniform) and this lane's source when a row
+    // holds one entry per source (W == 1), computed as entries() does.
+    auto value1 = [&](int tt, const double(&r)[7], double ws) -> double {
+      double u[K0][K1];
+      const double n[3] = {r[3], r[4], r[5]};
+      kmatrix<KER, NEAR_NEWTON>(u, tx[tt * 6 + 0] - r[0], tx[tt * 6 + 1] - r[1], tx[tt * 6 + 2] - r[2], n, mu);
+      if constexpr (NDOT) {
+        double v = 0.0;
+        v += (u[0][0] * ws) * tx[tt * 6 + 3];
+        v += (u[0][1] * ws) * tx[tt * 6 + 4];
+        v += (u[0][2] * ws) * tx[tt * 6 + 5];
+        return v;
+      } else {
+        return u[0][0] * ws;
+      }
+    };
     double cur[7], nxt[7];
     load(0, cur);
+    if constexpr (W == 1) {
+      // One entry per source and row: the row segments are written as whole
+      // 128-byte lines.  Row t starts a_t entries past a line boundary, so
+      // in the chunk at s0 lane l writes column s0 - a_t + l: the chunk's
+      // entries rotated by a_t lanes (ds_bpermute), the first a_t lanes
+      // taking the previous chunk's last a_t entries from the wave's LDS
+      // carry slots, and a final pass writes the last carry.  The stores go
+      // through a buffer resource of the row's extent, so columns before
+      // the row start (first chunk), past its end and rows past the stripe
+      // (extent 0) are dropped by the hardware's range check: each chunk
+      // issues exactly MAT_ST row stores with no branch.
+      __shared__ double carry_all[(NWG / 64) * MAT_ST * 16];
+      double* cs = carry_all + (threadIdx.x >> 6) * (MAT_ST * 16);
+      const int nch = (sc + 63) >> 6;
+      for (int c = 0; c <= nch; c++) {
+        const int s0 = c * 64;
+        asm volatile("" ::: "memory");
+        const bool tail = c == nch;  // wave-uniform: the carry flush only
+        const double ws = cur[6] * scale;
+        if (!tail) load(min(s0 + 64, (nch - 1) * 64), nxt);
+        double old[MAT_ST];
+        if constexpr (SUB) {
+#pragma unroll
+          for (int t = 0; t < MAT_ST; t++) {
+            const Row R = row_of(Ks, t, nts, ld);
+            old[t] = buffer_load_d(R.rsrc, (s0 - R.a + lane) * 8);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < MAT_ST; t++) {
+          // per-row values derived here, not kept across the chunk loop
+          int nr = nts;
+          asm volatile("" : "+s"(nr));
+          const Row R = row_of(Ks, t, nr, ld);
+          const double v = tail ? 0.0 : value1(min(t, nr - 1), cur, ws);
+          const double rot = bpermute_d(v, ((lane - R.a) & 63) << 2);
+          const double prev = cs[t * 16 + (lane & 15)];
+          double out = lane < R.a ? prev : rot;
+          if (lane < 16) cs[t * 16 + lane] = rot;
+          if constexpr (SUB) out = old[t] - out;
+          buffer_store_d(out, R.rsrc, (s0 - R.a + lane) * 8);
+        }
+#pragma unroll
+        for (int i = 0; i < 7; i++) cur[i] = nxt[i];
+      }
+      continue;
+    }
     for (int s0 = 0; s0 < sc; s0 += 64) {
       // keep the per-row target reads inside the chunk loop (no registers
       // held across chunks); no wait is issued for it
@@ -199,29 +315,10 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
       load(s0 + 64, nxt);  // clamped: in range past the last chunk too
       if constexpr (PIPE) {
 #pragma unroll
-        for (int t = 0; t < MAT_ST; t++) entries(min(t, nts - 1), cur, ws, col, true, nullptr);
-      } else if constexpr (SUBPRE) {
-        // The old entries of GT target rows are loaded together before any of
-        // them is rewritten (rows past the stripe's last target re-read it and
-        // are not rewritten), one memory round trip per group, not per row.
-        const bool active = s0 + lane < sc;
-        for (int g0 = 0; g0 < nts; g0 += GT) {
-          double old[GT][W];
-#pragma unroll
-          for (int t = 0; t < GT; t++) {
-            const int tt = min(g0 + t, nts - 1);
-#pragma unroll
-            for (int k1 = 0; k1 < DOF; k1++)
-#pragma unroll
-              for (int k0 = 0; k0 < K0; k0++) old[t][k1 * K0 + k0] = Ks[(int64_t)(tt * DOF + k1) * ld + col + k0];
-          }
-#pragma unroll
-          for (int t = 0; t < GT; t++)
-            if (g0 + t < nts) entries(g0 + t, cur, ws, col, active, old[t]);
-        }
+        for (int t = 0; t < MAT_ST; t++) entries(min(t, nts - 1), cur, ws, col, true);
       } else {
         const bool active = s0 + lane < sc;
-        for (int t = 0; t < nts; t++) entries(t, cur, ws, col, active, nullptr);
+        for (int t = 0; t < nts; t++) entries(t, cur, ws, col, active);
       }
 #pragma unroll
       for (int i = 0; i < 7; i++) cur[i] = nxt[i];
@@ -474,6 +571,9 @@ int sctl_amd_plan_near_layout(sctl_amd_plan p, const int64_t* blk_off, int trg_n
   std::vector<int64_t> off(nb), size(nb);
   int64_t len = 0;
   for (int64_t b = 0; b < nb; b++) {
+    // rows are addressed with 32-bit byte offsets
+    SCTL_CHECK_ARG((int64_t)p->src_cnt[b] * p->K0 < ((int64_t)1 << 28),
+                   "block " + std::to_string(b) + " has too many sources for a near-field matrix row");
     size[b] = p->trg_cnt[b] * dof * p->src_cnt[b] * p->K0;
     if (blk_off) {
       off[b] = blk_off[b] < 0 ? -1 : blk_off[b];

@@ -101,6 +101,36 @@ def test_dense_shapes_from_database_match_oracle(gpu, tmp_path, prec, tol):
         gpu.tune_db_path(None)
 
 
+@pytest.mark.parametrize("k", range(16))
+def test_mixed_every_targets_per_lane_matches_oracle(gpu, tmp_path, k):
+    """Mixed precision at each targets-per-lane count 1, 2, 4 (a kernel with
+    more outputs per target caps it lower) for every micro-kernel, on
+    uneven sizes with coincident points.  FP32 pairwise terms are accurate
+    relative to their own size, so the error is bounded relative to the sum
+    of the terms' magnitudes, |K|^T |f| from the oracle's kernel matrix."""
+    rng = np.random.default_rng(40 + k)
+    _, K0, _, _, _ = oracle.dims(k)
+    nt, ns = 801, 703
+    xs, xt = rng.random(ns * 3), rng.random(nt * 3)
+    xt[:90] = xs[:90]
+    n = rng.standard_normal(ns * 3)
+    n = (n.reshape(-1, 3) / np.linalg.norm(n.reshape(-1, 3), axis=1)[:, None]).ravel()
+    v = rng.uniform(-1, 1, ns * K0)
+    mu = 1.0 if k >= 12 else None
+    ref = oracle.eval(k, xt, xs, n, v, ctx=mu)
+    magnitude = np.max(np.abs(oracle.kernel_matrix(k, xt, xs, n, ctx=mu)).T @ np.abs(v))
+    gpu.tune_db_path(str(tmp_path / "tpl.db"))
+    try:
+        for tpl in (1, 2, 4):
+            gpu.set_dense_shape(k, nt, ns, tpl, 0, precision=gpu.MIXED)
+            out = gpu.eval(k, xt, xs, n, v, precision=gpu.MIXED, ctx=mu)
+            assert np.isfinite(out).all()
+            e = np.max(np.abs(out - ref)) / magnitude
+            assert e < 1e-6, (tpl, e)
+    finally:
+        gpu.tune_db_path(None)
+
+
 def test_tune_eval_records_choice(gpu, tmp_path):
     gpu.tune_db_path(str(tmp_path / "t.db"))
     try:
