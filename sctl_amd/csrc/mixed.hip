@@ -392,10 +392,12 @@ int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, Mix
   g->ntiles = (ns + TILE - 1) / TILE;
   SCTL_CHECK_ARG(g->ntiles <= 0x7fffffffLL, "too many sources for one launch");
   g->Q = layout_q(ker);
+  // Enough workgroups for 32 per CU (one tile per split at the least): a
+  // small problem (a few hundred tiles) still fills the GPU.
   constexpr int64_t SPLIT_TILES = 256;
-  const int64_t want = 8LL * std::max(dev.num_cus, 1);
+  const int64_t want = 32LL * std::max(dev.num_cus, 1);
   int64_t nsplit = g->tb < want ? (want + g->tb - 1) / g->tb : 1;
-  nsplit = std::min<int64_t>(std::min<int64_t>(nsplit, std::max<int64_t>(1, g->ntiles / 4)), 65535);
+  nsplit = std::min<int64_t>(std::min<int64_t>(nsplit, g->ntiles), 65535);
   nsplit = std::min<int64_t>(std::max<int64_t>(nsplit, (g->ntiles + SPLIT_TILES - 1) / SPLIT_TILES), 65535);
   g->tps = (g->ntiles + nsplit - 1) / nsplit;
   g->nsplit = (g->ntiles + g->tps - 1) / g->tps;

@@ -17,6 +17,7 @@ namespace sctl_amd {
 namespace {
 
 constexpr int BB_WG = 256;
+constexpr int64_t BB_PER_BLK = 1024;  // points per bounding-box workgroup: 4 per thread
 
 // Per-workgroup min/max of a chunk of points: part[blk*6 + {0..2 min, 3..5
 // max}].  Workgroups [0, blk_a) take chunks of set a, the others of set b.
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(BB_WG) k_split_perm(const uint32_t* __restrict
 }  // namespace
 
 int bounding_box(const double* a, int64_t na, const double* b, int64_t nb, double* box6, hipStream_t stream) {
-  const int64_t per_blk = 8192;
+  const int64_t per_blk = BB_PER_BLK;
   const int blk_a = (int)((na + per_blk - 1) / per_blk), blk_b = (int)((nb + per_blk - 1) / per_blk);
   SCTL_CHECK_ARG(blk_a + blk_b > 0, "bounding box of no points");
   Scratch part;
@@ -155,7 +156,7 @@ struct MortonLayout {
 };
 MortonLayout morton_layout(int64_t na, int64_t nb) {
   MortonLayout m;
-  const int64_t n = na + nb, nblk = (na + 8191) / 8192 + (nb + 8191) / 8192;
+  const int64_t n = na + nb, nblk = (na + BB_PER_BLK - 1) / BB_PER_BLK + (nb + BB_PER_BLK - 1) / BB_PER_BLK;
   Carve c;
   m.part = c.take(sizeof(double) * 6 * nblk);
   m.boxes = c.take(sizeof(double) * 12);
@@ -180,7 +181,7 @@ int morton_order(const double* a, int64_t na, uint32_t* perm_a, const double* b,
   if (n <= 0) return SCTL_AMD_SUCCESS;
   SCTL_CHECK_ARG(work, "no workspace for the Morton order");
   SCTL_CHECK_ARG(n < ((int64_t)1 << 32), "too many points for 32-bit indices");
-  const int64_t per_blk = 8192;
+  const int64_t per_blk = BB_PER_BLK;
   const int blk_a = (int)((na + per_blk - 1) / per_blk), blk_b = (int)((nb + per_blk - 1) / per_blk);
   const MortonLayout m = morton_layout(na, nb);
   char* w = static_cast<char*>(work);
