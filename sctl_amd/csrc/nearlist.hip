@@ -22,12 +22,15 @@
 //      inside the grown box and tests them 64 at a time exactly against the
 //      element's nodes (one ballot per candidate and 64 nodes, each node
 //      chunk loaded once per batch, first hit wins);
-//   3. a count pass sizes the output, a fill pass writes (element, target)
-//      keys, which are radix sorted; a second sort by (target, pair) gives
-//      the scatter index.
+//   3. a first pass counts the candidates inside the grown boxes only (an
+//      upper bound of the pairs, without the node tests), the second tests
+//      them and appends the (element, target) keys of the near ones through
+//      an atomic counter; the keys are radix sorted (so the list does not
+//      depend on the order of the appends), each element's range is found
+//      by binary search, and a second sort by (target, pair) gives the
+//      scatter index.
 // Each element is one workgroup of 4 waves that take turns over its
-// 64-candidate groups (counts are kept per wave, so the fill is
-// deterministic); elements are launched largest first.
+// 64-candidate groups; elements are launched largest first.
 // Every target within a node's radius lies in the grown box, hence in one of
 // the cells (the cell range is computed with the same quantisation as the
 // target keys), so the list is exact.
@@ -149,8 +152,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wave per element.  FILL = false: counts[e] = number of near targets;
-// FILL = true: writes (e << 32 | t) at out[offsets[e] ...].
+// One workgroup per element.  FILL = false: counts[e * NWAVE + w] = the
+// candidates inside the element's grown box that wave w fetched (no node
+// tests); FILL = true: appends (e << 32 | t) of each near target at
+// out[atomicAdd(counter)], cap entries at most.
 // Lanes hold the element's nodes (64 per chunk, the first NODE_REGS chunks in
 // registers).  Candidate targets are fetched 64 at a time; those inside the
 // grown box are appended to a per-wave LDS queue, and every 64 queued
@@ -164,7 +169,8 @@ k_nl_elements(const uint32_t* __restrict__ eorder, const int64_t* __restrict__ e
               const int64_t* __restrict__ edsp, int64_t nsrc, const double* __restrict__ xs,
               const double* __restrict__ rad, int64_t nt, const double* __restrict__ xt,
               const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ sidx, const double* __restrict__ box,
-              int64_t* __restrict__ counts, const int64_t* __restrict__ offsets, uint64_t* __restrict__ out) {
+              int64_t* __restrict__ counts, unsigned long long* __restrict__ counter, int64_t cap,
+              uint64_t* __restrict__ out) {
   __shared__ Cand queue_all[NWAVE][QCAP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   Cand* queue = queue_all[w];
@@ -248,13 +254,19 @@ k_nl_elements(const uint32_t* __restrict__ eorder, const int64_t* __restrict__ e
           }
         }
         if constexpr (FILL) {
-          if ((nearm >> lane) & 1ULL) {
-            const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(nearm >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)nearm, 0));
-            out[offsets[e * NWAVE + w] + count + below] = ((uint64_t)e << 32) | queue[lane].t;
+          const int nn = __builtin_popcountll(nearm);
+          if (nn) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(counter, (unsigned long long)nn);
+            base = __shfl(base, 0, 64);
+            if ((nearm >> lane) & 1ULL) {
+              const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(nearm >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)nearm, 0));
+              const int64_t pos = (int64_t)base + below;
+              if (pos < cap) out[pos] = ((uint64_t)e << 32) | queue[lane].t;
+            }
           }
         }
-        count += __builtin_popcountll(nearm);
       };
 
       if (rmax > 0) {
@@ -301,6 +313,10 @@ k_nl_elements(const uint32_t* __restrict__ eorder, const int64_t* __restrict__ e
                            mz <= ghi[2];
                 }
                 const uint64_t in = __builtin_amdgcn_ballot_w64(inside);
+                if constexpr (!FILL) {
+                  count += __builtin_popcountll(in);
+                  continue;
+                }
                 if (inside) {
                   const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(in >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)in, 0));
@@ -340,17 +356,18 @@ k_nl_elements(const uint32_t* __restrict__ eorder, const int64_t* __restrict__ e
   }
 }
 
-// near_elem_cnt[e] = sum of its waves' counts, near_elem_dsp[e] = the
-// exclusive scan at its first wave.
+// near_elem_dsp[e] / near_elem_cnt[e]: the range of element e in the sorted
+// keys (e << 32 | t), found by binary search (dsp = the pairs of the
+// elements before e, also for an element without pairs).
 __global__ void __launch_bounds__(NL_WG)
-k_nl_elem_totals(int64_t nelem, const int64_t* __restrict__ cnt_w, const int64_t* __restrict__ dsp_w,
-                 int64_t* __restrict__ ecnt_out, int64_t* __restrict__ edsp_out) {
+k_nl_elem_ranges(int64_t nelem, const uint64_t* __restrict__ keys, int64_t n, int64_t* __restrict__ ecnt_out,
+                 int64_t* __restrict__ edsp_out) {
   const int64_t e = (int64_t)blockIdx.x * NL_WG + threadIdx.x;
   if (e >= nelem) return;
-  int64_t c = 0;
-  for (int k = 0; k < NWAVE; k++) c += cnt_w[e * NWAVE + k];
-  ecnt_out[e] = c;
-  edsp_out[e] = dsp_w[e * NWAVE];
+  const int64_t lo = lower_bound_u64(keys, n, (uint64_t)e << 32);
+  const int64_t hi = lower_bound_u64(keys, n, (uint64_t)(e + 1) << 32);
+  ecnt_out[e] = hi - lo;
+  edsp_out[e] = lo;
 }
 
 // Elements ordered by node count, largest first (the order only schedules).
@@ -472,7 +489,8 @@ int build_list(sctl_amd_near_list_s* L, const double* xt, int64_t nt, const doub
                                          oidx_in.as<uint32_t>(), oidx.as<uint32_t>(), (size_t)nelem, 0, 32, s));
   tmp.release();
 
-  // count pass: per (element, wave)
+  // Count pass: candidates inside the grown boxes, per (element, wave): an
+  // upper bound of the near pairs.
   const int64_t nw = nelem * NWAVE;
   Scratch cnt_w, dsp_w;
   SCTL_TRY(cnt_w.alloc(sizeof(int64_t) * nw, s));
@@ -480,7 +498,7 @@ int build_list(sctl_amd_near_list_s* L, const double* xt, int64_t nt, const doub
   const dim3 egrid((unsigned)nelem);
   hipLaunchKernelGGL(k_nl_elements<false>, egrid, dim3(NL_WG), 0, s, oidx.as<uint32_t>(), ecnt, edsp, ns, xs, rad,
                      nt, xt, keys.as<uint64_t>(), idx.as<uint32_t>(), box.as<double>(), cnt_w.as<int64_t>(), nullptr,
-                     nullptr);
+                     (int64_t)0, nullptr);
   SCTL_HIP_TRY(hipGetLastError());
   SCTL_HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, cnt_w.as<int64_t>(), dsp_w.as<int64_t>(), (int64_t)0,
                                        (size_t)nw, rocprim::plus<int64_t>(), s));
@@ -488,27 +506,36 @@ int build_list(sctl_amd_near_list_s* L, const double* xt, int64_t nt, const doub
   SCTL_HIP_TRY(rocprim::exclusive_scan(tmp.as<void>(), bytes, cnt_w.as<int64_t>(), dsp_w.as<int64_t>(), (int64_t)0,
                                        (size_t)nw, rocprim::plus<int64_t>(), s));
   tmp.release();
-  hipLaunchKernelGGL(k_nl_elem_totals, dim3(grid_of(nelem)), dim3(NL_WG), 0, s, nelem, cnt_w.as<int64_t>(),
-                     dsp_w.as<int64_t>(), L->d_ecnt, L->d_edsp);
-  SCTL_HIP_TRY(hipGetLastError());
   int64_t last[2] = {0, 0};
   SCTL_HIP_TRY(hipMemcpyAsync(&last[0], cnt_w.as<int64_t>() + nw - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   SCTL_HIP_TRY(hipMemcpyAsync(&last[1], dsp_w.as<int64_t>() + nw - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   SCTL_HIP_TRY(hipStreamSynchronize(s));
-  const int64_t n = last[0] + last[1];
-  L->n_near = n;
-  if (n == 0) return SCTL_AMD_SUCCESS;
-  SCTL_CHECK_ARG(n < ((int64_t)1 << 32), "more than 2^32-1 near pairs");
+  cnt_w.release();
+  dsp_w.release();
+  const int64_t cap = last[0] + last[1];
 
-  Scratch pairs_in, pairs, key2_in, key2;
-  SCTL_TRY(pairs_in.alloc(sizeof(uint64_t) * n, s));
-  SCTL_TRY(pairs.alloc(sizeof(uint64_t) * n, s));
-  hipLaunchKernelGGL(k_nl_elements<true>, egrid, dim3(NL_WG), 0, s, oidx.as<uint32_t>(), ecnt, edsp, ns, xs, rad,
-                     nt, xt, keys.as<uint64_t>(), idx.as<uint32_t>(), box.as<double>(), nullptr, dsp_w.as<int64_t>(),
-                     pairs_in.as<uint64_t>());
-  SCTL_HIP_TRY(hipGetLastError());
+  // Fill pass: the node tests, near pairs appended in any order.
+  int64_t n = 0;
+  Scratch pairs_in, pairs, key2_in, key2, counter;
+  if (cap > 0) {
+    SCTL_TRY(pairs_in.alloc(sizeof(uint64_t) * cap, s));
+    SCTL_TRY(counter.alloc(sizeof(int64_t), s));
+    hipLaunchKernelGGL(k_nl_zero, dim3(1), dim3(NL_WG), 0, s, counter.as<int64_t>(), (int64_t)1);
+    hipLaunchKernelGGL(k_nl_elements<true>, egrid, dim3(NL_WG), 0, s, oidx.as<uint32_t>(), ecnt, edsp, ns, xs, rad,
+                       nt, xt, keys.as<uint64_t>(), idx.as<uint32_t>(), box.as<double>(), nullptr,
+                       reinterpret_cast<unsigned long long*>(counter.as<int64_t>()), cap, pairs_in.as<uint64_t>());
+    SCTL_HIP_TRY(hipGetLastError());
+    SCTL_HIP_TRY(hipMemcpyAsync(&n, counter.as<int64_t>(), sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SCTL_HIP_TRY(hipStreamSynchronize(s));
+  }
+  SCTL_CHECK_ARG(n <= cap, "near-pair count exceeds its bound (internal error)");
   keys.release();
   idx.release();
+  L->n_near = n;
+  if (n == 0) return SCTL_AMD_SUCCESS;  // d_ecnt, d_edsp were zeroed
+  SCTL_CHECK_ARG(n < ((int64_t)1 << 32), "more than 2^32-1 near pairs");
+
+  SCTL_TRY(pairs.alloc(sizeof(uint64_t) * n, s));
   const int ebits = 32 + bits_for(nelem);
   SCTL_HIP_TRY(rocprim::radix_sort_keys(nullptr, bytes, pairs_in.as<uint64_t>(), pairs.as<uint64_t>(), (size_t)n, 0,
                                         ebits, s));
@@ -516,6 +543,10 @@ int build_list(sctl_amd_near_list_s* L, const double* xt, int64_t nt, const doub
   SCTL_HIP_TRY(rocprim::radix_sort_keys(tmp.as<void>(), bytes, pairs_in.as<uint64_t>(), pairs.as<uint64_t>(),
                                         (size_t)n, 0, ebits, s));
   tmp.release();
+  pairs_in.release();
+  hipLaunchKernelGGL(k_nl_elem_ranges, dim3(grid_of(nelem)), dim3(NL_WG), 0, s, nelem, pairs.as<uint64_t>(), n,
+                     L->d_ecnt, L->d_edsp);
+  SCTL_HIP_TRY(hipGetLastError());
 
   SCTL_TRY(dev_alloc(&L->d_xtn, n * 3));
   if (xn) SCTL_TRY(dev_alloc(&L->d_xntn, n * 3));
