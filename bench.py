@@ -203,7 +203,7 @@ def kernel_matrix_end_to_end(kernel, B, nblocks, threads=8):
             one(b, bufs[w])
 
     with ThreadPoolExecutor(threads) as ex:
-        list(ex.map(lambda w: one(w, bufs[w]), range(threads)))  # warm-up (per-thread streams, pool)
+        list(ex.map(worker, range(threads)))  # warm-up pass: per-thread streams, device pool, pinned staging
         t = time.perf_counter()
         list(ex.map(worker, range(threads)))
         dt = time.perf_counter() - t
@@ -798,6 +798,9 @@ def main():
         "pct_fp32_valu_peak": round(100 * step_tflops / PEAK_FP32_TFLOPS, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic(N),
+                     "traffic_source": "not measured in this run: FETCH_SIZE/WRITE_SIZE of a separate rocprofv3 "
+                                       "run of the same launch, copy-calibrated, profiles/" +
+                                       os.path.basename(TRAFFIC_FILE),
                      "kernel": headline_kernel, "mean_launch_ms": round(kern_ms, 3),
                      "flop_per_interaction": FLOP_PER_INT[K]},
         "autotuned_shapes": tuned or None,
