@@ -139,12 +139,28 @@ __device__ __forceinline__ void mix_near(float* p, const float* th, const float*
   }
 }
 
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = fminf(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
 // One workgroup per tile: tile centre and H^2, then the records, sources in
-// the order perm[].  meta[tile] = {c_x, c_y, c_z, H^2}.
+// the order perm[].  meta[tile] = {c_x, c_y, c_z, H^2}; sub[4 tile + q] =
+// {b_x, b_y, b_z, h^2} of sub-tile q (sources 64q .. 64q + 63, one wave):
+// the centre of its FP32 offsets' bounding box, relative to the tile centre,
+// and the squared radius of its offsets about that centre ({0, 0, 0, 0} for
+// a sub-tile of padding only).
 template <int KER>
 __global__ void __launch_bounds__(WG)
 k_pack_tiles(const double* __restrict__ xs, const double* __restrict__ xn, const double* __restrict__ fs, int64_t ns,
-             const uint32_t* __restrict__ perm, float4* __restrict__ packed, double4* __restrict__ meta) {
+             const uint32_t* __restrict__ perm, float4* __restrict__ packed, double4* __restrict__ meta,
+             float4* __restrict__ sub) {
   using K = Ker<KER>;
   constexpr int K0 = K::K0, ND = K::ND, Q = MixLayout<KER>::Q;
   static_assert(TILE == WG, "one source per lane");
@@ -198,6 +214,21 @@ k_pack_tiles(const double* __restrict__ xs, const double* __restrict__ xn, const
     __syncthreads();
   }
   if (tid == 0) meta[blockIdx.x] = make_double4(c[0], c[1], c[2], (double)hred[0] * 1.0000001);
+  {
+    float b[3], bh2 = 0.0f;
+    const bool any = __builtin_amdgcn_ballot_w64(ok) != 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float lo = wave_minf(ok ? rec[k] : INFINITY), hi = wave_maxf(ok ? rec[k] : -INFINITY);
+      b[k] = any ? 0.5f * (lo + hi) : 0.0f;
+    }
+    if (ok) {
+      const float dx = rec[0] - b[0], dy = rec[1] - b[1], dz = rec[2] - b[2];
+      bh2 = dx * dx + dy * dy + dz * dz;
+    }
+    bh2 = wave_maxf(bh2);
+    if ((tid & 63) == 0) sub[(int64_t)blockIdx.x * (WG / 64) + (tid >> 6)] = make_float4(b[0], b[1], b[2], bh2 * 1.0001f);
+  }
   float4* dst = packed + ((int64_t)blockIdx.x * TILE + tid) * Q;
 #pragma unroll
   for (int q = 0; q < Q; q++) dst[q] = make_float4(rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], rec[4 * q + 3]);
@@ -206,8 +237,8 @@ k_pack_tiles(const double* __restrict__ xs, const double* __restrict__ xn, const
 template <int KER, int TPL, int NEWTON>
 __global__ void __launch_bounds__(WG)
 k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restrict__ tperm,
-             const float4* __restrict__ packed, const double4* __restrict__ meta, int64_t ntiles,
-             int64_t tiles_per_split, double* __restrict__ out, int direct, double scale, double ctx) {
+             const float4* __restrict__ packed, const double4* __restrict__ meta, const float4* __restrict__ sub,
+             int64_t ntiles, int64_t tiles_per_split, double* __restrict__ out, int direct, double scale, double ctx) {
   using K = Ker<KER>;
   constexpr int K1 = K::K1, Q = MixLayout<KER>::Q, R = 4 * Q;
   __shared__ float4 lds[TILE * Q];
@@ -239,30 +270,56 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
     const double4 m = meta[tile];
     __syncthreads();
 
-    // |t'| >= 3H: far (the expanded r^2 of the Laplace kernels); 2H <= |t'|:
-    // mid (one-part FP32 differences, r >= H); else near.  The non-Laplace
-    // kernels difference in one part in both far and mid.
+    // A wave whose targets are all |t'| >= 3H from the tile centre takes the
+    // far form over the whole tile (the expanded r^2 of the Laplace kernels,
+    // one-part differences for the others; the latter from |t'| >= 2H).
+    // Otherwise each 64-source sub-tile q (centre b_q, radius h_q about it)
+    // is classified on its own, with X = |t'| + H the scale of the rounding
+    // of t' and s' and r >= |t' - b_q| - h_q: far where that bound is >= X/2
+    // (relative error of the expanded r^2 <= 4 eps), mid (one-part
+    // differences, relative error <= 3 eps) where it is >= X/3, else near
+    // (two-part differences, masked).  A non-finite partial redoes the tile
+    // in the near form.
     constexpr bool EXPANDED = KER == LAP_FXU || KER == LAP_DXU;
-    const float far_r2 = (EXPANDED ? 9.0f : 4.0f) * (float)m.w, mid_r2 = 4.0f * (float)m.w;
-    float th[TPL][3], a[TPL][3], T[TPL], p[TPL][K1];
-    bool notfar = false, near = false;
+    const float far_r2 = (EXPANDED ? 9.0f : 4.0f) * (float)m.w;
+    float th[TPL][3], T[TPL], p[TPL][K1];
+    bool notfar = false;
 #pragma unroll
     for (int j = 0; j < TPL; j++) {
       th[j][0] = (float)(t64[j][0] - m.x);
       th[j][1] = (float)(t64[j][1] - m.y);
       th[j][2] = (float)(t64[j][2] - m.z);
-#pragma unroll
-      for (int k = 0; k < 3; k++) a[j][k] = -2.0f * th[j][k];
       T[j] = fmad(th[j][2], th[j][2], fmad(th[j][1], th[j][1], th[j][0] * th[j][0]));
       notfar |= T[j] < far_r2;
-      near |= T[j] < mid_r2;
 #pragma unroll
       for (int k = 0; k < K1; k++) p[j][k] = 0.0f;
     }
     const float* recs = reinterpret_cast<const float*>(lds);
-    bool use_near = __builtin_amdgcn_ballot_w64(near) != 0;
-    const bool use_mid = !use_near && EXPANDED && __builtin_amdgcn_ballot_w64(notfar) != 0;
-    if (use_mid) {
+    bool use_near = false;
+    // -2 t', formed where a far loop needs it (not held across the sub-tiles)
+    auto minus2 = [&](float(&a)[TPL][3]) {
+#pragma unroll
+      for (int j = 0; j < TPL; j++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) a[j][k] = -2.0f * th[j][k];
+    };
+    // At more than one target per lane a wave's targets span more space and
+    // the sub-tile classes would raise the kernel's registers (two targets
+    // per lane: 70 -> 93 VGPRs); such shapes classify whole tiles only, with
+    // the mid class at |t'| >= 2H.
+    constexpr bool SUBTILES = TPL == 1;
+    bool tile_mid = false;
+    if constexpr (!SUBTILES) {
+      const float mid_r2 = 4.0f * (float)m.w;
+      bool near = false;
+#pragma unroll
+      for (int j = 0; j < TPL; j++) near |= T[j] < mid_r2;
+      use_near = __builtin_amdgcn_ballot_w64(near) != 0;
+      tile_mid = !use_near && EXPANDED && __builtin_amdgcn_ballot_w64(notfar) != 0;
+    }
+    if (use_near) {
+      // the near loop below
+    } else if (tile_mid) {
       if constexpr (EXPANDED) {
 #pragma unroll 4
         for (int i = 0; i < TILE; i++) {
@@ -271,12 +328,62 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
           for (int j = 0; j < TPL; j++) mix_mid<KER, NEWTON>(p[j], th[j], rec);
         }
       }
-    } else if (!use_near) {
+    } else if (!SUBTILES || __builtin_amdgcn_ballot_w64(notfar) == 0) {
+      float a[TPL][3];
+      minus2(a);
 #pragma unroll 4
       for (int i = 0; i < TILE; i++) {
         const float* rec = recs + i * R;
 #pragma unroll
         for (int j = 0; j < TPL; j++) mix_far<KER, NEWTON>(p[j], th[j], a[j], T[j], rec, mu);
+      }
+    } else if constexpr (SUBTILES) {
+      const float H = sqrtf((float)m.w);
+#pragma unroll 1
+      for (int q = 0; q < WG / 64; q++) {
+        const float4 b = sub[tile * (WG / 64) + q];
+        const float h = sqrtf(b.w);
+        bool qnotfar = false, qnear = false;
+#pragma unroll
+        for (int j = 0; j < TPL; j++) {
+          const float X = sqrtf(T[j]) + H;
+          const float dx = th[j][0] - b.x, dy = th[j][1] - b.y, dz = th[j][2] - b.z;
+          const float d2 = fmad(dz, dz, fmad(dy, dy, dx * dx));
+          const float ef = fmad(0.5f, X, h), em = fmad(1.0f / 3.0f, X, h);
+          qnotfar |= d2 < ef * ef;
+          qnear |= d2 < em * em;
+        }
+        const float* qrec = recs + q * 64 * R;
+        if (__builtin_amdgcn_ballot_w64(qnear) != 0) {
+          float tl[TPL][3];
+#pragma unroll
+          for (int j = 0; j < TPL; j++) {
+            tl[j][0] = (float)((t64[j][0] - m.x) - (double)th[j][0]);
+            tl[j][1] = (float)((t64[j][1] - m.y) - (double)th[j][1]);
+            tl[j][2] = (float)((t64[j][2] - m.z) - (double)th[j][2]);
+          }
+#pragma unroll 2
+          for (int i = 0; i < 64; i++) {
+#pragma unroll
+            for (int j = 0; j < TPL; j++) mix_near<KER, NEWTON>(p[j], th[j], tl[j], qrec + i * R, mu);
+          }
+        } else if (EXPANDED && __builtin_amdgcn_ballot_w64(qnotfar) != 0) {
+          if constexpr (EXPANDED) {
+#pragma unroll 4
+            for (int i = 0; i < 64; i++) {
+#pragma unroll
+              for (int j = 0; j < TPL; j++) mix_mid<KER, NEWTON>(p[j], th[j], qrec + i * R);
+            }
+          }
+        } else {
+          float a[TPL][3];
+          minus2(a);
+#pragma unroll 4
+          for (int i = 0; i < 64; i++) {
+#pragma unroll
+            for (int j = 0; j < TPL; j++) mix_far<KER, NEWTON>(p[j], th[j], a[j], T[j], qrec + i * R, mu);
+          }
+        }
       }
     }
     if (!use_near) {
@@ -342,9 +449,10 @@ k_reduce_splits_perm(const double* __restrict__ part, int64_t nt, int K1, int ns
   out[(int64_t)tperm[q] * K1 + k] += s * scale;
 }
 
-using EvalMixFn = void (*)(const double*, int64_t, const uint32_t*, const float4*, const double4*, int64_t, int64_t,
-                           double*, int, double, double);
-using PackFn = void (*)(const double*, const double*, const double*, int64_t, const uint32_t*, float4*, double4*);
+using EvalMixFn = void (*)(const double*, int64_t, const uint32_t*, const float4*, const double4*, const float4*, int64_t,
+                           int64_t, double*, int, double, double);
+using PackFn = void (*)(const double*, const double*, const double*, int64_t, const uint32_t*, float4*, double4*,
+                        float4*);
 
 constexpr int TPLS[3] = {1, 2, 4};
 int tpl_index(int tpl) { return tpl >= 4 ? 2 : (tpl >= 2 ? 1 : 0); }
@@ -404,7 +512,7 @@ int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, Mix
   int K0, K1, ND;
   kernel_dims(ker, &K0, &K1, &ND);
   g->packed_bytes = sizeof(float4) * g->Q * TILE * g->ntiles;
-  g->meta_bytes = sizeof(double4) * g->ntiles;
+  g->meta_bytes = (sizeof(double4) + sizeof(float4) * (WG / 64)) * g->ntiles;  // tile centres, then sub-tiles
   const size_t split_bytes = sizeof(double) * (size_t)nt * K1;
   g->group = split_group(g->nsplit, split_bytes, part_budget);
   g->part_bytes = g->group > 1 ? split_bytes * g->group : 0;
@@ -431,8 +539,10 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
   SCTL_CHECK_ARG(g.group <= 1 || part, "no buffer for the split partial sums");
   int K0, K1, ND;
   kernel_dims(ker, &K0, &K1, &ND);
+  // The sub-tile records follow the tile centres in the same buffer.
+  float4* sub = reinterpret_cast<float4*>(meta + g.ntiles);
   hipLaunchKernelGGL(pack_fn(ker), dim3((unsigned)g.ntiles), dim3(WG), 0, stream, r_src, n_src, v_src, ns, sperm,
-                     packed, meta);
+                     packed, meta, sub);
   SCTL_HIP_TRY(hipGetLastError());
   const double scale = kernel_scale(ker) * fold_factor(ker, newton ? 1 : 0);
   const EvalMixFn fn = mix_fn(ker, newton, g.ti);
@@ -447,7 +557,8 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
     {
       ProfileScope prof(stream);
       hipLaunchKernelGGL(fn, dim3((unsigned)g.tb, (unsigned)gy), dim3(WG), dyn, stream, r_trg, nt, tperm,
-                         packed + tile0 * (int64_t)(TILE * g.Q), meta + tile0, ntiles, g.tps, direct ? v_trg : part,
+                         packed + tile0 * (int64_t)(TILE * g.Q), meta + tile0, sub + tile0 * (WG / 64), ntiles, g.tps,
+                         direct ? v_trg : part,
                          direct ? 1 : 0, scale, ctx);
     }
     SCTL_HIP_TRY(hipGetLastError());
