@@ -96,12 +96,14 @@ struct Row {
   int a;
 };
 
-__device__ __forceinline__ Row row_of(double* Ks, int t, int nts, int64_t ld) {
+// Row k of target t (dof rows per target) of a stripe of nts targets; the
+// rows of targets past the stripe get extent 0.
+__device__ __forceinline__ Row row_of(double* Ks, int t, int nts, int64_t ld, int dof = 1, int k = 0) {
   // Built where it is used: kept opaque so that the resources of all rows
   // are not hoisted out of the caller's loop (64 scalar registers).
   uint64_t base = reinterpret_cast<uint64_t>(Ks);
   asm volatile("" : "+s"(base), "+s"(ld), "+s"(nts));
-  double* p = reinterpret_cast<double*>(base) + (int64_t)min(t, nts - 1) * ld;
+  double* p = reinterpret_cast<double*>(base) + ((int64_t)min(t, nts - 1) * dof + k) * ld;
   Row R;
   R.a = (int)((reinterpret_cast<uintptr_t>(p) >> 3) & 15);
   R.rsrc = __builtin_amdgcn_make_buffer_rsrc(p, 0, t < nts ? (int)(ld * 8) : 0, 0x00020000);
@@ -125,25 +127,29 @@ __device__ __forceinline__ double buffer_load_d(__amdgpu_buffer_rsrc_t r, int of
 // the wave walks the sources 64 at a time (one per lane) and, per chunk,
 // writes every target row's 64*KDIM0 entries.
 //
-// Rows of one entry per source (the Laplace kernels, and any kernel whose
-// outputs are contracted to one per target) are written as whole 128-byte
-// lines: a row segment that starts inside a line would leave that line
-// half-written until the next chunk, by when it has usually left L2 and the
-// memory must read it back to merge the halves.  See the W == 1 branch.
+// Rows are written as whole 128-byte lines wherever the LDS allows it: a
+// row segment that starts inside a line would leave that line half-written
+// until the next chunk, by when it has usually left L2 and the memory must
+// read it back to merge the halves.  Rows of one entry per source (the
+// Laplace kernels, and kernels contracted to one output per target) rotate
+// the chunk across lanes (the W == 1 branch); rows of K0 entries per source
+// with at most 3 rows per target go through an LDS stage (ALIGNED_WIDE).
+// Kernels with more rows per target (9 outputs, uncontracted) keep per-lane
+// stores of their K0 entries.
 //
 // Stores and loads share one completion counter on this GPU, so a wait for
 // a chunk's source records would also wait for the previous chunk's stores.
-// Both modes therefore prefetch the next chunk's records before the target
-// loop and issue a fixed number of stores per chunk (MAT_ST rows), so that
-// the wait for the prefetched records can leave the chunk's stores in
-// flight: the W == 1 branch through the range check of its buffer stores,
-// the wider store mode (at most 3 entries per source) by repeating the
-// stripe's last target and the block's last source (a repeat writes the
-// identical value to the same entry).  The wider subtract mode cannot
-// repeat an update and keeps its stores exact; the W == 1 subtract loads the
-// old entries of all MAT_ST rows before rewriting any of them, where a
-// row-by-row update would wait for the previous row's store before each
-// load.
+// The kernels therefore prefetch the next chunk's records before the target
+// loop and issue a fixed number of stores per chunk, so that the wait for
+// the prefetched records can leave the chunk's stores in flight: the aligned
+// branches through the range check of their buffer stores (rows past the
+// stripe have extent 0), the per-lane store mode (at most 3 entries per
+// source) by repeating the stripe's last target and the block's last source
+// (a repeat writes the identical value to the same entry).  The per-lane
+// subtract mode cannot repeat an update and keeps its stores exact; the
+// W == 1 subtract loads the old entries of all MAT_ST rows before rewriting
+// any of them, where a row-by-row update would wait for the previous row's
+// store before each load.
 template <int KER, bool NDOT, bool SUB>
 __global__ void __launch_bounds__(NWG)
 k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ tile_dsp, int64_t ntiles,
@@ -160,6 +166,9 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
   // cost the kernel its occupancy.
   constexpr int W = K0 * DOF;
   constexpr bool PIPE = !SUB && W <= 3;
+  // Rows of more than one entry per source, at most 3 rows per target (the
+  // LDS carries of a stripe: MAT_ST * DOF * 128 bytes per wave).
+  constexpr bool ALIGNED_WIDE = W > 1 && DOF <= 3;
   static_assert(!NDOT || K1 % 3 == 0, "the target-normal contraction needs 3 outputs per row");
   const int lane = threadIdx.x & 63;
   __shared__ double tslot_all[(NWG / 64) * MAT_ST * 6];
@@ -300,6 +309,85 @@ k_near_matrix(const int32_t* __restrict__ tile_blk, const int64_t* __restrict__ 
           if (lane < 16) cs[t * 16 + lane] = rot;
           if constexpr (SUB) out = old[t] - out;
           buffer_store_d(out, R.rsrc, (s0 - R.a + lane) * 8);
+        }
+#pragma unroll
+        for (int i = 0; i < 7; i++) cur[i] = nxt[i];
+      }
+      continue;
+    }
+    if constexpr (ALIGNED_WIDE) {
+      // K0 entries per source and up to 3 rows per target: each row's chunk
+      // segment (64 K0 consecutive entries) goes through the wave's LDS
+      // stage, lane l holding entries l K0 .. l K0 + K0 - 1, and is written
+      // back as whole 128-byte lines: entry s0 K0 - a + 64 i + l by lane l in
+      // store i, the first a entries from the row's carry (the previous
+      // chunk's last 16 entries); a final pass writes the last carry.
+      __shared__ double stage_all[(NWG / 64) * 64 * K0];
+      __shared__ double carry_w[(NWG / 64) * MAT_ST * DOF * 16];
+      double* stg = stage_all + (threadIdx.x >> 6) * (64 * K0);
+      double* cw = carry_w + (threadIdx.x >> 6) * (MAT_ST * DOF * 16);
+      const int nch = (sc + 63) >> 6;
+      for (int c = 0; c <= nch; c++) {
+        const int s0 = c * 64;
+        asm volatile("" ::: "memory");
+        const bool tail = c == nch;  // wave-uniform: the carry flush only
+        const double ws = cur[6] * scale;
+        if (!tail) load(min(s0 + 64, (nch - 1) * 64), nxt);
+#pragma unroll
+        for (int t = 0; t < MAT_ST; t++) {
+          int nr = nts;
+          asm volatile("" : "+s"(nr));
+          double e[DOF][K0];
+          if (!tail) {
+            const int tt = min(t, nr - 1);
+            double u[K0][K1];
+            const double n[3] = {cur[3], cur[4], cur[5]};
+            kmatrix<KER, NEAR_NEWTON>(u, tx[tt * 6 + 0] - cur[0], tx[tt * 6 + 1] - cur[1], tx[tt * 6 + 2] - cur[2], n,
+                                     mu);
+#pragma unroll
+            for (int k1 = 0; k1 < DOF; k1++)
+#pragma unroll
+              for (int k0 = 0; k0 < K0; k0++) {
+                if constexpr (NDOT) {
+                  double v = 0.0;
+                  v += (u[k0][k1 * 3 + 0] * ws) * tx[tt * 6 + 3];
+                  v += (u[k0][k1 * 3 + 1] * ws) * tx[tt * 6 + 4];
+                  v += (u[k0][k1 * 3 + 2] * ws) * tx[tt * 6 + 5];
+                  e[k1][k0] = v;
+                } else {
+                  e[k1][k0] = u[k0][k1] * ws;
+                }
+              }
+          } else {
+#pragma unroll
+            for (int k1 = 0; k1 < DOF; k1++)
+#pragma unroll
+              for (int k0 = 0; k0 < K0; k0++) e[k1][k0] = 0.0;
+          }
+#pragma unroll
+          for (int k1 = 0; k1 < DOF; k1++) {
+            const Row R = row_of(Ks, t, nr, ld, DOF, k1);
+            double* cr = cw + (t * DOF + k1) * 16;
+#pragma unroll
+            for (int k0 = 0; k0 < K0; k0++) stg[lane * K0 + k0] = e[k1][k0];
+            wave_sync();
+            double out[K0];
+#pragma unroll
+            for (int i = 0; i < K0; i++) {
+              const int idx = i * 64 + lane - R.a;
+              out[i] = idx >= 0 ? stg[idx] : cr[16 + idx];
+            }
+            const double last = stg[64 * K0 - 16 + (lane & 15)];
+            wave_sync();
+            if (lane < 16) cr[lane] = last;
+#pragma unroll
+            for (int i = 0; i < K0; i++) {
+              const int off = (s0 * K0 - R.a + i * 64 + lane) * 8;
+              double v = out[i];
+              if constexpr (SUB) v = buffer_load_d(R.rsrc, off) - v;
+              buffer_store_d(v, R.rsrc, off);
+            }
+          }
         }
 #pragma unroll
         for (int i = 0; i < 7; i++) cur[i] = nxt[i];
