@@ -116,21 +116,25 @@ def cpu_baseline(kernel, xt, xs, n, f, seconds):
         run = lambda m: oracle.eval(kernel, xt[:3 * m], xs, n, f, mode=mode, nthreads=cores)
         what = "oracle restatement of Eval"
     ns = xs.size // 3
-    m0 = 256
+    m0 = min(256, xt.size // 3)
+    run(m0)  # warm-up: the OpenMP thread pool
     t = time.perf_counter()
     run(m0)
     rate = m0 * ns / max(time.perf_counter() - t, 1e-6)
     m = int(min(xt.size // 3, max(m0, seconds * rate / ns / 3)))
-    m = max(64, m // 64 * 64)
+    m = max(min(64, xt.size // 3), m // 64 * 64)
+    # a sample shorter than a third of `seconds` is repeated (small configurations)
+    reps = max(1, int(seconds / 3 * rate / (m * ns)))
     times = []
     for _ in range(3):
         t = time.perf_counter()
-        run(m)
-        times.append(time.perf_counter() - t)
+        for _ in range(reps):
+            run(m)
+        times.append((time.perf_counter() - t) / reps)
     tmed = float(np.median(times))
     return {"value": round(m * ns / tmed / 1e9, 3), "unit": "G interactions/s", "cores": cores, "kind": kind,
-            "sample": "%s: %d of the %d targets x all %d sources, median of 3 runs (%.2f s each)" % (
-                what, m, xt.size // 3, ns, tmed), "host": host_cpu(oracle.have_avx512())}
+            "sample": "%s: %d of the %d targets x all %d sources, median of 3 runs of %d repeats (%.4f s each)" % (
+                what, m, xt.size // 3, ns, reps, tmed), "host": host_cpu(oracle.have_avx512())}
 
 
 def cpu_blocks_baseline(kernel, B, seconds, mode=0):
