@@ -613,7 +613,7 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--near-only", action="store_true",
-                    help="only the near-field operator and near-list lines of config 4 (for kernel traces)")
+                    help="only the near-field operator and near-list lines of configs 4 and 5 (for kernel traces)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the other micro-kernels' FP64 and mixed Eval lines (for kernel traces)")
     ap.add_argument("--batched-only", choices=["mixed", "fp64"], default=None,
@@ -652,6 +652,9 @@ def main():
         B = W.torus_blocks(64, 64, seed=0, k0=1)
         out = {"c4_near_matrix_laplace_dxu_4096": near_ops(S.LAPLACE3D_DXU, B, stream, reps=3),
                "c4_near_list_4096": near_list_build(B, 0.05, stream, cpu=False)}
+        log("config 5 blocks: near-field matrices, Stokes DxU")
+        out["c5_near_matrix_stokes_dxu_2048"] = near_ops(S.STOKES3D_DXU, W.torus_blocks(64, 32, seed=0, k0=3), stream,
+                                                         reps=2)
         print(json.dumps({"near_only": True, "configs": out}), flush=True)
         return
 
