@@ -140,8 +140,7 @@ __global__ void __launch_bounds__(WG)
 k_reduce_splits(const double* __restrict__ part, int64_t n, int nsplit, double* __restrict__ out, double scale) {
   const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
   if (i >= n) return;
-  double s = 0.0;
-  for (int y = 0; y < nsplit; y++) s += part[(int64_t)y * n + i];
+  const double s = sum_splits(part, n, nsplit, i);
   out[i] += s * scale;
 }
 

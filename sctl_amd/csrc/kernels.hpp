@@ -105,6 +105,24 @@ __device__ __forceinline__ double hw_rsq(double x) { return __builtin_amdgcn_rsq
 __device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// sum_{y < nsplit} part[y * n + i], added in split order; the loads of eight
+// splits are issued before their additions, so a thread keeps eight reads in
+// flight (a split reduction has few threads and many splits when targets are
+// few).
+__device__ __forceinline__ double sum_splits(const double* __restrict__ part, int64_t n, int nsplit, int64_t i) {
+  double s = 0.0;
+  int y = 0;
+  for (; y + 8 <= nsplit; y += 8) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = part[(int64_t)(y + j) * n + i];
+#pragma unroll
+    for (int j = 0; j < 8; j++) s += v[j];
+  }
+  for (; y < nsplit; y++) s += part[(int64_t)y * n + i];
+  return s;
+}
 // sin and cos of one argument, for the Helmholtz phase mu*r.  Cody and
 // Waite's reduction x = k*pi/2 + r (W. J. Cody, W. Waite, Software Manual for
 // the Elementary Functions, 1980), with pi/2 split into three FP64 (two FP32)

@@ -443,8 +443,7 @@ k_reduce_splits_perm(const double* __restrict__ part, int64_t nt, int K1, int ns
   const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
   const int64_t n = nt * K1;
   if (i >= n) return;
-  double s = 0.0;
-  for (int y = 0; y < nsplit; y++) s += part[(int64_t)y * n + i];
+  const double s = sum_splits(part, n, nsplit, i);
   const int64_t q = i / K1, k = i - q * K1;
   out[(int64_t)tperm[q] * K1 + k] += s * scale;
 }
