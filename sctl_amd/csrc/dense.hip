@@ -135,13 +135,13 @@ k_eval_f64(const double* __restrict__ xt, int64_t nt, const double* __restrict__
   }
 }
 
-// v_trg[i] += scale * sum_{y < nsplit} part[y][i], summed in split order.
+// v_trg[i] += scale * sum_{y < nsplit} part[y][i]; one lane quad per element
+// (sum_splits_quad).
 __global__ void __launch_bounds__(WG)
 k_reduce_splits(const double* __restrict__ part, int64_t n, int nsplit, double* __restrict__ out, double scale) {
-  const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
-  if (i >= n) return;
-  const double s = sum_splits(part, n, nsplit, i);
-  out[i] += s * scale;
+  const int64_t i = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 2;
+  const double s = sum_splits_quad(part, n, nsplit, i, i < n);
+  if (i < n && (threadIdx.x & 3) == 0) out[i] += s * scale;
 }
 
 // ------------------------------------------------------------- KernelMatrix
@@ -280,7 +280,7 @@ int f64_apply(int ker, int newton, const double* r_trg, int64_t nt, const double
     SCTL_HIP_TRY(hipGetLastError());
     if (!direct) {
       const int64_t n = nt * K1;
-      hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part, n,
+      hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((4 * n + WG - 1) / WG)), dim3(WG), 0, stream, part, n,
                          (int)gy, v_trg, scale);
       SCTL_HIP_TRY(hipGetLastError());
     }

@@ -106,21 +106,30 @@ __device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(
 __device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-// sum_{y < nsplit} part[y * n + i], added in split order; the loads of eight
-// splits are issued before their additions, so a thread keeps eight reads in
-// flight (a split reduction has few threads and many splits when targets are
-// few).
-__device__ __forceinline__ double sum_splits(const double* __restrict__ part, int64_t n, int nsplit, int64_t i) {
+// sum_{y < nsplit} part[y * n + i] by the four lanes of a lane quad (lanes
+// 4m .. 4m+3 of a wave, all with the same i): lane j adds the splits
+// [j*q, min((j+1)*q, nsplit)) in split order, q = ceil(nsplit / 4), issuing
+// the reads of sixteen splits before their additions, and the quad returns
+// (s0 + s1) + (s2 + s3) on every lane.  A reduction of many splits over few
+// targets thus keeps 64 reads per target in flight.  `ok` lanes only read;
+// the others take part in the exchange with zeros.
+__device__ __forceinline__ double sum_splits_quad(const double* __restrict__ part, int64_t n, int nsplit, int64_t i,
+                                                  bool ok) {
+  const int j = threadIdx.x & 3, q = (nsplit + 3) >> 2;
+  const int y0 = j * q, y1 = y0 + q < nsplit ? y0 + q : nsplit;
   double s = 0.0;
-  int y = 0;
-  for (; y + 8 <= nsplit; y += 8) {
-    double v[8];
+  if (ok) {
+    for (int y = y0; y < y1; y += 16) {
+      double v[16];
 #pragma unroll
-    for (int j = 0; j < 8; j++) v[j] = part[(int64_t)(y + j) * n + i];
+      for (int k = 0; k < 16; k++) v[k] = y + k < y1 ? part[(int64_t)(y + k) * n + i] : 0.0;
 #pragma unroll
-    for (int j = 0; j < 8; j++) s += v[j];
+      for (int k = 0; k < 16; k++)
+        if (y + k < y1) s += v[k];
+    }
   }
-  for (; y < nsplit; y++) s += part[(int64_t)y * n + i];
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
   return s;
 }
 // sin and cos of one argument, for the Helmholtz phase mu*r.  Cody and

@@ -29,6 +29,7 @@
 
 #include "internal.hpp"
 #include "kernels.hpp"
+#include "wave.hpp"
 
 namespace sctl_amd {
 
@@ -139,17 +140,6 @@ __device__ __forceinline__ void mix_near(float* p, const float* th, const float*
   }
 }
 
-__device__ __forceinline__ float wave_minf(float v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v = fminf(v, __shfl_xor(v, m, 64));
-  return v;
-}
-__device__ __forceinline__ float wave_maxf(float v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
-  return v;
-}
-
 // One workgroup per tile: tile centre and H^2, then the records, sources in
 // the order perm[].  meta[tile] = {c_x, c_y, c_z, H^2}; sub[4 tile + q] =
 // {b_x, b_y, b_z, h^2} of sub-tile q (sources 64q .. 64q + 63, one wave):
@@ -164,69 +154,74 @@ k_pack_tiles(const double* __restrict__ xs, const double* __restrict__ xn, const
   using K = Ker<KER>;
   constexpr int K0 = K::K0, ND = K::ND, Q = MixLayout<KER>::Q;
   static_assert(TILE == WG, "one source per lane");
-  __shared__ double red[6][WG];
-  __shared__ float hred[WG];
-  const int tid = threadIdx.x;
+  __shared__ double red[WG / 64][6];
+  __shared__ float hred[WG / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t s = (int64_t)blockIdx.x * TILE + tid;
   const bool ok = s < ns;
   const int64_t src = ok ? (int64_t)perm[s] : 0;
-  double p[3] = {0, 0, 0};
-  if (ok)
+  // every read of the source issued at once
+  double p[3] = {0, 0, 0}, n[3] = {0, 0, 0}, f[KMAX0] = {0, 0, 0, 0};
+  if (ok) {
 #pragma unroll
     for (int k = 0; k < 3; k++) p[k] = xs[src * 3 + k];
 #pragma unroll
+    for (int k = 0; k < ND; k++) n[k] = xn[src * 3 + k];
+#pragma unroll
+    for (int k = 0; k < K0; k++) f[k] = fs[src * K0 + k];
+  }
+  // tile bounding box: per wave by exchanges, then over the four waves
+#pragma unroll
   for (int k = 0; k < 3; k++) {
-    red[k][tid] = ok ? p[k] : INFINITY;
-    red[3 + k][tid] = ok ? p[k] : -INFINITY;
+    const double lo = wave_min(ok ? p[k] : INFINITY), hi = wave_max(ok ? p[k] : -INFINITY);
+    if (lane == 0) {
+      red[wv][k] = lo;
+      red[wv][3 + k] = hi;
+    }
   }
   __syncthreads();
-  for (int w = WG / 2; w > 0; w >>= 1) {
-    if (tid < w)
+  double bb[6];
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        red[k][tid] = fmin(red[k][tid], red[k][tid + w]);
-        red[3 + k][tid] = fmax(red[3 + k][tid], red[3 + k][tid + w]);
-      }
-    __syncthreads();
+  for (int k = 0; k < 6; k++) {
+    bb[k] = red[0][k];
+#pragma unroll
+    for (int w = 1; w < WG / 64; w++) bb[k] = k < 3 ? fmin(bb[k], red[w][k]) : fmax(bb[k], red[w][k]);
   }
-  const double c[3] = {0.5 * (red[0][0] + red[3][0]), 0.5 * (red[1][0] + red[4][0]), 0.5 * (red[2][0] + red[5][0])};
+  const double c[3] = {0.5 * (bb[0] + bb[3]), 0.5 * (bb[1] + bb[4]), 0.5 * (bb[2] + bb[5])};
   float rec[4 * Q];
 #pragma unroll
   for (int k = 0; k < 4 * Q; k++) rec[k] = 0.0f;
   float h2 = 0.0f;
   if (ok) {
     const double d[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
-    double n[3] = {0, 0, 0}, f[KMAX0] = {0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < ND; k++) n[k] = xn[src * 3 + k];
-#pragma unroll
-    for (int k = 0; k < K0; k++) f[k] = fs[src * K0 + k];
     fill_record<KER>(rec, d, n, f);
     h2 = rec[0] * rec[0] + rec[1] * rec[1] + rec[2] * rec[2];
   } else {
     rec[0] = rec[1] = rec[2] = PADF;
     if constexpr (KER == LAP_FXU || KER == LAP_DXU) rec[3] = 3.0f * PADF * PADF;
   }
-  hred[tid] = h2;
+  h2 = wave_max(h2);
+  if (lane == 0) hred[wv] = h2;
   __syncthreads();
-  for (int w = WG / 2; w > 0; w >>= 1) {
-    if (tid < w) hred[tid] = fmaxf(hred[tid], hred[tid + w]);
-    __syncthreads();
+  if (tid == 0) {
+    float hm = hred[0];
+#pragma unroll
+    for (int w = 1; w < WG / 64; w++) hm = fmaxf(hm, hred[w]);
+    meta[blockIdx.x] = make_double4(c[0], c[1], c[2], (double)hm * 1.0000001);
   }
-  if (tid == 0) meta[blockIdx.x] = make_double4(c[0], c[1], c[2], (double)hred[0] * 1.0000001);
   {
     float b[3], bh2 = 0.0f;
     const bool any = __builtin_amdgcn_ballot_w64(ok) != 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const float lo = wave_minf(ok ? rec[k] : INFINITY), hi = wave_maxf(ok ? rec[k] : -INFINITY);
+      const float lo = wave_min(ok ? rec[k] : INFINITY), hi = wave_max(ok ? rec[k] : -INFINITY);
       b[k] = any ? 0.5f * (lo + hi) : 0.0f;
     }
     if (ok) {
       const float dx = rec[0] - b[0], dy = rec[1] - b[1], dz = rec[2] - b[2];
       bh2 = dx * dx + dy * dy + dz * dz;
     }
-    bh2 = wave_maxf(bh2);
+    bh2 = wave_max(bh2);
     if ((tid & 63) == 0) sub[(int64_t)blockIdx.x * (WG / 64) + (tid >> 6)] = make_float4(b[0], b[1], b[2], bh2 * 1.0001f);
   }
   float4* dst = packed + ((int64_t)blockIdx.x * TILE + tid) * Q;
@@ -436,14 +431,15 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
   }
 }
 
-// v_trg[tperm[q]*K1+k] += scale * sum_y part[y][q*K1+k], in split order.
+// v_trg[tperm[q]*K1+k] += scale * sum_y part[y][q*K1+k]; one lane quad per
+// element (sum_splits_quad).
 __global__ void __launch_bounds__(WG)
 k_reduce_splits_perm(const double* __restrict__ part, int64_t nt, int K1, int nsplit, const uint32_t* __restrict__ tperm,
                      double* __restrict__ out, double scale) {
-  const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
+  const int64_t i = ((int64_t)blockIdx.x * WG + threadIdx.x) >> 2;
   const int64_t n = nt * K1;
-  if (i >= n) return;
-  const double s = sum_splits(part, n, nsplit, i);
+  const double s = sum_splits_quad(part, n, nsplit, i, i < n);
+  if (i >= n || (threadIdx.x & 3) != 0) return;
   const int64_t q = i / K1, k = i - q * K1;
   out[(int64_t)tperm[q] * K1 + k] += s * scale;
 }
@@ -563,7 +559,7 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
     SCTL_HIP_TRY(hipGetLastError());
     if (!direct) {
       const int64_t n = nt * K1;
-      hipLaunchKernelGGL(k_reduce_splits_perm, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, stream, part, nt, K1,
+      hipLaunchKernelGGL(k_reduce_splits_perm, dim3((unsigned)((4 * n + WG - 1) / WG)), dim3(WG), 0, stream, part, nt, K1,
                          (int)gy, tperm, v_trg, scale);
       SCTL_HIP_TRY(hipGetLastError());
     }

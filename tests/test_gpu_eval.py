@@ -146,6 +146,43 @@ def test_eval_degenerate_point_sets(gpu, k, case):
     assert e < MIXED_TOL, (case, e)
 
 
+def _morton_keys(p, bits):
+    """Morton keys of points p (flat x,y,z): `bits` per axis over the set's
+    bounding cube, x in the lowest bit of each triple."""
+    p = p.reshape(-1, 3)
+    if len(p) == 0:
+        return np.zeros(0, np.int64)
+    lo = p.min(0)
+    ext = (p.max(0) - lo).max()
+    inv = float(1 << bits) / ext if ext > 0 else 0.0
+    q = np.minimum(np.maximum((p - lo) * inv, 0.0), float((1 << bits) - 1)).astype(np.int64)
+    k = np.zeros(len(p), np.int64)
+    for b in range(bits):
+        for d in range(3):
+            k |= ((q[:, d] >> b) & 1) << (3 * b + d)
+    return k
+
+
+@pytest.mark.parametrize("na,nb", [(1, 0), (0, 7), (1, 1), (1000, 3000), (1024, 1024), (5000, 700), (20000, 20000),
+                                   (20480, 20480), (20481, 20480), (32768, 32768), (65535, 1), (40000, 26000)])
+def test_morton_order(gpu, na, nb):
+    """The orders a mixed Eval sums in: each set sorted stably by its Morton
+    key over its own bounding cube (5 bits per axis up to 65 536 points in
+    all, by the two-pass radix sort; 10 bits above, by rocPRIM), with
+    repeated points and a second set of another extent and offset."""
+    rng = np.random.default_rng(na * 7 + nb)
+    a = rng.random(na * 3)
+    b = rng.random(nb * 3) * 5.0 - 2.0
+    if na > 30:
+        a[3:30] = np.tile(a[:3], 9)
+    if nb > 30:
+        b[30:60] = b[0]
+    bits = 5 if na + nb <= 65536 else 10
+    pa, pb = gpu.morton_order(a, b)
+    assert (pa.astype(np.int64) == np.argsort(_morton_keys(a, bits), kind="stable")).all()
+    assert (pb.astype(np.int64) == np.argsort(_morton_keys(b, bits), kind="stable")).all()
+
+
 @pytest.mark.parametrize("k", KERNELS[2:], ids=IDS[2:])
 def test_eval_mixed_clusters(gpu, k):
     """Gaussian clusters (sigma 0.02): the case that needs tile centring."""
