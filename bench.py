@@ -228,45 +228,70 @@ def time_once(fn, stream, reps=1):
     return e0.elapsed_ms(e1) / reps
 
 
-def dense_pair(kernel, xt, xs, n, f, stream, reps, digits=-1, ctx=None):
-    """FP64 and mixed Eval of one dense configuration: rates and mixed-vs-FP64 errors."""
+def dense_pair(kernel, xt, xs, n, f, stream, reps, digits=-1, ctx=None, warm=1, blocks=1):
+    """FP64 and mixed Eval of one dense configuration: rates and mixed-vs-FP64
+    errors.  Each precision is warmed with `warm` calls; with blocks > 1 the
+    two precisions are timed in alternating blocks of `reps` calls and each
+    reports its median block (calls of a fraction of a millisecond are timed
+    at the device's steady clock rather than its ramp)."""
     _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
     nt, ns = xt.size // 3, xs.size // 3
     d = [S.DeviceArray.from_host(a) for a in (xt, xs, n if ND else np.zeros(3), f)]
     dn = d[2] if ND else None
     res = {}
     outs = {}
+    calls = {}
     for name, prec in (("fp64", S.FP64), ("mixed", S.MIXED)):
         o = S.DeviceArray.from_host(np.zeros(nt * K1))
         S.eval(kernel, d[0], d[1], dn, d[3], o, precision=prec, digits=digits, stream=stream, ctx=ctx)  # warm-up
         S.synchronize(stream)
         outs[name] = o.download()
-        ms = time_once(lambda: S.eval(kernel, d[0], d[1], dn, d[3], o, precision=prec, digits=digits,
-                                      stream=stream, ctx=ctx), stream, reps)
-        res[name] = {"ms": round(ms, 3), "G_int_per_s": round(nt * ns / ms / 1e6, 2)}
+        calls[name] = (lambda o=o, prec=prec: S.eval(kernel, d[0], d[1], dn, d[3], o, precision=prec, digits=digits,
+                                                     stream=stream, ctx=ctx))
+        for _ in range(warm - 1):
+            calls[name]()
+    S.synchronize(stream)
+    times = {name: [] for name in calls}
+    for _ in range(blocks):
+        for name, fn in calls.items():
+            times[name].append(time_once(fn, stream, reps))
+    for name in calls:
+        ms = float(np.median(times[name]))
+        res[name] = {"ms": round(ms, 4 if ms < 1 else 3), "G_int_per_s": round(nt * ns / ms / 1e6, 2)}
+        if blocks > 1:
+            res[name]["timing"] = "median of %d alternating blocks of %d calls after %d warm-up calls" % (blocks, reps,
+                                                                                                       warm)
     e_inf, e_rms = W.rel_errors(outs["mixed"], outs["fp64"])
     res["mixed_vs_fp64"] = {"max_rel": float("%.3g" % e_inf), "rms_rel": float("%.3g" % e_rms)}
     res["speedup"] = round(res["fp64"]["ms"] / res["mixed"]["ms"], 3)
     return res, outs
 
 
-def dense_plan_rates(kernel, xt, xs, n, f, stream, reps):
+def dense_plan_rates(kernel, xt, xs, n, f, stream, reps, blocks=10):
     """Repeated applies of a dense plan (geometry resident, set-up once):
-    rates per precision and the setup time."""
+    rates per precision (median of `blocks` alternating blocks of `reps`
+    applies after `reps` warm-up applies) and the setup time."""
     _, K0, K1, ND, _, _, _ = S.kernel_info(kernel)
     nt, ns = xt.size // 3, xs.size // 3
     d = [S.DeviceArray.from_host(a) for a in (xt, xs, n if ND else np.zeros(3), f)]
-    res = {}
+    res, apply = {}, {}
     for name, prec in (("fp64", S.FP64), ("mixed", S.MIXED)):
         t = time.perf_counter()
         P = S.DensePlan(kernel, d[0], d[1], d[2] if ND else None, precision=prec, stream=stream)
         setup_ms = (time.perf_counter() - t) * 1e3
         o = S.DeviceArray.from_host(np.zeros(nt * K1))
-        P.apply(d[3], o, stream=stream)
-        S.synchronize(stream)
-        ms = time_once(lambda: P.apply(d[3], o, stream=stream), stream, reps)
-        res[name] = {"ms_per_apply": round(ms, 4), "G_int_per_s": round(nt * ns / ms / 1e6, 2),
-                     "setup_ms": round(setup_ms, 3)}
+        apply[name] = (lambda P=P, o=o: P.apply(d[3], o, stream=stream))
+        for _ in range(reps):
+            apply[name]()
+        res[name] = {"setup_ms": round(setup_ms, 3)}
+    S.synchronize(stream)
+    times = {name: [] for name in apply}
+    for _ in range(blocks):
+        for name, fn in apply.items():
+            times[name].append(time_once(fn, stream, reps))
+    for name in apply:
+        ms = float(np.median(times[name]))
+        res[name].update({"ms_per_apply": round(ms, 4), "G_int_per_s": round(nt * ns / ms / 1e6, 2)})
     res["speedup"] = round(res["fp64"]["ms_per_apply"] / res["mixed"]["ms_per_apply"], 3)
     return res
 
@@ -483,7 +508,7 @@ def other_configs(stream, quick=False, cpu=True, cpu_seconds=3.0):
     out = {}
     log("config 1: Laplace FxU FP64, 20000 x 20000 uniform cube, parity vs the oracle")
     xt, xs, f = W.cube(20000, 20000, seed=0)
-    r, o = dense_pair(S.LAPLACE3D_FXU, xt, xs, None, f, stream, reps=5)
+    r, o = dense_pair(S.LAPLACE3D_FXU, xt, xs, None, f, stream, reps=20, warm=20, blocks=10)
     add_roofline(r, S.LAPLACE3D_FXU)
     if cpu:
         r["cpu_reference"] = cpu_baseline(S.LAPLACE3D_FXU, xt, xs, None, f, 1.0)
