@@ -276,6 +276,7 @@ def dense_plan_rates(kernel, xt, xs, n, f, stream, reps, blocks=10):
     d = [S.DeviceArray.from_host(a) for a in (xt, xs, n if ND else np.zeros(3), f)]
     res, apply = {}, {}
     for name, prec in (("fp64", S.FP64), ("mixed", S.MIXED)):
+        S.synchronize(stream)  # the setup clock must not include the other plan's warm-up applies
         t = time.perf_counter()
         P = S.DensePlan(kernel, d[0], d[1], d[2] if ND else None, precision=prec, stream=stream)
         setup_ms = (time.perf_counter() - t) * 1e3
