@@ -122,12 +122,18 @@ int device_info(DeviceInfo* out) {
     d.device = dev;
     d.num_cus = p.multiProcessorCount;
     d.max_threads = p.maxThreadsPerBlock;
-    // Keep freed stream-ordered allocations in the pool for reuse.
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-      uint64_t threshold = UINT64_MAX;
-      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
-    }
+    // The library's own stream-ordered pool on this device (the process-wide
+    // default pool keeps its settings): freed scratch stays reserved for
+    // reuse up to POOL_KEEP_BYTES, since returning it to the driver and
+    // reserving it again costs tens of microseconds per call.
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    SCTL_HIP_TRY(hipMemPoolCreate(&d.pool, &props));
+    uint64_t threshold = POOL_KEEP_BYTES;
+    SCTL_HIP_TRY(hipMemPoolSetAttribute(d.pool, hipMemPoolAttrReleaseThreshold, &threshold));
     d.ok = true;
   }
   *out = d;

@@ -37,18 +37,21 @@ const char* last_error();
     }                                                   \
   } while (0)
 
-// Device facts, looked up once per device.
+// Device facts, looked up once per device, and the library's memory pool on it.
 struct DeviceInfo {
   int device = -1;
   int num_cus = 0;
   int max_threads = 0;
+  hipMemPool_t pool = nullptr;
   bool ok = false;
 };
 int device_info(DeviceInfo* out);  // for the current device
+// Freed scratch the library's pool keeps reserved for reuse (its release
+// threshold); beyond it, memory goes back to the driver at synchronisation.
+constexpr uint64_t POOL_KEEP_BYTES = (uint64_t)4 << 30;
 
-// Stream-ordered device scratch: allocated from the device's default memory
-// pool (whose release threshold is raised at init, so memory is reused
-// across calls rather than returned to the driver), freed on the same stream.
+// Stream-ordered device scratch: allocated from the library's pool on the
+// current device, freed on the same stream.
 class Scratch {
  public:
   Scratch() = default;
@@ -59,7 +62,9 @@ class Scratch {
     release();
     stream_ = s;
     if (!bytes) return SCTL_AMD_SUCCESS;
-    SCTL_HIP_TRY(hipMallocAsync(&ptr_, bytes, s));
+    DeviceInfo dev;
+    SCTL_TRY(device_info(&dev));
+    SCTL_HIP_TRY(hipMallocFromPoolAsync(&ptr_, bytes, dev.pool, s));
     bytes_ = bytes;
     return SCTL_AMD_SUCCESS;
   }
