@@ -236,26 +236,34 @@ k_blocks_f64(const Item* __restrict__ items, int nitems, int* __restrict__ count
 // FP64 sums with Kahan compensation; a non-finite partial (r == 0 in the
 // stage) is redone with the r^2 > 0 mask.
 //
-// The Laplace single and double layers need only r^2 and f n.r of each pair,
-// and FP64 FMAs issue at the FP32 rate on CDNA4, so they form both in FP64
-// from the FP64 differences and round each once (f n is staged in FP64):
-// 6 (FxU) or 9 (DxU) FP64 operations and 1 or 2 conversions instead of 3
-// conversions and 3 or 6 FP32 operations, and r^2 correctly rounded.
+// The Laplace single and double layers need only r^2 and f n.r of each pair.
+// The single layer forms r^2 in FP64 from the FP64 differences (FP64 FMAs
+// issue at the FP32 rate on CDNA4) and rounds it once: 6 FP64 operations and
+// 1 conversion.  The double layer rounds its three FP64 differences and forms
+// r^2 and f n.r in FP32 from f n staged in FP32: 3 FP64 operations, 3
+// conversions and 6 FP32 operations, which can pair with other waves' FP32
+// work where FP64 cannot (4 % faster on the 4096 torus blocks than forming
+// both in FP64; the single layer is 3.5 % slower that way).
 template <int KER> constexpr bool lap_r2d() { return KER == LAP_FXU || KER == LAP_DXU; }
 
-// One pair of the Laplace FP64-r^2 form: pr = {s, f n} (DxU) or {s, -} (FxU),
-// fs the FP32 density (FxU).  z carries the same power as in interact(), so
-// the deferred-Newton constant folds the same way.
+// One Laplace pair: pr = the source position (FP64), ar = {f} (FxU) or
+// {f n_x, f n_y, f n_z} (DxU) in FP32.  z carries the same power as in
+// interact(), so the deferred-Newton constant folds the same way.
 template <int KER, int NEWTON, bool MASK>
-__device__ __forceinline__ void lap_pair_r2d(float& p, double tx, double ty, double tz, const double* pr, float fs) {
-  const double dx = tx - pr[0], dy = ty - pr[1], dz = tz - pr[2];
-  const float r2 = (float)fmad(dz, dz, fmad(dy, dy, dx * dx));
-  float z = rsq_deferred<NEWTON>(r2);
-  if constexpr (MASK) z = r2 > 0.0f ? z : 0.0f;
+__device__ __forceinline__ void lap_pair_r2d(float& p, double tx, double ty, double tz, const double* pr,
+                                             const float* ar) {
   if constexpr (KER == LAP_FXU) {
-    p = fmad(z, fs, p);
+    const double dx = tx - pr[0], dy = ty - pr[1], dz = tz - pr[2];
+    const float r2 = (float)fmad(dz, dz, fmad(dy, dy, dx * dx));
+    float z = rsq_deferred<NEWTON>(r2);
+    if constexpr (MASK) z = r2 > 0.0f ? z : 0.0f;
+    p = fmad(z, ar[0], p);
   } else {
-    const float nd = (float)fmad(dz, pr[5], fmad(dy, pr[4], dx * pr[3]));
+    const float dx = (float)(tx - pr[0]), dy = (float)(ty - pr[1]), dz = (float)(tz - pr[2]);
+    const float r2 = fmad(dz, dz, fmad(dy, dy, dx * dx));
+    float z = rsq_deferred<NEWTON>(r2);
+    if constexpr (MASK) z = r2 > 0.0f ? z : 0.0f;
+    const float nd = fmad(dz, ar[2], fmad(dy, ar[1], dx * ar[0]));
     p = fmad(nd * z, z * z, p);
   }
 }
@@ -271,8 +279,8 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
   using K = Ker<KER>;
   constexpr int K0 = K::K0, K1 = K::K1, ND = K::ND, STAGE = 64 * SPL;
   constexpr bool R2D = lap_r2d<KER>();
-  constexpr int PW = KER == LAP_DXU ? 6 : 4;                       // doubles per source: s (, f n)
-  constexpr int AW = KER == LAP_DXU ? 1 : ((ND + K0) + 3) & ~3;  // float words of normal + density
+  constexpr int PW = 4;                                          // doubles per source: s, padded
+  constexpr int AW = KER == LAP_DXU ? 4 : ((ND + K0) + 3) & ~3;  // float words: f n (DxU), else normal, density
   __shared__ double pos_all[(BWG / 64) * STAGE * PW];
   __shared__ float aux_all[(BWG / 64) * STAGE * AW];
   const int lane = threadIdx.x & 63;
@@ -316,7 +324,8 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
         if constexpr (KER == LAP_DXU) {
           const double f = real ? fs[s] : 0.0;
 #pragma unroll
-          for (int k = 0; k < 3; k++) pr[3 + k] = f * xn[s * 3 + k];
+          for (int k = 0; k < 3; k++) ar[k] = (float)(f * xn[s * 3 + k]);
+          ar[3] = 0.0f;
         } else {
 #pragma unroll
           for (int k = 0; k < ND; k++) ar[k] = real ? (float)xn[s * 3 + k] : 0.0f;
@@ -337,7 +346,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
         const float* ar = aux + i * AW;
         if constexpr (R2D) {
 #pragma unroll
-          for (int j = 0; j < TPL; j++) lap_pair_r2d<KER, NEWTON, false>(p[j][0], tx[j], ty[j], tz[j], pr, KER == LAP_FXU ? ar[0] : 0.0f);
+          for (int j = 0; j < TPL; j++) lap_pair_r2d<KER, NEWTON, false>(p[j][0], tx[j], ty[j], tz[j], pr, ar);
         } else {
           float n[3] = {0, 0, 0}, f[K0];
 #pragma unroll
@@ -365,7 +374,7 @@ k_blocks_mixed(const Item* __restrict__ items, int nitems, int* __restrict__ cou
           const float* ar = aux + i * AW;
           if constexpr (R2D) {
 #pragma unroll
-            for (int j = 0; j < TPL; j++) lap_pair_r2d<KER, NEWTON, true>(p[j][0], tx[j], ty[j], tz[j], pr, KER == LAP_FXU ? ar[0] : 0.0f);
+            for (int j = 0; j < TPL; j++) lap_pair_r2d<KER, NEWTON, true>(p[j][0], tx[j], ty[j], tz[j], pr, ar);
           } else {
             float n[3] = {0, 0, 0}, f[K0];
             for (int k = 0; k < ND; k++) n[k] = ar[k];
