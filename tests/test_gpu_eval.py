@@ -163,6 +163,18 @@ def _morton_keys(p, bits):
     return k
 
 
+@pytest.mark.parametrize("na,nb", [(300, 5), (70000, 3)])
+def test_morton_order_zero_extent(gpu, na, nb):
+    """A set collapsed to one point has a zero-extent box: every key is 0 and
+    the order is the input order (both sorts, with a tiny second set)."""
+    a = np.tile([0.25, -3.0, 7.5], na)
+    b = np.random.default_rng(na).random(nb * 3)
+    pa, pb = gpu.morton_order(a, b)
+    assert (pa.astype(np.int64) == np.arange(na)).all()
+    bits = 5 if na + nb <= 65536 else 10
+    assert (pb.astype(np.int64) == np.argsort(_morton_keys(b, bits), kind="stable")).all()
+
+
 @pytest.mark.parametrize("na,nb", [(1, 0), (0, 7), (1, 1), (1000, 3000), (1024, 1024), (5000, 700), (20000, 20000),
                                    (20480, 20480), (20481, 20480), (32768, 32768), (65535, 1), (40000, 26000)])
 def test_morton_order(gpu, na, nb):
