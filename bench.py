@@ -8,7 +8,7 @@ inputs already resident in HBM.  The headline value is the mixed-precision
 path (FP64 tile-centred differencing, FP32 pairwise, Kahan FP64 accumulation)
 in G interactions/s; the plain FP64 path is timed on the same inputs and the
 mixed result is checked against it (normwise max and RMS relative error).
-A step includes everything Eval does on the device (Morton ordering, tile
+A step includes everything Eval does on the device (Hilbert ordering, tile
 packing, the summation kernel).
 
 roofline: the dominant kernel (the pairwise summation) is VALU-bound;

@@ -134,7 +134,7 @@ int sctl_amd_eval_ctx(int kernel, const double* ctx, int precision, int digits, 
  * sctl_amd_eval_noalloc_scratch_bytes() bytes (128 MiB), held by the library
  * until the process ends.  FP64 Evals of any size fit (their split partial
  * sums run in as many launches as the array needs); a mixed-precision Eval
- * whose Morton orders, packed source tiles and sort workspace exceed the
+ * whose Hilbert orders, packed source tiles and sort workspace exceed the
  * array (about 2^20 + 2^20 Laplace points is the largest that fits) fails
  * with SCTL_AMD_ERROR_INVALID_VALUE.  Every call on a GPU uses that GPU's one
  * array: calls that overlap on a GPU (other streams or host threads) race,
@@ -362,15 +362,15 @@ int sctl_amd_graph_destroy(sctl_amd_graph graph);
  * (seed, stream_id) -- SplitMix64 counters, Box-Muller pairs. */
 int sctl_amd_gen_normal(uint64_t seed, uint64_t stream_id, int64_t n, double* out, void* stream);
 
-/* The Morton orders that the mixed-precision path applies to its sources
+/* The Hilbert-curve orders that the mixed-precision path applies to its sources
  * (set a) and targets (set b) before summing: perm_a is the order of the
- * points of r_a (na x 3, device) by their Morton key over r_a's bounding
+ * points of r_a (na x 3, device) by their Hilbert key over r_a's bounding
  * cube, ties kept in input order, and likewise for set b.  Keys take 5 bits
  * per axis when na + nb <= 65536, else 10.  No counterpart in the reference
  * (whose direct Eval sums in input order); exposed so that callers and tests
  * can see the order a mixed Eval sums in. */
-int sctl_amd_morton_order(const double* r_a, int64_t na, uint32_t* perm_a, const double* r_b, int64_t nb,
-                          uint32_t* perm_b, void* stream);
+int sctl_amd_hilbert_order(const double* r_a, int64_t na, uint32_t* perm_a, const double* r_b, int64_t nb,
+                           uint32_t* perm_b, void* stream);
 
 /* ----------------------------------------------------------------- tuning
  * The autotuner times candidate launch shapes on the caller's own data (HIP

@@ -56,7 +56,7 @@ SYMBOLS = [
     "sctl_amd_near_list_size", "sctl_amd_near_list_get", "sctl_amd_near_list_destroy", "sctl_amd_eval_ctx",
     "sctl_amd_kernel_matrix_ctx", "sctl_amd_plan_set_ctx", "sctl_amd_dense_plan_create", "sctl_amd_dense_plan_apply",
     "sctl_amd_dense_plan_shape", "sctl_amd_dense_plan_destroy", "sctl_amd_dense_plan_graph_create",
-    "sctl_amd_eval_noalloc", "sctl_amd_eval_noalloc_scratch_bytes", "sctl_amd_morton_order",
+    "sctl_amd_eval_noalloc", "sctl_amd_eval_noalloc_scratch_bytes", "sctl_amd_hilbert_order",
 ]
 
 
@@ -118,7 +118,7 @@ def lib():
     L.sctl_amd_graph_interactions.argtypes = [_vp, _i64p]
     L.sctl_amd_graph_destroy.argtypes = [_vp]
     L.sctl_amd_gen_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, i64, _vp, _vp]
-    L.sctl_amd_morton_order.argtypes = [_vp, i64, _vp, _vp, i64, _vp, _vp]
+    L.sctl_amd_hilbert_order.argtypes = [_vp, i64, _vp, _vp, i64, _vp, _vp]
     ip = ctypes.POINTER(i32)
     L.sctl_amd_plan_get_shape.argtypes = [_vp, ip, ip, ip]
     L.sctl_amd_plan_set_shape.argtypes = [_vp, i32, i32, i32]
@@ -574,15 +574,15 @@ def gen_normal(seed, stream_id, out, stream=None):
     return out
 
 
-def morton_order(r_a, r_b):
-    """Morton orders of two host point sets (flat x,y,z) as the mixed path
+def hilbert_order(r_a, r_b):
+    """Hilbert-curve orders of two host point sets (flat x,y,z) as the mixed path
     sums them: (perm_a, perm_b) as uint32 arrays."""
     r_a = np.ascontiguousarray(r_a, dtype=np.float64).ravel()
     r_b = np.ascontiguousarray(r_b, dtype=np.float64).ravel()
     na, nb = r_a.size // 3, r_b.size // 3
     da, db = DeviceArray.from_host(r_a), DeviceArray.from_host(r_b)
     pa, pb = DeviceArray(na, np.uint32), DeviceArray(nb, np.uint32)
-    check(lib().sctl_amd_morton_order(da.ptr, na, pa.ptr, db.ptr, nb, pb.ptr, None))
+    check(lib().sctl_amd_hilbert_order(da.ptr, na, pa.ptr, db.ptr, nb, pb.ptr, None))
     return pa.download(), pb.download()
 
 

@@ -5,7 +5,7 @@
 // fmm-wrapper.txx:551-623).
 //
 // Creation copies the positions (and normals) to plan-owned device memory,
-// resolves the launch shape and, for mixed precision, computes the Morton
+// resolves the launch shape and, for mixed precision, computes the Hilbert
 // orders of targets and sources once; it allocates every buffer an apply
 // needs.  An apply is then the dense Eval without its per-call set-up: for
 // mixed precision the tile packing (the densities change), the pairwise

@@ -131,7 +131,7 @@ int launch_kernel_matrix(int ker, int newton, const double* r_trg, int64_t nt, c
                          const double* n_src, int64_t ns, double* M, double ctx, hipStream_t stream);
 
 // The dense Eval launchers in two parts, for callers that keep the geometry
-// (dense plans): a one-time part (sizes; for mixed precision the Morton
+// (dense plans): a one-time part (sizes; for mixed precision the Hilbert
 // orders) and an apply that allocates nothing and issues no memset.
 // Both take the same part_budget (split_group); part holds f64_part_bytes.
 int f64_part_bytes(int ker, int64_t nt, int64_t ns, const DenseShape& shape, size_t* bytes,
@@ -148,7 +148,7 @@ struct MixedGeometry {
 // part_budget bounds part_bytes (split_group).
 int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, MixedGeometry* g,
                    size_t part_budget = PART_BUDGET);
-// Morton orders of targets and sources; `work` (device) holds
+// Hilbert orders of targets and sources; `work` (device) holds
 // mixed_order_bytes(nt, ns) bytes, or is null to allocate it per call.
 size_t mixed_order_bytes(int64_t nt, int64_t ns);
 int mixed_order(const double* r_trg, int64_t nt, const double* r_src, int64_t ns, uint32_t* tperm, uint32_t* sperm,
@@ -167,11 +167,11 @@ int mixed_eval_in(int ker, int newton, const MixedGeometry& g, const DenseShape&
 // True while `s` is being captured into a graph (no host-side counting then).
 bool stream_capturing(hipStream_t s);
 
-// Morton orderings of two point sets (30-bit keys over each set's bounding
+// Hilbert-curve orderings of two point sets (keys over each set's bounding
 // box) by one sort: permutations of [0, na) and [0, nb).  `work` (device)
-// holds morton_order_bytes(na, nb) bytes.  Implemented in sort.hip.
-size_t morton_order_bytes(int64_t na, int64_t nb);
-int morton_order(const double* a, int64_t na, uint32_t* perm_a, const double* b, int64_t nb, uint32_t* perm_b,
+// holds hilbert_order_bytes(na, nb) bytes.  Implemented in sort.hip.
+size_t hilbert_order_bytes(int64_t na, int64_t nb);
+int hilbert_order(const double* a, int64_t na, uint32_t* perm_a, const double* b, int64_t nb, uint32_t* perm_b,
                  void* work, hipStream_t stream);
 // Axis-aligned bounding box of two point sets (either may be empty):
 // box6 (device) = {min x, min y, min z, max x, max y, max z}.  sort.hip.

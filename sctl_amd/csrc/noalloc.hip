@@ -1,7 +1,7 @@
 // sctl_amd_eval_noalloc: the dense Eval of sctl_amd_eval_ctx for callers
 // that cannot allocate device memory (code under HIP graph capture, serving
 // loops, hosts that forbid hipMalloc).  Its scratch -- the mixed path's
-// Morton orders, packed source tiles, tile centres, sort workspace and split
+// Hilbert orders, packed source tiles, tile centres, sort workspace and split
 // partial sums; the FP64 path's split partial sums -- comes from one
 // file-scope device array, `scratch`, of NOALLOC_SCRATCH_BYTES per device.
 //

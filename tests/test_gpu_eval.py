@@ -12,6 +12,7 @@ import pytest
 
 import oracle
 import pins
+from hilbert_ref import hilbert_keys as _hilbert_keys
 from sctl_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -114,7 +115,7 @@ def test_eval_mixed_matches_oracle(gpu, k, newton, nt, ns, coincide):
 @pytest.mark.parametrize("case", ["sources_one_point", "targets_one_point", "one_source", "one_target", "far_apart"])
 @pytest.mark.parametrize("k", [0, 1, 2, 3], ids=IDS[:4])
 def test_eval_degenerate_point_sets(gpu, k, case):
-    """Point sets on which the Morton ordering of the mixed path degenerates:
+    """Point sets on which the Hilbert ordering of the mixed path degenerates:
     a set collapsed to one point (a zero-extent box, every key equal), a
     single source or target, and targets in a tiny box far from the sources.
     Both precisions against the oracle.  With random-sign densities these
@@ -146,39 +147,22 @@ def test_eval_degenerate_point_sets(gpu, k, case):
     assert e < MIXED_TOL, (case, e)
 
 
-def _morton_keys(p, bits):
-    """Morton keys of points p (flat x,y,z): `bits` per axis over the set's
-    bounding cube, x in the lowest bit of each triple."""
-    p = p.reshape(-1, 3)
-    if len(p) == 0:
-        return np.zeros(0, np.int64)
-    lo = p.min(0)
-    ext = (p.max(0) - lo).max()
-    inv = float(1 << bits) / ext if ext > 0 else 0.0
-    q = np.minimum(np.maximum((p - lo) * inv, 0.0), float((1 << bits) - 1)).astype(np.int64)
-    k = np.zeros(len(p), np.int64)
-    for b in range(bits):
-        for d in range(3):
-            k |= ((q[:, d] >> b) & 1) << (3 * b + d)
-    return k
-
-
 @pytest.mark.parametrize("na,nb", [(300, 5), (70000, 3)])
-def test_morton_order_zero_extent(gpu, na, nb):
+def test_hilbert_order_zero_extent(gpu, na, nb):
     """A set collapsed to one point has a zero-extent box: every key is 0 and
     the order is the input order (both sorts, with a tiny second set)."""
     a = np.tile([0.25, -3.0, 7.5], na)
     b = np.random.default_rng(na).random(nb * 3)
-    pa, pb = gpu.morton_order(a, b)
+    pa, pb = gpu.hilbert_order(a, b)
     assert (pa.astype(np.int64) == np.arange(na)).all()
     bits = 5 if na + nb <= 65536 else 10
-    assert (pb.astype(np.int64) == np.argsort(_morton_keys(b, bits), kind="stable")).all()
+    assert (pb.astype(np.int64) == np.argsort(_hilbert_keys(b, bits), kind="stable")).all()
 
 
 @pytest.mark.parametrize("na,nb", [(1, 0), (0, 7), (1, 1), (1000, 3000), (1024, 1024), (5000, 700), (20000, 20000),
                                    (20480, 20480), (20481, 20480), (32768, 32768), (65535, 1), (40000, 26000)])
-def test_morton_order(gpu, na, nb):
-    """The orders a mixed Eval sums in: each set sorted stably by its Morton
+def test_hilbert_order(gpu, na, nb):
+    """The orders a mixed Eval sums in: each set sorted stably by its Hilbert
     key over its own bounding cube (5 bits per axis up to 65 536 points in
     all, by the two-pass radix sort; 10 bits above, by rocPRIM), with
     repeated points and a second set of another extent and offset."""
@@ -190,9 +174,9 @@ def test_morton_order(gpu, na, nb):
     if nb > 30:
         b[30:60] = b[0]
     bits = 5 if na + nb <= 65536 else 10
-    pa, pb = gpu.morton_order(a, b)
-    assert (pa.astype(np.int64) == np.argsort(_morton_keys(a, bits), kind="stable")).all()
-    assert (pb.astype(np.int64) == np.argsort(_morton_keys(b, bits), kind="stable")).all()
+    pa, pb = gpu.hilbert_order(a, b)
+    assert (pa.astype(np.int64) == np.argsort(_hilbert_keys(a, bits), kind="stable")).all()
+    assert (pb.astype(np.int64) == np.argsort(_hilbert_keys(b, bits), kind="stable")).all()
 
 
 @pytest.mark.parametrize("k", KERNELS[2:], ids=IDS[2:])
