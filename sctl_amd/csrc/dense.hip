@@ -43,9 +43,10 @@ k_eval_f64(const double* __restrict__ xt, int64_t nt, const double* __restrict__
   __shared__ double lds[TILE * R];
 
   const int tid = threadIdx.x;
-  const int64_t s_begin = (int64_t)blockIdx.y * chunk;
+  const SplitCoord wg = split_coord((nt + WG * TPL - 1) / (WG * TPL));
+  const int64_t s_begin = wg.split * chunk;
   const int64_t s_end = s_begin + chunk < ns ? s_begin + chunk : ns;
-  const int64_t t0 = (int64_t)blockIdx.x * (WG * TPL) + tid;
+  const int64_t t0 = wg.block * (WG * TPL) + tid;
 
   double tx[TPL], ty[TPL], tz[TPL], acc[TPL][K1];
 #pragma unroll
@@ -129,7 +130,7 @@ k_eval_f64(const double* __restrict__ xt, int64_t nt, const double* __restrict__
 #pragma unroll
       for (int k = 0; k < K1; k++) {
         if (direct) out[t * K1 + k] += acc[j][k] * scale;
-        else out[(int64_t)blockIdx.y * nt * K1 + t * K1 + k] = acc[j][k];
+        else out[wg.split * nt * K1 + t * K1 + k] = acc[j][k];
       }
     }
   }
@@ -273,7 +274,7 @@ int f64_apply(int ker, int newton, const double* r_trg, int64_t nt, const double
     const bool direct = gy == 1;
     {
       ProfileScope prof(stream);
-      hipLaunchKernelGGL(fn, dim3((unsigned)tb, (unsigned)gy), dim3(WG), dyn, stream, r_trg, nt, r_src + s0 * 3,
+      hipLaunchKernelGGL(fn, split_grid(tb, gy, WG), dim3(WG), dyn, stream, r_trg, nt, r_src + s0 * 3,
                          ND ? n_src + s0 * 3 : n_src, v_src + s0 * K0, ns_g, chunk, direct ? v_trg : part,
                          direct ? 1 : 0, scale, ctx);
     }

@@ -243,9 +243,10 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
   const float mu = (float)ctx;
 
   const int tid = threadIdx.x;
-  const int64_t tile_begin = (int64_t)blockIdx.y * tiles_per_split;
+  const SplitCoord wg = split_coord((nt + WG * TPL - 1) / (WG * TPL));
+  const int64_t tile_begin = wg.split * tiles_per_split;
   const int64_t tile_end = tile_begin + tiles_per_split < ntiles ? tile_begin + tiles_per_split : ntiles;
-  const int64_t q0 = (int64_t)blockIdx.x * (WG * TPL) + tid;
+  const int64_t q0 = wg.block * (WG * TPL) + tid;
 
   double t64[TPL][3], acc[TPL][K1], comp[TPL][K1];
   int64_t tidx[TPL];
@@ -457,7 +458,7 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
       for (int k = 0; k < K1; k++) {
         const double v = acc[j][k] - comp[j][k];
         if (direct) out[tidx[j] * K1 + k] += v * scale;
-        else out[(int64_t)blockIdx.y * nt * K1 + q * K1 + k] = v;
+        else out[wg.split * nt * K1 + q * K1 + k] = v;
       }
     }
   }
@@ -583,7 +584,7 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
     const bool direct = gy == 1;
     {
       ProfileScope prof(stream);
-      hipLaunchKernelGGL(fn, dim3((unsigned)g.tb, (unsigned)gy), dim3(WG), dyn, stream, r_trg, nt, tperm,
+      hipLaunchKernelGGL(fn, split_grid(g.tb, gy, WG), dim3(WG), dyn, stream, r_trg, nt, tperm,
                          packed + tile0 * (int64_t)(TILE * g.Q), meta + tile0, sub + tile0 * (WG / 64), ntiles, g.tps,
                          direct ? v_trg : part,
                          direct ? 1 : 0, scale, ctx);
