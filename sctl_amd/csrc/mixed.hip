@@ -232,8 +232,15 @@ k_pack_tiles(const double* __restrict__ xs, const double* __restrict__ xn, const
   for (int q = 0; q < Q; q++) dst[q] = make_float4(rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], rec[4 * q + 3]);
 }
 
+// Waves per SIMD that the register allocation must leave room for: the
+// Helmholtz gradient kernels (6 outputs per target) need 140-143 VGPRs at one
+// target per lane, which allows 3 waves; holding them to 128 allows 4.
+template <int KER, int TPL> constexpr int mixed_min_waves() {
+  return (KER == HLM_FXDU || KER == HLMD_FXDU) && TPL == 1 ? 4 : 1;
+}
+
 template <int KER, int TPL, int NEWTON>
-__global__ void __launch_bounds__(WG)
+__global__ void __launch_bounds__(WG, (mixed_min_waves<KER, TPL>()))
 k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restrict__ tperm,
              const float4* __restrict__ packed, const double4* __restrict__ meta, const float4* __restrict__ sub,
              int64_t ntiles, int64_t tiles_per_split, double* __restrict__ out, int direct, double scale, double ctx) {
