@@ -153,13 +153,23 @@ template <int KER>
 __global__ void __launch_bounds__(WG)
 k_pack_tiles(const double* __restrict__ xs, const double* __restrict__ xn, const double* __restrict__ fs, int64_t ns,
              const uint32_t* __restrict__ perm, float4* __restrict__ packed, double4* __restrict__ meta,
-             float4* __restrict__ sub) {
+             float4* __restrict__ sub, int64_t ntiles, const double* __restrict__ xt, int64_t nt,
+             const uint32_t* __restrict__ tperm, double* __restrict__ xt_sorted) {
   using K = Ker<KER>;
   constexpr int K0 = K::K0, ND = K::ND, Q = MixLayout<KER>::Q;
   static_assert(TILE == WG, "one source per lane");
   __shared__ double red[WG / 64][6];
   __shared__ float hred[WG / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if ((int64_t)blockIdx.x >= ntiles) {  // the workgroups after the tiles copy the targets into their order
+    const int64_t q = ((int64_t)blockIdx.x - ntiles) * WG + tid;
+    if (q < nt) {
+      const int64_t t = tperm[q];
+#pragma unroll
+      for (int k = 0; k < 3; k++) xt_sorted[q * 3 + k] = xt[t * 3 + k];
+    }
+    return;
+  }
   const int64_t s = (int64_t)blockIdx.x * TILE + tid;
   const bool ok = s < ns;
   const int64_t src = ok ? (int64_t)perm[s] : 0;
@@ -263,7 +273,7 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
     const bool ok = q < nt;
     tidx[j] = ok ? (int64_t)tperm[q] : -1;
 #pragma unroll
-    for (int k = 0; k < 3; k++) t64[j][k] = ok ? xt[tidx[j] * 3 + k] : 0.0;
+    for (int k = 0; k < 3; k++) t64[j][k] = ok ? xt[q * 3 + k] : 0.0;  // xt in the order tperm
 #pragma unroll
     for (int k = 0; k < K1; k++) acc[j][k] = comp[j][k] = 0.0;
   }
@@ -487,7 +497,7 @@ k_reduce_splits_perm(const double* __restrict__ part, int64_t nt, int K1, int ns
 using EvalMixFn = void (*)(const double*, int64_t, const uint32_t*, const float4*, const double4*, const float4*, int64_t,
                            int64_t, double*, int, double, double);
 using PackFn = void (*)(const double*, const double*, const double*, int64_t, const uint32_t*, float4*, double4*,
-                        float4*);
+                        float4*, int64_t, const double*, int64_t, const uint32_t*, double*);
 
 constexpr int TPLS[3] = {1, 2, 4};
 int tpl_index(int tpl) { return tpl >= 4 ? 2 : (tpl >= 2 ? 1 : 0); }
@@ -547,7 +557,8 @@ int mixed_geometry(int ker, int64_t nt, int64_t ns, const DenseShape& shape, Mix
   int K0, K1, ND;
   kernel_dims(ker, &K0, &K1, &ND);
   g->packed_bytes = sizeof(float4) * g->Q * TILE * g->ntiles;
-  g->meta_bytes = (sizeof(double4) + sizeof(float4) * (WG / 64)) * g->ntiles;  // tile centres, then sub-tiles
+  // tile centres, then sub-tiles, then the targets in their order
+  g->meta_bytes = (sizeof(double4) + sizeof(float4) * (WG / 64)) * g->ntiles + sizeof(double) * 3 * nt;
   const size_t split_bytes = sizeof(double) * (size_t)nt * K1;
   g->group = split_group(g->nsplit, split_bytes, part_budget);
   g->part_bytes = g->group > 1 ? split_bytes * g->group : 0;
@@ -574,10 +585,13 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
   SCTL_CHECK_ARG(g.group <= 1 || part, "no buffer for the split partial sums");
   int K0, K1, ND;
   kernel_dims(ker, &K0, &K1, &ND);
-  // The sub-tile records follow the tile centres in the same buffer.
+  // The sub-tile records follow the tile centres in the same buffer, then
+  // the targets in the order tperm (written by the pack launch's last
+  // workgroups), which the pairwise kernel reads once per split.
   float4* sub = reinterpret_cast<float4*>(meta + g.ntiles);
-  hipLaunchKernelGGL(pack_fn(ker), dim3((unsigned)g.ntiles), dim3(WG), 0, stream, r_src, n_src, v_src, ns, sperm,
-                     packed, meta, sub);
+  double* xt_sorted = reinterpret_cast<double*>(sub + g.ntiles * (WG / 64));
+  hipLaunchKernelGGL(pack_fn(ker), dim3((unsigned)(g.ntiles + (nt + WG - 1) / WG)), dim3(WG), 0, stream, r_src, n_src,
+                     v_src, ns, sperm, packed, meta, sub, g.ntiles, r_trg, nt, tperm, xt_sorted);
   SCTL_HIP_TRY(hipGetLastError());
   const double scale = kernel_scale(ker) * fold_factor(ker, newton ? 1 : 0);
   const EvalMixFn fn = mix_fn(ker, newton, g.ti);
@@ -591,7 +605,7 @@ int mixed_apply(int ker, int newton, const MixedGeometry& g, const DenseShape& s
     const bool direct = gy == 1;
     {
       ProfileScope prof(stream);
-      hipLaunchKernelGGL(fn, split_grid(g.tb, gy, WG), dim3(WG), dyn, stream, r_trg, nt, tperm,
+      hipLaunchKernelGGL(fn, split_grid(g.tb, gy, WG), dim3(WG), dyn, stream, xt_sorted, nt, tperm,
                          packed + tile0 * (int64_t)(TILE * g.Q), meta + tile0, sub + tile0 * (WG / 64), ntiles, g.tps,
                          direct ? v_trg : part,
                          direct ? 1 : 0, scale, ctx);
