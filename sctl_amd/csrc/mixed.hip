@@ -266,12 +266,10 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
   const int64_t q0 = wg.block * (WG * TPL) + tid;
 
   double t64[TPL][3], acc[TPL][K1], comp[TPL][K1];
-  int64_t tidx[TPL];
 #pragma unroll
   for (int j = 0; j < TPL; j++) {
     const int64_t q = q0 + (int64_t)j * WG;
     const bool ok = q < nt;
-    tidx[j] = ok ? (int64_t)tperm[q] : -1;
 #pragma unroll
     for (int k = 0; k < 3; k++) t64[j][k] = ok ? xt[q * 3 + k] : 0.0;  // xt in the order tperm
 #pragma unroll
@@ -471,10 +469,13 @@ k_eval_mixed(const double* __restrict__ xt, int64_t nt, const uint32_t* __restri
   for (int j = 0; j < TPL; j++) {
     const int64_t q = q0 + (int64_t)j * WG;
     if (q < nt) {
+      // the target's own index is read only here, so it holds no register
+      // through the tile loop
+      const int64_t t = direct ? (int64_t)tperm[q] : 0;
 #pragma unroll
       for (int k = 0; k < K1; k++) {
         const double v = acc[j][k] - comp[j][k];
-        if (direct) out[tidx[j] * K1 + k] += v * scale;
+        if (direct) out[t * K1 + k] += v * scale;
         else out[wg.split * nt * K1 + q * K1 + k] = v;
       }
     }
